@@ -1,0 +1,199 @@
+"""Throughput of the GFDL microphysics forward on one MI355X.
+
+A step is one Microphysics call (pySHiELD/stencils/microphysics.py:1981-2263)
+over the whole workload, launched through the C ABI (shield_mp_forward), with
+the state resident in HBM and the cloud-fraction accumulator carried from
+call to call.  Tendencies are written, not accumulated, which is the
+Physics driver's use (prepare_microphysics zeroes them before the call,
+physics.py:133-143); the zeroing is fused into the kernel.
+
+Default workload: C384 (6 x 384 x 384 columns x 79 levels, dt 150 s),
+mixed precision.  The JSON line also carries the plain FP64 kernel's rate on
+the same state, the dominant kernel's HBM roofline fraction (algorithmic
+bytes / HIP-event kernel time / 8 TB/s) and the CPU baseline (the C oracle on
+the host cores, on a bounded sample of the same workload).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_CELL = 8 * (13 + 11)   # FP64 fields read (12 state + qaz) and written (10 tendencies + qaz)
+B_COL = 8 * (2 + 4)      # land, area in; 4 surface precipitation fields out
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c384")
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-compare", action="store_true", help="skip the second precision")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--only-kernel", action="store_true",
+                    help="launch only the timed kernel (for profiler runs)")
+    return ap.parse_args()
+
+
+def load_state(name, device):
+    import torch
+    from pyshield_amd.synthetic import WORKLOADS, make_state, WORKLOAD_IDS
+    w = WORKLOADS[name]
+    ni, nj, nz = w.ni * w.ntiles, w.nj, w.nz
+    names3 = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz", "w")
+    f = {n: torch.empty((ni, nj, nz), dtype=torch.float64, device=device) for n in names3}
+    for n in ("land", "area"):
+        f[n] = torch.empty((ni, nj), dtype=torch.float64, device=device)
+    for t in range(w.ntiles):
+        off = (-10.0 + 4.0 * t) if w.tile_offsets else 0.0
+        st = make_state(w.ni, w.nj, w.nz, 1000 * WORKLOAD_IDS[name] + t, ptop=w.ptop, t_offset=off,
+                        cloud_top=w.cloud_top, median=w.median, qg_median=w.qg_median)
+        sl = slice(t * w.ni, (t + 1) * w.ni)
+        for n in names3 + ("land", "area"):
+            f[n][sl].copy_(torch.from_numpy(st[n]))
+        del st
+    for n in ("pt_dt", "qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa_dt", "udt", "vdt", "qaz"):
+        f[n] = torch.zeros((ni, nj, nz), dtype=torch.float64, device=device)
+    for n in ("rain", "snow", "ice", "graupel"):
+        f[n] = torch.zeros((ni, nj), dtype=torch.float64, device=device)
+    return w, f
+
+
+def time_mode(f, params, mode, steps, warmup):
+    import torch
+    from pyshield_amd.microphysics import forward
+    for _ in range(warmup):
+        forward(f, params, mode)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        forward(f, params, mode)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    return wall, kernel_ms
+
+
+def cpu_baseline(name, seconds):
+    """The C oracle (OpenMP over columns) on tile 0 of the workload, repeated
+    until `seconds` of wall time have passed."""
+    import oracle
+    from pyshield_amd._config import PhysicsConfig
+    from pyshield_amd.synthetic import WORKLOADS, WORKLOAD_IDS, make_state
+    w = WORKLOADS[name]
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # a bounded sample: at most 96 x 384 columns of tile 0
+    ni = min(w.ni, 96)
+    st = make_state(ni, w.nj, w.nz, 1000 * WORKLOAD_IDS[name], ptop=w.ptop, cloud_top=w.cloud_top,
+                    median=w.median, qg_median=w.qg_median)
+    cfg = PhysicsConfig(dt_atmos=int(w.timestep))
+    nml = cfg.mp_namelist()
+    oracle.run(nml, w.timestep, st, nthreads=nthreads)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    times = []
+    while True:
+        t1 = time.perf_counter()
+        oracle.run(nml, w.timestep, st, nthreads=nthreads)
+        times.append(time.perf_counter() - t1)
+        reps += 1
+        if time.perf_counter() - t0 > seconds and reps >= 5:
+            break
+    ncol = ni * w.nj
+    med = float(np.median(times))
+    return {"value": ncol / med, "unit": "columns/s", "cores": nthreads, "kind": "port",
+            "sample": f"{ni}x{w.nj} columns x {w.nz} levels of tile 0 of {name}, dt {w.timestep:g} s, "
+                      f"FP64, median of {reps} calls"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the microphysics shards by columns with no "
+              f"exchange, so --gpus {args.gpus} is not run here", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from pyshield_amd import _lib
+    from pyshield_amd._config import PhysicsConfig
+
+    dev = torch.device("cuda:0")
+    w, f = load_state(args.workload, dev)
+    ncol = w.ni * w.nj * w.ntiles
+    cfg = PhysicsConfig(dt_atmos=int(w.timestep), npz=w.nz)
+    params = _lib.setup(cfg.mp_namelist(), w.timestep)
+    params.accumulate = 0
+    print(f"# workload {w.name}: {ncol} columns x {w.nz} levels, dt {w.timestep} s, "
+          f"layout {_lib.layout(w.nz)}", file=sys.stderr, flush=True)
+
+    wall, kms = time_mode(f, params, args.mode, args.steps, args.warmup)
+    ms_per_step = 1e3 * wall / args.steps
+    value = ncol / (wall / args.steps)
+    k_ms = float(np.mean(kms))
+    alg_bytes = ncol * (w.nz * B_CELL + B_COL)
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    print(f"# {args.mode}: {ms_per_step:.3f} ms/step, kernel {k_ms:.3f} ms, {achieved:.0f} GB/s",
+          file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_compare and not args.only_kernel:
+        other = "fp64" if args.mode == "mixed" else "mixed"
+        wall2, kms2 = time_mode(f, params, other, max(2, args.steps // 2), 1)
+        k2 = float(np.mean(kms2))
+        extra = {f"{other}_columns_per_s": ncol / (wall2 / max(2, args.steps // 2)),
+                 f"{other}_kernel_ms": k2,
+                 "mixed_over_fp64": (k2 / k_ms) if args.mode == "mixed" else (k_ms / k2)}
+        print(f"# {other}: kernel {k2:.3f} ms", file=sys.stderr, flush=True)
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"mp_forward_{w.name}_{args.mode}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline and not args.only_kernel:
+        del f
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(args.workload, args.cpu_seconds)
+
+    line = {
+        "metric": "column-level updates/s, fwd+adjoint, mixed vs FP64 on 1 MI355X at C384×79",
+        "value": value,
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.mode == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.ntiles}x{w.ni}x{w.nj} columns x {w.nz} levels, "
+                               f"dt {w.timestep:g} s, {args.mode} forward",
+                   "columns": ncol, "levels": w.nz, "timestep": w.timestep, "mode": args.mode},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "kernel_ms": k_ms,
+        "algorithmic_bytes_per_launch": alg_bytes,
+        "adjoint_columns_per_s": None,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * shield_mp.h -- C ABI of the MI355X GFDL cloud-microphysics column physics.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are
+ * caller-owned FP64 buffers; launches are ordered on the HIP stream passed as
+ * `stream` (a hipStream_t, NULL = default stream).  No entry point allocates
+ * device memory.  Errors are returned as negative codes; shield_strerror()
+ * gives the text.
+ *
+ * Field layout: a 3D field is addressed as base[i*si + j*sj + k] for
+ * 0 <= i < ni, 0 <= j < nj, 0 <= k < nk (k stride 1, k = 0 is the model top,
+ * k = nk-1 the surface layer), which is the C-ordered [X, Y, Z] storage of the
+ * reference's Quantity (halo offsets folded into the base pointer).  A 2D
+ * field is base[i*si2 + j*sj2].
+ */
+#ifndef SHIELD_MP_H
+#define SHIELD_MP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+enum shield_status {
+  SHIELD_OK = 0,
+  SHIELD_ERR_NULL = -1,        /* a required pointer is NULL                      */
+  SHIELD_ERR_SHAPE = -2,       /* ni, nj, nk or strides out of range              */
+  SHIELD_ERR_UNSUPPORTED = -3, /* use_ppm or prog_ccn requested (broken upstream)  */
+  SHIELD_ERR_MODE = -4,        /* unknown precision mode                          */
+  SHIELD_ERR_ALIAS = -5,       /* an output aliases another field                 */
+  SHIELD_ERR_LAUNCH = -6,      /* HIP launch or runtime failure                   */
+  SHIELD_ERR_TIMESTEP = -7,    /* timestep <= 0 or not finite                     */
+  SHIELD_ERR_NOGPU = -8        /* no HIP device visible                           */
+};
+
+/* Precision modes of shield_mp_forward. */
+enum shield_mp_mode {
+  SHIELD_MP_FP64 = 0,  /* every operation in FP64 (the reference's arithmetic)   */
+  SHIELD_MP_MIXED = 1  /* FP32 process rates; FP64 state, transfers, column
+                          sedimentation/flux sums and precipitation             */
+};
+
+/* ---------------------------------------------------------------- config */
+/* Microphysics fields of PhysicsConfig (pySHiELD/_config.py:21-109).        */
+typedef struct shield_mp_namelist {
+  double c_cracw, c_paut, c_pgacs, c_psaci, ccn_l, ccn_o;
+  double vs_fac, vg_fac, vi_fac, vr_fac;
+  double tau_imlt, tau_i2s, tau_g2v, tau_v2g, tau_l2v;
+  double ql_mlt, qs_mlt, t_sub, qi_gen, qi_lim, qi0_max;
+  double dw_ocean, dw_land, rh_inc, rh_inr;
+  double qc_crt, qi0_crt, qs0_crt, rthresh, mp_time;
+  double vg_max, vi_max, vr_max, vs_max;
+  double tice, alin, clin;
+  int32_t hydrostatic, do_qa, const_vg, const_vi, const_vr, const_vs, de_ice;
+  int32_t rad_snow, rad_rain, do_sedi_heat, do_sedi_w, fast_sat_adj;
+  int32_t fix_negative, irain_f, prog_ccn, sedi_transport, use_ppm;
+  int32_t z_slope_ice, z_slope_liq;
+  /* Extension, not in the reference: 1 = zero the cloud-fraction accumulator
+   * qaz at the start of every call (GFDL Fortran behaviour).  0 = keep it
+   * across calls exactly as pySHiELD/stencils/microphysics.py:1674,2156 does. */
+  int32_t reset_qaz;
+} shield_mp_namelist;
+
+/* Derived scalars: Microphysics.__init__ (microphysics.py:1643-1659,1712-1713),
+ * gfdl_cloud_microphys_init/setupm (:1806-1954) and _set_timestep
+ * (:1964-1979).  Filled by shield_mp_setup(); passed by value to kernels.   */
+typedef struct shield_mp_params {
+  shield_mp_namelist nml;
+  double c_air, c_vap, d0_vap, lv00, cpaut, so3, zs, log_10, tice0, t_wfr;
+  double fac_rc, cracs, csacr, cgacr, cgacs;
+  double acco[3][4];
+  double csacw, csaci, cgacw, cgaci, cracw;
+  double cssub[5], cgsub[5], crevp[5], cgfr[2], csmlt[5], cgmlt[5];
+  double ces0;
+  double timestep, mpdt, rdt, dts, dt_rain, rdts, dt_evap;
+  double fac_i2s, fac_g2v, fac_v2g, fac_imlt, fac_l2v;
+  int32_t ntimes;
+  int32_t use_ccn;   /* = !prog_ccn                       (microphysics.py:1658) */
+  int32_t do_sedi_w; /* = do_sedi_w && !hydrostatic       (microphysics.py:1655) */
+  /* 1 (set by shield_mp_setup): tendencies are accumulated with += as
+   * Microphysics.__call__ does.  0: they are overwritten, which equals
+   * zeroing them first as Physics.__call__ does with prepare_microphysics
+   * (physics.py:133-143) and saves reading them.                           */
+  int32_t accumulate;
+} shield_mp_params;
+
+/* ----------------------------------------------------------------- shape */
+typedef struct shield_mp_shape {
+  int64_t ni, nj, nk; /* compute domain: ni*nj columns of nk levels          */
+  int64_t si, sj;     /* element strides of i and j for 3D fields (k stride 1) */
+  int64_t si2, sj2;   /* element strides of i and j for 2D fields            */
+} shield_mp_shape;
+
+/* ---------------------------------------------------------------- fields */
+/* MicrophysicsState (microphysics.py:1556-1620) plus the driver's persistent
+ * buffers.  Names follow the reference; the reference passes delprsi as
+ * `delp`, delz as `dz` and wmp as `w` (microphysics.py:2003-2004,2058).     */
+typedef struct shield_mp_fields {
+  /* state in (3D); qi and qs are rewritten in place when de_ice, as the
+   * reference's fields_init does (microphysics.py:110-120)                  */
+  const double *pt, *qv, *ql, *qr;
+  double *qi, *qs;
+  const double *qg, *qa, *ua, *va, *delp, *delz;
+  double *w;                  /* vertical velocity; in/out when do_sedi_w     */
+  const double *land, *area;  /* 2D land mask and cell area                   */
+  /* tendencies, accumulated with += (microphysics.py:1510-1538);
+   * qi_dt and qs_dt are also read first when de_ice (:110-120)              */
+  double *pt_dt, *qv_dt, *ql_dt, *qr_dt, *qi_dt, *qs_dt, *qg_dt, *qa_dt;
+  double *udt, *vdt;
+  double *qaz;                /* 3D cloud-fraction accumulator, in/out        */
+  /* 2D surface precipitation of this call in mm/day (the reference keeps it
+   * in private 3D accumulators, microphysics.py:1547-1553,1664-1667); any of
+   * the four may be NULL                                                    */
+  double *rain, *snow, *ice, *graupel;
+} shield_mp_fields;
+
+/* --------------------------------------------------------- entry points */
+
+/* Fill *out from the namelist for the given timestep: the host-side part of
+ * Microphysics.__init__ (microphysics.py:1624-1811).  Pure host code.       */
+int shield_mp_setup(const shield_mp_namelist* nml, double timestep,
+                    shield_mp_params* out);
+
+/* Recompute the timestep-dependent scalars in place when `timestep` differs
+ * from p->timestep: Microphysics._update_timestep_if_needed
+ * (microphysics.py:1959-1962).  Pure host code.                             */
+int shield_mp_update_timestep(shield_mp_params* p, double timestep);
+
+/* One Microphysics.__call__(state, timestep) (microphysics.py:1981-2263) on
+ * the GPU: fields_init, ntimes x [warm_rain, sedimentation, warm_rain,
+ * icloud], fields_update, all in one fused kernel.  `mode` is a
+ * shield_mp_mode.                                                           */
+int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p,
+                      const shield_mp_fields* f, int mode, void* stream);
+
+/* ------------------------------------------------- state-update path */
+
+/* prepare_microphysics (pySHiELD/stencils/physics.py:105-143):
+ * dz = (phii[k+1]-phii[k])/g, wmp = -omga(1+zvir qv) pt/delp Rd/g, and the
+ * ten tendencies zeroed.  phii has nk+1 levels (interfaces) with the same
+ * i/j strides as the 3D fields.  tend[10] = {udt, vdt, pt_dt, qv_dt, ql_dt,
+ * qr_dt, qi_dt, qs_dt, qg_dt, qa_dt}.                                       */
+int shield_prepare_microphysics(const shield_mp_shape* shape, double* dz,
+                                const double* phii, double* wmp,
+                                const double* omga, const double* qvapor,
+                                const double* pt, const double* delp,
+                                double* const tend[10], void* stream);
+
+/* update_physics_state_with_tendencies (physics.py:146-194): for each of the
+ * ten fields out[n] = in[n] + tend[n]*dt, in the order qvapor, qliquid,
+ * qrain, qice, qsnow, qgraupel, qcld, pt, ua, va.                           */
+int shield_update_physics_state_with_tendencies(const shield_mp_shape* shape,
+                                                const double* const in[10],
+                                                const double* const tend[10],
+                                                double* const out[10], double dt,
+                                                void* stream);
+
+/* fill_gfs_delp (pySHiELD/update/update_atmos_state.py:21-37): vertical
+ * filling of q below q_min, in place, over the nk layers of the reference's
+ * nk+1-level domain.                                                         */
+int shield_fill_gfs_delp(const shield_mp_shape* shape, const double* delp,
+                         double* q, double q_min, void* stream);
+
+/* prepare_tendencies_and_update_tracers (update_atmos_state.py:40-92) over
+ * all nk layers (the reference's interval(0, -1) on an nk+1-level domain):
+ * u_dt, v_dt, pt_dt += (x_t1 - x_t0) * rdt; the GFS mixing ratios *_t1 are
+ * converted to FV3 ones in *_t0 and delp is replaced by the new total mass.
+ * prsi has nk+1 interface levels at the same i/j strides.                  */
+int shield_prepare_tendencies_and_update_tracers(
+    const shield_mp_shape* shape, double* u_dt, double* v_dt, double* pt_dt,
+    const double* u_t1, const double* v_t1, const double* pt_t1,
+    const double* qv_t1, const double* ql_t1, const double* qr_t1,
+    const double* qs_t1, const double* qi_t1, const double* qg_t1,
+    const double* u_t0, const double* v_t0, const double* pt_t0,
+    double* qv_t0, double* ql_t0, double* qr_t0, double* qs_t0, double* qi_t0,
+    double* qg_t0, const double* prsi, double* delp, double rdt, void* stream);
+
+/* ------------------------------------------------------------ utilities */
+const char* shield_strerror(int code);
+/* Number of HIP devices visible (0 without a GPU; never fails).            */
+int shield_device_count(void);
+/* Columns per wavefront and slots per lane the forward kernel picks for nk. */
+int shield_mp_layout(int64_t nk, int* cols_per_wave, int* slots);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHIELD_MP_H */

@@ -1,0 +1,16 @@
+"""pyshield_amd -- MI355X-native GFDL cloud microphysics column physics.
+
+Drop-in for pySHiELD's microphysics path (``Microphysics(state, timestep)``)
+and the column-local state-update stencils around it, on hand-written HIP
+kernels for gfx950 behind the C ABI in include/shield_mp.h.
+"""
+
+from ._config import PHYSICS_PACKAGES, PhysicsConfig
+from .microphysics import (GridData, Microphysics, MicrophysicsState, QuantityFactory,
+                           StencilFactory, forward)
+
+__all__ = [
+    "PHYSICS_PACKAGES", "PhysicsConfig", "Microphysics", "MicrophysicsState",
+    "StencilFactory", "QuantityFactory", "GridData", "forward",
+]
+__version__ = "0.2.0"

@@ -1,0 +1,155 @@
+"""ctypes binding of libshield_mp.so (the C ABI in include/shield_mp.h).
+
+The library is built in-tree by ``pyshield_amd.build``.  Nothing here falls
+back to another implementation: if the library is missing or no GPU is
+visible, the entry points raise.
+"""
+
+import ctypes
+import os
+
+from ._config import MP_NAMELIST_FLOATS, MP_NAMELIST_INTS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libshield_mp.so")
+
+SHIELD_MP_FP64 = 0
+SHIELD_MP_MIXED = 1
+MODES = {"fp64": SHIELD_MP_FP64, "mixed": SHIELD_MP_MIXED}
+
+# every symbol the header declares
+EXPORTS = (
+    "shield_mp_setup", "shield_mp_update_timestep", "shield_mp_forward",
+    "shield_prepare_microphysics", "shield_update_physics_state_with_tendencies",
+    "shield_fill_gfs_delp", "shield_prepare_tendencies_and_update_tracers",
+    "shield_strerror", "shield_device_count", "shield_mp_layout",
+)
+
+_D = ctypes.c_double
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_DP = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.c_void_p
+
+
+class ShieldMpNamelist(ctypes.Structure):
+    _fields_ = ([(n, _D) for n in MP_NAMELIST_FLOATS]
+                + [(n, _I32) for n in MP_NAMELIST_INTS] + [("reset_qaz", _I32)])
+
+
+class ShieldMpParams(ctypes.Structure):
+    _fields_ = [
+        ("nml", ShieldMpNamelist),
+        *[(n, _D) for n in ("c_air", "c_vap", "d0_vap", "lv00", "cpaut", "so3", "zs", "log_10",
+                            "tice0", "t_wfr", "fac_rc", "cracs", "csacr", "cgacr", "cgacs")],
+        ("acco", (_D * 4) * 3),
+        *[(n, _D) for n in ("csacw", "csaci", "cgacw", "cgaci", "cracw")],
+        ("cssub", _D * 5), ("cgsub", _D * 5), ("crevp", _D * 5), ("cgfr", _D * 2),
+        ("csmlt", _D * 5), ("cgmlt", _D * 5),
+        ("ces0", _D),
+        *[(n, _D) for n in ("timestep", "mpdt", "rdt", "dts", "dt_rain", "rdts", "dt_evap",
+                            "fac_i2s", "fac_g2v", "fac_v2g", "fac_imlt", "fac_l2v")],
+        ("ntimes", _I32), ("use_ccn", _I32), ("do_sedi_w", _I32), ("accumulate", _I32),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            if name in ("nml", "accumulate"):
+                continue
+            v = getattr(self, name)
+            if name == "acco":
+                for i in range(3):
+                    for k in range(4):
+                        out[f"acco{i}{k}"] = v[i][k]
+            elif hasattr(v, "__len__"):
+                for i, x in enumerate(v):
+                    out[f"{name}{i}"] = x
+            else:
+                out[name] = v
+        return out
+
+
+class ShieldMpShape(ctypes.Structure):
+    _fields_ = [(n, _I64) for n in ("ni", "nj", "nk", "si", "sj", "si2", "sj2")]
+
+
+FIELD_NAMES = (
+    "pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz", "w",
+    "land", "area",
+    "pt_dt", "qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa_dt", "udt", "vdt",
+    "qaz", "rain", "snow", "ice", "graupel",
+)
+
+
+class ShieldMpFields(ctypes.Structure):
+    _fields_ = [(n, _VP) for n in FIELD_NAMES]
+
+
+_lib = None
+
+
+class ShieldError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libshield_mp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ShieldError(
+                f"{LIB_PATH} is missing: build it with `python -m pyshield_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.shield_mp_setup.argtypes = [ctypes.POINTER(ShieldMpNamelist), _D, ctypes.POINTER(ShieldMpParams)]
+        L.shield_mp_update_timestep.argtypes = [ctypes.POINTER(ShieldMpParams), _D]
+        L.shield_mp_forward.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldMpParams),
+                                        ctypes.POINTER(ShieldMpFields), ctypes.c_int, _VP]
+        L.shield_prepare_microphysics.argtypes = [ctypes.POINTER(ShieldMpShape), _VP, _VP, _VP, _VP, _VP,
+                                                  _VP, _VP, _VP * 10, _VP]
+        L.shield_update_physics_state_with_tendencies.argtypes = [ctypes.POINTER(ShieldMpShape), _VP * 10,
+                                                                  _VP * 10, _VP * 10, _D, _VP]
+        L.shield_fill_gfs_delp.argtypes = [ctypes.POINTER(ShieldMpShape), _VP, _VP, _D, _VP]
+        L.shield_prepare_tendencies_and_update_tracers.argtypes = (
+            [ctypes.POINTER(ShieldMpShape)] + [_VP] * 23 + [_D, _VP])
+        L.shield_strerror.argtypes = [ctypes.c_int]
+        L.shield_strerror.restype = ctypes.c_char_p
+        L.shield_mp_layout.argtypes = [_I64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        for fn in EXPORTS:
+            if fn != "shield_strerror":
+                getattr(L, fn).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().shield_strerror(rc).decode()
+        raise ShieldError(f"{what} failed ({rc}): {msg}")
+
+
+def make_namelist(values: dict) -> ShieldMpNamelist:
+    s = ShieldMpNamelist()
+    for n in MP_NAMELIST_FLOATS:
+        setattr(s, n, float(values[n]))
+    for n in MP_NAMELIST_INTS:
+        setattr(s, n, int(values[n]))
+    s.reset_qaz = int(values.get("reset_qaz", 0))
+    return s
+
+
+def setup(values: dict, timestep: float) -> ShieldMpParams:
+    p = ShieldMpParams()
+    check(lib().shield_mp_setup(ctypes.byref(make_namelist(values)), float(timestep), ctypes.byref(p)),
+          "shield_mp_setup")
+    return p
+
+
+def layout(nk):
+    P, S = ctypes.c_int(), ctypes.c_int()
+    check(lib().shield_mp_layout(int(nk), ctypes.byref(P), ctypes.byref(S)), "shield_mp_layout")
+    return P.value, S.value
+
+
+def device_count():
+    return lib().shield_device_count()

@@ -1,0 +1,137 @@
+// mp_common.h -- constants, coefficient block and precision-generic math for
+// the GFDL microphysics kernels (gfx950).
+//
+// Precision model: the prognostic state (six water species, temperature, the
+// cloud-fraction accumulator) is always FP64 in registers; every process rate
+// is computed in the "rate type" R (double for SHIELD_MP_FP64, float for
+// SHIELD_MP_MIXED) and applied to the FP64 state as a paired FP64 transfer, so
+// total water stays conserved to FP64 rounding in both modes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/shield_mp.h"
+
+namespace shield {
+
+// ndsl.constants, GFS set
+constexpr double GRAV = 9.80665;
+constexpr double RGRAV = 1.0 / GRAV;
+constexpr double RDGAS = 287.05;
+constexpr double RVGAS = 461.50;
+constexpr double HLV = 2.5e6;
+constexpr double HLF = 3.3358e5;
+constexpr double CP_AIR = 1004.6;
+constexpr double TICE = 273.16;
+constexpr double C_LIQ = 4185.5;
+constexpr double C_ICE = 1972.0;
+constexpr double E00 = 611.21;
+constexpr double T_MIN = 178.0;
+constexpr double CV_AIR = CP_AIR - RDGAS;
+constexpr double CV_VAP = 3.0 * RVGAS;
+constexpr double CP_VAP = 4.0 * RVGAS;
+constexpr double DC_VAP = CP_VAP - C_LIQ;
+constexpr double DC_ICE = C_LIQ - C_ICE;
+constexpr double D2ICE = DC_VAP + DC_ICE;
+constexpr double LV0 = HLV - DC_VAP * TICE;
+constexpr double LI00 = HLF - DC_ICE * TICE;
+constexpr double LI2 = LV0 + LI00;
+constexpr double T_SAT_MIN = TICE - 160.0;
+constexpr double LAT2 = (HLV + HLF) * (HLV + HLF);
+
+// microphysics_funcs.py:7-42
+constexpr double VCONS = 6.6280504;
+constexpr double VCONG = 87.2382675;
+constexpr double NORMS = 942477796.076938;
+constexpr double NORMG = 5026548245.74367;
+constexpr double VCONR = 2503.23638966667;
+constexpr double NORMR = 25132741228.7183;
+constexpr double THR = 1.0e-8;
+constexpr double THI = 1.0e-8;
+constexpr double THG = 1.0e-8;
+constexpr double THS = 1.0e-8;
+constexpr double AA = -4.14122e-5;
+constexpr double BB = -0.00538922;
+constexpr double CC = -0.0516344;
+constexpr double DD_FS = 0.00216078;
+constexpr double EE = 1.9714;
+constexpr double VR_MIN = 1.0e-3;
+constexpr double VF_MIN = 1.0e-5;
+constexpr double P_MIN = 100.0;
+constexpr double DT_FR = 8.0;
+constexpr double SFCRHO = 1.2;
+constexpr double RHOR = 1.0e3;
+constexpr double QCMIN = 1.0e-12;
+constexpr double QRMIN = 1.0e-8;
+constexpr double QVMIN = 1.0e-20;
+
+// ---------------------------------------------------------------- math
+__device__ __forceinline__ double mexp(double x) { return exp(x); }
+__device__ __forceinline__ double mlog(double x) { return log(x); }
+__device__ __forceinline__ double msqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double mpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ double mabs(double x) { return fabs(x); }
+__device__ __forceinline__ float mexp(float x) { return __expf(x); }
+__device__ __forceinline__ float mlog(float x) { return __logf(x); }
+__device__ __forceinline__ float msqrt(float x) { return __fsqrt_rn(x); }
+__device__ __forceinline__ float mpow(float x, float y) { return __expf(y * __logf(x)); }
+__device__ __forceinline__ float mabs(float x) { return fabsf(x); }
+template <class T>
+__device__ __forceinline__ T mmin(T a, T b) { return a < b ? a : b; }
+template <class T>
+__device__ __forceinline__ T mmax(T a, T b) { return a > b ? a : b; }
+// microphysics_funcs.py:45-49
+template <class T>
+__device__ __forceinline__ T dim(T x, T y) {
+  T diff = x - y;
+  return diff > T(0) ? diff : T(0);
+}
+
+// --------------------------------------------------- coefficient block
+// The scalars the kernels read, in the rate type R.  Filled on the host from
+// shield_mp_params (which is built by shield_mp_setup).
+template <class R>
+struct Coef {
+  R c_air, c_vap, d0_vap, lv00, so3, log_10, tice, tice0, t_wfr, t_sub;
+  R fac_rc, cracs, csacr, cgacr, cgacs, csacw, csaci, cgacw, cgaci, cracw, ces0;
+  R acco[3][4];
+  R cssub[5], crevp[5], cgfr[2], csmlt[5], cgmlt[5];
+  R dts, rdts, dt_rain, fac_i2s, fac_g2v, fac_v2g, fac_imlt, fac_l2v;
+  R ql_mlt, qs_mlt, qi_gen, qi_lim, qi0_crt, qs0_crt, qc_crt;
+  R vs_fac, vg_fac, vi_fac, vr_fac, vs_max, vg_max, vi_max, vr_max;
+  int32_t const_vi, const_vs, const_vg, const_vr, fast_sat_adj, do_qa, rad_snow, rad_rain;
+  int32_t use_ccn, pad_;
+};
+
+template <class R>
+inline Coef<R> make_coef(const shield_mp_params& p) {
+  Coef<R> c;
+  const shield_mp_namelist& n = p.nml;
+  c.c_air = R(p.c_air); c.c_vap = R(p.c_vap); c.d0_vap = R(p.d0_vap); c.lv00 = R(p.lv00);
+  c.so3 = R(p.so3); c.log_10 = R(p.log_10); c.tice = R(n.tice); c.tice0 = R(p.tice0);
+  c.t_wfr = R(p.t_wfr); c.t_sub = R(n.t_sub);
+  c.fac_rc = R(p.fac_rc); c.cracs = R(p.cracs); c.csacr = R(p.csacr); c.cgacr = R(p.cgacr);
+  c.cgacs = R(p.cgacs); c.csacw = R(p.csacw); c.csaci = R(p.csaci); c.cgacw = R(p.cgacw);
+  c.cgaci = R(p.cgaci); c.cracw = R(p.cracw); c.ces0 = R(p.ces0);
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 4; ++k) c.acco[i][k] = R(p.acco[i][k]);
+  for (int i = 0; i < 5; ++i) {
+    c.cssub[i] = R(p.cssub[i]); c.crevp[i] = R(p.crevp[i]);
+    c.csmlt[i] = R(p.csmlt[i]); c.cgmlt[i] = R(p.cgmlt[i]);
+  }
+  c.cgfr[0] = R(p.cgfr[0]); c.cgfr[1] = R(p.cgfr[1]);
+  c.dts = R(p.dts); c.rdts = R(p.rdts); c.dt_rain = R(p.dt_rain);
+  c.fac_i2s = R(p.fac_i2s); c.fac_g2v = R(p.fac_g2v); c.fac_v2g = R(p.fac_v2g);
+  c.fac_imlt = R(p.fac_imlt); c.fac_l2v = R(p.fac_l2v);
+  c.ql_mlt = R(n.ql_mlt); c.qs_mlt = R(n.qs_mlt); c.qi_gen = R(n.qi_gen); c.qi_lim = R(n.qi_lim);
+  c.qi0_crt = R(n.qi0_crt); c.qs0_crt = R(n.qs0_crt); c.qc_crt = R(n.qc_crt);
+  c.vs_fac = R(n.vs_fac); c.vg_fac = R(n.vg_fac); c.vi_fac = R(n.vi_fac); c.vr_fac = R(n.vr_fac);
+  c.vs_max = R(n.vs_max); c.vg_max = R(n.vg_max); c.vi_max = R(n.vi_max); c.vr_max = R(n.vr_max);
+  c.const_vi = n.const_vi; c.const_vs = n.const_vs; c.const_vg = n.const_vg; c.const_vr = n.const_vr;
+  c.fast_sat_adj = n.fast_sat_adj; c.do_qa = n.do_qa; c.rad_snow = n.rad_snow; c.rad_rain = n.rad_rain;
+  c.use_ccn = p.use_ccn; c.pad_ = 0;
+  return c;
+}
+
+}  // namespace shield

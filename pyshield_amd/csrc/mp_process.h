@@ -1,0 +1,664 @@
+// mp_process.h -- point-wise GFDL microphysics processes for one cell.
+//
+// Each function follows the pySHiELD process library
+// (pySHiELD/functions/microphysics_funcs.py) statement by statement.  The
+// cell state `Cell` is FP64; rates are computed in the rate type R and
+// applied as FP64 transfers (s.q_a -= x; s.q_b += x).  With R = double every
+// expression has the reference's operation order.  With R = float, limiters
+// that bound a sink by the species it drains are evaluated against the FP64
+// state (`cap`), so no species is driven below zero by FP32 rounding.
+#pragma once
+
+#include "mp_common.h"
+
+namespace shield {
+
+struct Cell {
+  double qv, ql, qr, qi, qs, qg, tz, qa;
+};
+
+template <class R>
+struct is_f32 {
+  static constexpr bool value = false;
+};
+template <>
+struct is_f32<float> {
+  static constexpr bool value = true;
+};
+
+// Bound a transfer x out of species q (mixed precision only; identity in FP64
+// where the reference's arithmetic already guarantees x <= q).
+template <class R>
+__device__ __forceinline__ double cap(double x, double q) {
+  if constexpr (is_f32<R>::value) return x < q ? x : q;
+  return x;
+}
+
+#define SH_CVM(c, s, q_liq, q_sol) \
+  ((c).c_air + R((s).qv) * (c).c_vap + (q_liq) * R(C_LIQ) + (q_sol) * R(C_ICE))
+
+// ------------------------------------------------ saturation (f:53-145)
+template <class R>
+__device__ __forceinline__ R wqs1(R ta, R den) {
+  return (R(E00) * mexp((R(DC_VAP) * mlog(ta / R(TICE)) + R(LV0) * (ta - R(TICE)) / (ta * R(TICE))) /
+                        R(RVGAS))) /
+         (R(RVGAS) * ta * den);
+}
+template <class R>
+__device__ __forceinline__ R wqs2(R ta, R den, R& dqdt) {
+  R tmp = wqs1(ta, den);
+  dqdt = tmp * (R(DC_VAP) + R(LV0) / ta) / (R(RVGAS) * ta);
+  return tmp;
+}
+template <class R>
+__device__ __forceinline__ R iqs1(R ta, R den) {
+  R tmp;
+  if (ta < R(TICE)) {
+    if (ta >= R(T_SAT_MIN)) {
+      tmp = (R(E00) * mexp((R(D2ICE) * mlog(ta / R(TICE)) + R(LI2) * (ta - R(TICE)) / (ta * R(TICE))) /
+                           R(RVGAS))) /
+            (R(RVGAS) * ta * den);
+    } else {
+      tmp = (R(E00) * mexp((R(D2ICE) * mlog(R(1.0) - R(160.0) / R(TICE)) -
+                            R(LI2) * R(160.0) / (R(T_SAT_MIN) * R(TICE))) /
+                           R(RVGAS))) /
+            (R(RVGAS) * R(T_SAT_MIN) * den);
+    }
+  } else {
+    if (ta <= R(TICE) + R(102.0))
+      tmp = wqs1(ta, den);
+    else
+      tmp = wqs1(R(TICE) + R(102.0), den);
+  }
+  return tmp;
+}
+template <class R>
+__device__ __forceinline__ R iqs2(R ta, R den, R& dqdt) {
+  R tmp = iqs1(ta, den);
+  if (ta < R(TICE)) {
+    if (ta >= R(T_SAT_MIN))
+      dqdt = tmp * (R(D2ICE) + R(LI2) / ta) / (R(RVGAS) * ta);
+    else
+      dqdt = tmp * (R(D2ICE) + R(LI2) / R(T_SAT_MIN)) / (R(RVGAS) * R(T_SAT_MIN));
+  } else {
+    if (ta <= R(TICE) + R(102.0))
+      dqdt = tmp * (R(DC_VAP) + R(LV0) / ta) / (R(RVGAS) * ta);
+    else
+      dqdt = tmp * (R(DC_VAP) + R(LV0) / (R(TICE) + R(102.0))) / (R(RVGAS) * (R(TICE) + R(102.0)));
+  }
+  return tmp;
+}
+
+// ------------------------------------------------ f:149-179
+template <class R>
+__device__ __forceinline__ R acr3d(R v1, R v2, R q1, R q2, R c, R cac_ik, R cac_i1k, R cac_i2k, R rho) {
+  R t1 = msqrt(q1 * rho);
+  R s1 = msqrt(q2 * rho);
+  R s2 = msqrt(s1);
+  return c * mabs(v1 - v2) * q1 * s2 * (cac_ik * t1 + cac_i1k * msqrt(t1) * s2 + cac_i2k * s1);
+}
+template <class R>
+__device__ __forceinline__ R smlt(R tc, R dqs, R qsrho, R psacw, R psacr, const R* k, R rho, R rhofac) {
+  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qsrho) + k[3] * mpow(qsrho, R(0.65625)) * msqrt(rhofac)) +
+         k[4] * tc * (psacw + psacr);
+}
+template <class R>
+__device__ __forceinline__ R gmlt(R tc, R dqs, R qgrho, R pgacw, R pgacr, const R* k, R rho) {
+  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qgrho) + k[3] * mpow(qgrho, R(0.6875)) / mpow(rho, R(0.25))) +
+         k[4] * tc * (pgacw + pgacr);
+}
+
+// ------------------------------------------------ f:183-265
+template <class R>
+__device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, Cell& s, R den,
+                                           R denfac) {
+  if ((s.tz > t_wfr) && (s.qr > QRMIN)) {
+    R lhl = c.lv00 + c.d0_vap * R(s.tz);
+    R q_liq = R(s.ql) + R(s.qr);
+    R q_sol = R(s.qi) + R(s.qs) + R(s.qg);
+    R cvm = SH_CVM(c, s, q_liq, q_sol);
+    R lcpk = lhl / cvm;
+    R tin = R(s.tz) - lcpk * R(s.ql);
+    R qpz = R(s.qv) + R(s.ql);
+    R dqsdt;
+    R qsat = wqs2(tin, den, dqsdt);
+    R dqh = mmax(R(s.ql), h_var * mmax(qpz, R(QCMIN)));
+    dqh = mmin(dqh, R(0.2) * qpz);
+    R dqv = qsat - R(s.qv);
+    R q_minus = qpz - dqh;
+    R q_plus = qpz + dqh;
+    if ((dqv > R(QVMIN)) && (qsat > q_minus)) {
+      R dq;
+      if (qsat > q_plus)
+        dq = qsat - qpz;
+      else
+        dq = R(0.25) * ((q_minus - qsat) * (q_minus - qsat)) / dqh;
+      R qden = R(s.qr) * den;
+      R t2 = tin * tin;
+      R evap = c.crevp[0] * t2 * dq * (c.crevp[1] * msqrt(qden) + c.crevp[2] * mexp(R(0.725) * mlog(qden))) /
+               (c.crevp[3] * t2 + c.crevp[4] * qsat * den);
+      double ev = mmin(s.qr, double(mmin(dt * evap, dqv / (R(1.0) + lcpk * dqsdt))));
+      s.qr = s.qr - ev;
+      s.qv = s.qv + ev;
+      q_liq = q_liq - R(ev);
+      cvm = SH_CVM(c, s, q_liq, q_sol);
+      s.tz = s.tz - double(R(ev) * lhl / cvm);
+    }
+    if ((s.qr > QRMIN) && (s.ql > 1.0e-6) && (qsat < q_minus)) {
+      R sink = dt * denfac * c.cracw * mexp(R(0.95) * mlog(R(s.qr) * den));
+      double sk = cap<R>(double(sink / (R(1.0) + sink) * R(s.ql)), s.ql);
+      s.ql = s.ql - sk;
+      s.qr = s.qr + sk;
+    }
+  }
+}
+
+// ------------------------------------------------ f:269-341
+template <class R>
+__device__ __forceinline__ void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R tk, R den, R& vtg, R& vti,
+                                           R& vts) {
+  R rhof = msqrt(mmin(R(10.0), R(SFCRHO) / den));
+  if (c.const_vi) {
+    vti = c.vi_fac;
+  } else {
+    R vi0 = R(0.01) * c.vi_fac;
+    if (qi < R(THI)) {
+      vti = R(VF_MIN);
+    } else {
+      R tc = tk - c.tice;
+      R v = (R(3.0) + mlog(qi * den) / c.log_10) * (tc * (R(AA) * tc + R(BB)) + R(CC)) + R(DD_FS) * tc + R(EE);
+      v = vi0 * mexp(c.log_10 * v) * R(0.8);
+      vti = mmin(c.vi_max, mmax(R(VF_MIN), v));
+    }
+  }
+  if (c.const_vs) {
+    vts = c.vs_fac;
+  } else if (qs < R(THS)) {
+    vts = R(VF_MIN);
+  } else {
+    R v = c.vs_fac * R(VCONS) * rhof * mexp(R(0.0625) * mlog(qs * den / R(NORMS)));
+    vts = mmin(c.vs_max, mmax(R(VF_MIN), v));
+  }
+  if (c.const_vg) {
+    vtg = c.vg_fac;
+  } else if (qg < R(THG)) {
+    vtg = R(VF_MIN);
+  } else {
+    R v = c.vg_fac * R(VCONG) * rhof * msqrt(msqrt(msqrt(qg * den / R(NORMG))));
+    vtg = mmin(c.vg_max, mmax(R(VF_MIN), v));
+  }
+}
+
+// f:344-372 (the r1 half is handled by the caller)
+template <class R>
+__device__ __forceinline__ R rain_fspeed(const Coef<R>& c, bool no_fall, R qrz, R den) {
+  if (no_fall) return R(VF_MIN);
+  if (c.const_vr) return c.vr_fac;
+  R qden = qrz * den;
+  if (qrz < R(THR)) return R(VR_MIN);
+  R v = c.vr_fac * R(VCONR) * msqrt(mmin(R(10.0), R(SFCRHO) / den)) * mexp(R(0.2) * mlog(qden / R(NORMR)));
+  return mmin(c.vr_max, mmax(R(VR_MIN), v));
+}
+
+// ------------------------------------------------ f:375-428
+template <class R>
+__device__ __forceinline__ void autoconv_no_subgrid_var(const Coef<R>& c, double t_wfr, Cell& s, R den, R ccn,
+                                                        R c_praut) {
+  R qc0 = c.fac_rc * ccn;
+  if (s.tz > t_wfr) {
+    R qc = c.use_ccn ? qc0 : qc0 / den;
+    R dq = R(s.ql) - qc;
+    if (dq > R(0.0)) {
+      double sink = mmin(double(dq), double(c.dt_rain * c_praut * den * mexp(c.so3 * mlog(R(s.ql)))));
+      sink = cap<R>(sink, s.ql);
+      s.ql = s.ql - sink;
+      s.qr = s.qr + sink;
+    }
+  }
+}
+template <class R>
+__device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_wfr, Cell& s, R den, R ccn,
+                                                     R c_praut, R dl) {
+  R qc0 = c.fac_rc * ccn;
+  if (s.tz > t_wfr + DT_FR) {
+    dl = mmin(mmax(R(1.0e-6), dl), R(0.5) * R(s.ql));
+    R qc = c.use_ccn ? qc0 : qc0 / den;
+    R dq = R(0.5) * (R(s.ql) + dl - qc);
+    if (dq > R(0.0)) {
+      R sink = mmin(R(1.0), dq / dl) * c.dt_rain * c_praut * den * mexp(c.so3 * mlog(R(s.ql)));
+      double sk = cap<R>(double(sink), s.ql);
+      s.ql = s.ql - sk;
+      s.qr = s.qr + sk;
+    }
+  }
+}
+
+// ------------------------------------------------ f:431-850
+template <class R>
+__device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, Cell& s, R h_var, R rh_adj,
+                                               R rh_rain, R den, R denfac, double p1) {
+  const R tice = c.tice, t_wfr = c.t_wfr, t_sub = c.t_sub, dts = c.dts;
+  R lhl = c.lv00 + c.d0_vap * R(s.tz);
+  R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  R q_liq = R(s.ql) + R(s.qr);
+  R q_sol = R(s.qi) + R(s.qs) + R(s.qg);
+  R cvm = SH_CVM(c, s, q_liq, q_sol);
+  R lcpk = lhl / cvm;
+  R icpk = lhi / cvm;
+  R tcpk = lcpk + icpk;
+  R tcp3 = lcpk + icpk * mmin(R(1.0), dim(tice, R(s.tz)) / (tice - t_wfr));
+  if (!(p1 >= P_MIN)) return;
+  if (s.tz < T_MIN) {
+    double sink = dim(1.0e-7, s.qv);
+    s.qv = s.qv - sink;
+    s.qi = s.qi + sink;
+    q_sol = q_sol + R(sink);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz + double(R(sink) * (lhl + lhi) / cvm);
+    if (!c.do_qa) s.qa = s.qa + 1.0;
+    return;
+  }
+  lhl = c.lv00 + c.d0_vap * R(s.tz);
+  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lcpk = lhl / cvm;
+  icpk = lhi / cvm;
+  tcpk = lcpk + icpk;
+  tcp3 = lcpk + icpk * mmin(R(1.0), dim(tice, R(s.tz)) / (tice - t_wfr));
+
+  // instant evaporation / sublimation of all clouds when rh < rh_adj
+  R qpz = R(s.qv) + R(s.ql) + R(s.qi);
+  const R dtin = (lhl * (R(s.ql) + R(s.qi)) + lhi * R(s.qi)) /
+                 (c.c_air + qpz * c.c_vap + R(s.qr) * R(C_LIQ) + (R(s.qs) + R(s.qg)) * R(C_ICE));
+  R tin = R(s.tz) - dtin;
+  R rh = R(0.0);
+  if (tin > t_sub + R(6.0)) {
+    rh = qpz / iqs1(tin, den);
+    if (rh < rh_adj) {
+      // all vapour, liquid and ice of the cell collapse into vapour; the
+      // temperature change is applied as an FP64 increment
+      s.tz = s.tz - double(dtin);
+      if constexpr (is_f32<R>::value)
+        s.qv = s.qv + s.ql + s.qi;
+      else
+        s.qv = double(qpz);
+      s.ql = 0.0;
+      s.qi = 0.0;
+    }
+  }
+  if (!(((tin > t_sub + R(6.0)) && (rh >= rh_adj)) || (tin <= t_sub + R(6.0)))) return;
+
+  // cloud water <-> vapour adjustment
+  R dwsdt;
+  R qsw = wqs2(R(s.tz), den, dwsdt);
+  R dq0 = qsw - R(s.qv);
+  double evap;
+  if (dq0 > R(0.0)) {
+    R factor = mmin(R(1.0), c.fac_l2v * (R(10.0) * dq0 / qsw));
+    evap = mmin(s.ql, double(factor * dq0 / (R(1.0) + tcp3 * dwsdt)));
+  } else {
+    evap = double(dq0 / (R(1.0) + tcp3 * dwsdt));
+  }
+  s.qv = s.qv + evap;
+  s.ql = s.ql - evap;
+  q_liq = q_liq - R(evap);
+  cvm = SH_CVM(c, s, q_liq, q_sol);
+  s.tz = s.tz - double(R(evap) * lhl / cvm);
+  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  icpk = lhi / cvm;
+
+  // complete freezing below t_wfr
+  R dtmp = t_wfr - R(s.tz);
+  if ((dtmp > R(0.0)) && (s.ql > QCMIN)) {
+    double sink = mmin(s.ql, double(mmin(R(s.ql) * dtmp * R(0.125), dtmp / icpk)));
+    s.ql = s.ql - sink;
+    s.qi = s.qi + sink;
+    q_liq = q_liq - R(sink);
+    q_sol = q_sol + R(sink);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz + double(R(sink) * lhi / cvm);
+  }
+  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  icpk = lhi / cvm;
+
+  // Bigg mechanism
+  R dt_pisub;
+  if (c.fast_sat_adj) {
+    dt_pisub = R(0.5) * dts;
+  } else {
+    dt_pisub = dts;
+    R tc = tice - R(s.tz);
+    if ((s.ql > QRMIN) && (tc > R(0.0))) {
+      R sink = R(3.3333e-10) * dts * (mexp(R(0.66) * tc) - R(1.0)) * den * R(s.ql) * R(s.ql);
+      double sk = mmin(s.ql, double(mmin(tc / icpk, sink)));
+      s.ql = s.ql - sk;
+      s.qi = s.qi + sk;
+      q_liq = q_liq - R(sk);
+      q_sol = q_sol + R(sk);
+      cvm = SH_CVM(c, s, q_liq, q_sol);
+      s.tz = s.tz + double(R(sk) * lhi / cvm);
+    }
+  }
+  lhl = c.lv00 + c.d0_vap * R(s.tz);
+  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lcpk = lhl / cvm;
+  icpk = lhi / cvm;
+  tcpk = lcpk + icpk;
+
+  // sublimation / deposition of cloud ice
+  if (s.tz < tice_d) {
+    R dqsdt;
+    R qsi = iqs2(R(s.tz), den, dqsdt);
+    R dq = R(s.qv) - qsi;
+    R sink = dq / (R(1.0) + tcpk * dqsdt);
+    R pidep;
+    if (s.qi > QRMIN) {
+      pidep = dt_pisub * dq * R(349138.78) * mexp(R(0.875) * mlog(R(s.qi) * den)) /
+              (qsi * den * R(LAT2) / (R(0.0243) * R(RVGAS) * (R(s.tz) * R(s.tz))) + R(4.42478e4));
+    } else {
+      pidep = R(0.0);
+    }
+    double sk;
+    if (dq > R(0.0)) {
+      R tmp = tice - R(s.tz);
+      R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * tmp) / den;
+      sk = double(mmin(sink, mmin(mmax(qi_crt - R(s.qi), pidep), tmp / tcpk)));
+      sk = cap<R>(sk, s.qv);
+    } else {
+      pidep = pidep * mmin(R(1.0), dim(R(s.tz), t_sub) * R(0.2));
+      sk = mmax(double(mmax(pidep, sink)), -s.qi);
+    }
+    s.qv = s.qv - sk;
+    s.qi = s.qi + sk;
+    q_sol = q_sol + R(sk);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz + double(R(sk) * (lhl + lhi) / cvm);
+  }
+  lhl = c.lv00 + c.d0_vap * R(s.tz);
+  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lcpk = lhl / cvm;
+  icpk = lhi / cvm;
+  tcpk = lcpk + icpk;
+
+  // sublimation / deposition of snow
+  if (s.qs > QRMIN) {
+    R dqsdt;
+    R qsi = iqs2(R(s.tz), den, dqsdt);
+    R qden = R(s.qs) * den;
+    R tmp = mexp(R(0.65625) * mlog(qden));
+    R tsq = R(s.tz) * R(s.tz);
+    R dq = (qsi - R(s.qv)) / (R(1.0) + tcpk * dqsdt);
+    R pssub = c.cssub[0] * tsq * (c.cssub[1] * msqrt(qden) + c.cssub[2] * tmp * msqrt(denfac)) /
+              (c.cssub[3] * tsq + c.cssub[4] * qsi * den);
+    pssub = (qsi - R(s.qv)) * dts * pssub;
+    double ps;
+    if (pssub > R(0.0)) {
+      ps = mmin(double(pssub * mmin(R(1.0), dim(R(s.tz), t_sub) * R(0.2))), s.qs);
+    } else {
+      if (s.tz > tice_d)
+        ps = 0.0;
+      else
+        ps = cap<R>(-double(mmax(pssub, mmax(dq, (R(s.tz) - tice) / tcpk))), s.qv) * -1.0;
+    }
+    s.qs = s.qs - ps;
+    s.qv = s.qv + ps;
+    q_sol = q_sol - R(ps);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz - double(R(ps) * (lhl + lhi) / cvm);
+  }
+  lhl = c.lv00 + c.d0_vap * R(s.tz);
+  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lcpk = lhl / cvm;
+  icpk = lhi / cvm;
+  tcpk = lcpk + icpk;
+
+  // two-way graupel sublimation / deposition
+  if (s.qg > QRMIN) {
+    R dqsdt;
+    R qsi = iqs2(R(s.tz), den, dqsdt);
+    R dq = (R(s.qv) - qsi) / (R(1.0) + tcpk * dqsdt);
+    R pgsub = (R(s.qv) / qsi - R(1.0)) * R(s.qg);
+    double pg;
+    if (pgsub > R(0.0)) {
+      if (s.tz > tice_d)
+        pg = 0.0;
+      else
+        pg = cap<R>(double(mmin(mmin(c.fac_v2g * pgsub, R(0.2) * dq),
+                                mmin(R(s.ql) + R(s.qr), (tice - R(s.tz)) / tcpk))),
+                    s.qv);
+    } else {
+      pg = -cap<R>(-double(mmax(c.fac_g2v * pgsub, dq) * mmin(R(1.0), dim(R(s.tz), t_sub) * R(0.1))), s.qg);
+    }
+    s.qg = s.qg + pg;
+    s.qv = s.qv - pg;
+    q_sol = q_sol + R(pg);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz + double(R(pg) * (lhl + lhi) / cvm);
+  }
+
+  // minimum evaporation of rain in dry air
+  lhl = c.lv00 + c.d0_vap * R(s.tz);
+  lcpk = lhl / cvm;
+  if (s.qr > QCMIN) {
+    R dqsdt;
+    R qsw2 = wqs2(R(s.tz), den, dqsdt);
+    double sink = mmin(s.qr, double(dim(rh_rain * qsw2, R(s.qv)) / (R(1.0) + lcpk * dqsdt)));
+    s.qv = s.qv + sink;
+    s.qr = s.qr - sink;
+    q_liq = q_liq - R(sink);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz - double(R(sink) * lhl / cvm);
+  }
+  lhl = c.lv00 + c.d0_vap * R(s.tz);
+  cvm = c.c_air + (R(s.qv) + q_liq + q_sol) * c.c_vap;
+  lcpk = lhl / cvm;
+
+  // cloud fraction
+  if (!c.do_qa) {
+    q_sol = c.rad_snow ? R(s.qi) + R(s.qs) : R(s.qi);
+    q_liq = c.rad_rain ? R(s.ql) + R(s.qr) : R(s.ql);
+    R q_cond = q_liq + q_sol;
+    R qpz2 = R(s.qv) + q_cond;
+    R tin2 = R(s.tz) - (lcpk * q_cond + icpk * q_sol);
+    R qstar;
+    if (tin2 <= t_wfr) {
+      qstar = iqs1(tin2, den);
+    } else if (tin2 >= tice) {
+      qstar = wqs1(tin2, den);
+    } else {
+      R qsi = iqs1(tin2, den);
+      R qsw3 = wqs1(tin2, den);
+      R rqi;
+      if (q_cond > R(3.0e-6))
+        rqi = q_sol / q_cond;
+      else
+        rqi = (tice - tin2) / (tice - t_wfr);
+      qstar = rqi * qsi + (R(1.0) - rqi) * qsw3;
+    }
+    if (qpz2 > R(QRMIN)) {
+      R dq = mmax(R(QCMIN), h_var * qpz2);
+      R q_plus = qpz2 + dq;
+      R q_minus = qpz2 - dq;
+      if (qstar < q_minus)
+        s.qa = s.qa + 1.0;
+      else if ((qstar < q_plus) && (q_cond > c.qc_crt))
+        s.qa = s.qa + double((q_plus - qstar) / (dq + dq));
+    }
+  }
+}
+
+// ------------------------------------------------ f:853-1260
+template <class R>
+__device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, Cell& s, R h_var,
+                                            R rh_adj, R rh_rain, R den, R denfac, R vtgz, R vtrz, R vtsz,
+                                            double p1d, R di, R q_liq, R q_sol, R cvm) {
+  const R tice = c.tice, dts = c.dts, rdts = c.rdts;
+  R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  R icpk = lhi / cvm;
+  if (p1d >= P_MIN) {
+    const R p1 = R(p1d);
+    R pgacr = R(0.0), pgacw = R(0.0);
+    R tc = R(s.tz) - tice;
+    if (s.tz - tice_d >= 0.0) {
+      R dqs0 = c.ces0 / p1 - R(s.qv);
+      if (s.qs > QCMIN) {
+        R psacw, psacr, pracs;
+        if (s.ql > QRMIN) {
+          R factor = denfac * c.csacw * mexp(R(0.8125) * mlog(R(s.qs) * den));
+          psacw = factor / (R(1.0) + dts * factor) * R(s.ql);
+        } else {
+          psacw = R(0.0);
+        }
+        if (s.qr > QRMIN) {
+          psacr = mmin(acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, c.acco[0][1], c.acco[1][1], c.acco[2][1], den),
+                       R(s.qr) * rdts);
+          pracs = acr3d(vtrz, vtsz, R(s.qs), R(s.qr), c.cracs, c.acco[0][0], c.acco[1][0], c.acco[2][0], den);
+        } else {
+          psacr = R(0.0);
+          pracs = R(0.0);
+        }
+        R psmlt = mmax(R(0.0), smlt(tc, dqs0, R(s.qs) * den, psacw, psacr, c.csmlt, den, denfac));
+        double sink = mmin(s.qs, double(mmin(dts * (psmlt + pracs), tc / icpk)));
+        s.qs = s.qs - sink;
+        double tmp = mmin(sink, double(dim(c.qs_mlt, R(s.ql))));
+        s.ql = s.ql + tmp;
+        s.qr = s.qr + sink - tmp;
+        q_liq = q_liq + R(sink);
+        q_sol = q_sol - R(sink);
+        cvm = SH_CVM(c, s, q_liq, q_sol);
+        s.tz = s.tz - double(R(sink) * lhi / cvm);
+        tc = R(s.tz) - tice;
+      }
+      lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+      icpk = lhi / cvm;
+      if ((s.qg > QCMIN) && (tc > R(0.0))) {
+        if (s.qr > QRMIN)
+          pgacr = mmin(acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, c.acco[0][2], c.acco[1][2], c.acco[2][2], den),
+                       rdts * R(s.qr));
+        R qden = R(s.qg) * den;
+        if (s.ql > QRMIN) {
+          R factor = c.cgacw * qden / msqrt(den * msqrt(msqrt(qden)));
+          pgacw = factor / (R(1.0) + dts * factor) * R(s.ql);
+        }
+        R pgmlt = dts * gmlt(tc, dqs0, qden, pgacw, pgacr, c.cgmlt, den);
+        double pm = mmin(double(mmax(R(0.0), pgmlt)), mmin(s.qg, double(tc / icpk)));
+        s.qg = s.qg - pm;
+        s.qr = s.qr + pm;
+        q_liq = q_liq + R(pm);
+        q_sol = q_sol - R(pm);
+        cvm = SH_CVM(c, s, q_liq, q_sol);
+        s.tz = s.tz - double(R(pm) * lhi / cvm);
+      }
+    } else {
+      if (s.qi > 3.0e-7) {
+        R psaci;
+        if (s.qs > 1.0e-7) {
+          R factor = dts * denfac * c.csaci * mexp(R(0.05) * tc + R(0.8125) * mlog(R(s.qs) * den));
+          psaci = factor / (R(1.0) + factor) * R(s.qi);
+        } else {
+          psaci = R(0.0);
+        }
+        R qim = c.qi0_crt / den;
+        R tmp = c.const_vi ? c.fac_i2s : c.fac_i2s * mexp(R(0.025) * tc);
+        di = mmax(di, R(QRMIN));
+        R q_plus = R(s.qi) + di;
+        R psaut;
+        if (q_plus > (qim + R(QRMIN))) {
+          R dq;
+          if (qim > (R(s.qi) - di))
+            dq = (R(0.25) * ((q_plus - qim) * (q_plus - qim))) / di;
+          else
+            dq = R(s.qi) - qim;
+          psaut = tmp * dq;
+        } else {
+          psaut = R(0.0);
+        }
+        double sink = mmin(0.75 * s.qi, double(psaci + psaut));
+        s.qi = s.qi - sink;
+        s.qs = s.qs + sink;
+        if (s.qg > 1.0e-6) {
+          R factor = dts * c.cgaci * msqrt(den) * R(s.qg);
+          double pgaci = cap<R>(double(factor / (R(1.0) + factor) * R(s.qi)), s.qi);
+          s.qi = s.qi - pgaci;
+          s.qg = s.qg + pgaci;
+        }
+      }
+      tc = R(s.tz) - tice;
+      if ((s.qr > 1e-7) && (s.tz - tice_d < 0.0)) {
+        R psacr;
+        if (s.qs > 1.0e-7)
+          psacr = dts * acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, c.acco[0][1], c.acco[1][1], c.acco[2][1], den);
+        else
+          psacr = R(0.0);
+        R pgfr = dts * c.cgfr[0] / den * (mexp(-c.cgfr[1] * tc) - R(1.0)) * mexp(R(1.75) * mlog(R(s.qr) * den));
+        R sink = psacr + pgfr;
+        R factor = mmin(sink, mmin(R(s.qr), -tc / icpk)) / mmax(sink, R(QRMIN));
+        psacr = factor * psacr;
+        pgfr = factor * pgfr;
+        double ps = double(psacr), pg = double(pgfr);
+        if constexpr (is_f32<R>::value) {
+          double tot = ps + pg;
+          if (tot > s.qr) {
+            double f = s.qr / tot;
+            ps *= f;
+            pg = s.qr - ps;
+          }
+        }
+        double sk = ps + pg;
+        s.qr = s.qr - sk;
+        s.qs = s.qs + ps;
+        s.qg = s.qg + pg;
+        q_liq = q_liq - R(sk);
+        q_sol = q_sol + R(sk);
+        cvm = SH_CVM(c, s, q_liq, q_sol);
+        s.tz = s.tz + double(R(sk) * lhi / cvm);
+      }
+      lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+      icpk = lhi / cvm;
+      if (s.qs > 1.0e-7) {
+        R sink;
+        if (s.qg > QRMIN)
+          sink = dts * acr3d(vtgz, vtsz, R(s.qs), R(s.qg), c.cgacs, c.acco[0][3], c.acco[1][3], c.acco[2][3], den);
+        else
+          sink = R(0.0);
+        R qsm = c.qs0_crt / den;
+        if (R(s.qs) > qsm) {
+          R factor = dts * R(1.0e-3) * mexp(R(0.09) * (R(s.tz) - tice));
+          sink = sink + factor / (R(1.0) + factor) * (R(s.qs) - qsm);
+        }
+        double sk = mmin(s.qs, double(sink));
+        s.qs = s.qs - sk;
+        s.qg = s.qg + sk;
+      }
+      if ((s.qg > 1.0e-7) && (s.tz < tice0_d)) {
+        if (s.ql > 1.0e-6) {
+          R qden = R(s.qg) * den;
+          R factor = dts * c.cgacw * qden / msqrt(den * msqrt(msqrt(qden)));
+          pgacw = factor / (R(1.0) + factor) * R(s.ql);
+        } else {
+          pgacw = R(0.0);
+        }
+        if (s.qr > 1.0e-6)
+          pgacr = mmin(dts * acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, c.acco[0][2], c.acco[1][2], c.acco[2][2], den),
+                       R(s.qr));
+        else
+          pgacr = R(0.0);
+        R sink = pgacr + pgacw;
+        R factor = mmin(sink, dim(tice, R(s.tz)) / icpk) / mmax(sink, R(QRMIN));
+        pgacr = factor * pgacr;
+        pgacw = factor * pgacw;
+        double pr = cap<R>(double(pgacr), s.qr), pw = cap<R>(double(pgacw), s.ql);
+        double sk = pr + pw;
+        s.qg = s.qg + sk;
+        s.qr = s.qr - pr;
+        s.ql = s.ql - pw;
+        q_liq = q_liq - R(sk);
+        q_sol = q_sol + R(sk);
+        cvm = SH_CVM(c, s, q_liq, q_sol);
+        s.tz = s.tz + double(R(sk) * lhi / cvm);
+      }
+    }
+  }
+  subgrid_z_proc<R>(c, tice_d, s, h_var, rh_adj, rh_rain, den, denfac, p1d);
+}
+
+}  // namespace shield

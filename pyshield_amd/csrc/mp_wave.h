@@ -1,0 +1,217 @@
+// mp_wave.h -- column operations for the "levels across lanes" layout.
+//
+// One 64-lane wavefront owns `ncols` consecutive columns.  Their cells are
+// flattened column-major (cell = col_local * nz + k, k = 0 at the model top)
+// and dealt to the wavefront in S slots: slot s, lane l holds cell 64 s + l.
+// With the reference's [X, Y, Z] storage (k contiguous) one slot of one field
+// is a contiguous 512-byte run, so every global load and store is coalesced.
+//
+// Vertical operations of the reference's FORWARD/BACKWARD blocks become:
+//   * column any / first-hit  -> wave ballots + scalar bit arithmetic,
+//   * k-1 / k+1 / surface / top values -> one LDS round trip,
+//   * linear recurrences x[k] = a[k] x[k-1] + b[k] (and the mirror image
+//     from the surface up) -> segmented Hillis-Steele scans over the 64 lanes
+//     of a slot with a carry between slots; a segment starts at k = 0
+//     (forward) or k = nz-1 (backward), so columns never mix.
+// The workgroup is exactly one wavefront, so __syncthreads() is the only
+// synchronisation the LDS round trips need.
+#pragma once
+
+#include "mp_common.h"
+
+namespace shield {
+
+template <int S>
+struct Lay {
+  int k[S];   // level of the cell (0 for cells past the last column)
+  int kb[S];  // nz-1-k (0 for cells past the last column)
+  int cl[S];  // local column index (0 for cells past the last column)
+  bool v[S];  // cell belongs to a column of this wavefront
+  int nz, ncols;
+};
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// bits [lo, hi) of a 64-bit mask, clipped to [0, 64)
+__device__ __forceinline__ uint64_t range_mask(int lo, int hi) {
+  lo = lo < 0 ? 0 : lo;
+  hi = hi > 64 ? 64 : hi;
+  if (hi <= lo) return 0ull;
+  const int n = hi - lo;
+  const uint64_t m = (n == 64) ? ~0ull : ((1ull << n) - 1ull);
+  return m << lo;
+}
+
+__device__ __forceinline__ double readlane(double v, int l) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_readlane(lo, l);
+  hi = __builtin_amdgcn_readlane(hi, l);
+  return __hiloint2double(hi, lo);
+}
+
+template <int S>
+__device__ __forceinline__ bool wave_any(const bool (&p)[S]) {
+  bool a = false;
+#pragma unroll
+  for (int s = 0; s < S; ++s) a = a || (__ballot(p[s]) != 0ull);
+  return a;
+}
+
+// out = "some cell of my column satisfies pred" (pred must be false on
+// cells past the last column).  FORWARD+BACKWARD no_fall pair of
+// microphysics.py:349-368 and its three analogues.
+template <int S>
+__device__ __forceinline__ void col_any(const Lay<S>& L, const bool (&pred)[S], bool (&out)[S]) {
+  uint64_t m[S], res[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    m[s] = __ballot(pred[s]);
+    res[s] = 0ull;
+  }
+  for (int c = 0; c < L.ncols; ++c) {
+    const int lo = c * L.nz, hi = lo + L.nz;
+    uint64_t hit = 0ull;
+#pragma unroll
+    for (int s = 0; s < S; ++s) hit |= m[s] & range_mask(lo - 64 * s, hi - 64 * s);
+    if (hit) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) res[s] |= range_mask(lo - 64 * s, hi - 64 * s);
+    }
+  }
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) out[s] = ((res[s] >> lane) & 1ull) != 0ull;
+}
+
+// out = "k is at or below the first level (from the top) where pred holds,
+// or k is the surface layer": the stop_k sweep of microphysics.py:706-724.
+template <int S>
+__device__ __forceinline__ void col_from_first(const Lay<S>& L, const bool (&pred)[S], bool (&out)[S]) {
+  uint64_t m[S], res[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    m[s] = __ballot(pred[s]);
+    res[s] = 0ull;
+  }
+  for (int c = 0; c < L.ncols; ++c) {
+    const int lo = c * L.nz, hi = lo + L.nz;
+    int first = hi - 1;
+    bool found = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const uint64_t h = m[s] & range_mask(lo - 64 * s, hi - 64 * s);
+      if (!found && h) {
+        first = 64 * s + __builtin_ctzll(h);
+        found = true;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) res[s] |= range_mask(first - 64 * s, hi - 64 * s);
+  }
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) out[s] = ((res[s] >> lane) & 1ull) != 0ull;
+}
+
+// ---------------------------------------------------------- LDS round trips
+// value of the cell above (k-1); undefined at k = 0
+template <int S, class T>
+__device__ __forceinline__ void col_prev(T* buf, const T (&x)[S], T (&out)[S]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) buf[64 * s + lane] = x[s];
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int j = 64 * s + lane - 1;
+    out[s] = buf[j < 0 ? 0 : j];
+  }
+  __syncthreads();
+}
+// value of the cell below (k+1); undefined at k = nz-1
+template <int S, class T>
+__device__ __forceinline__ void col_next(T* buf, const T (&x)[S], T (&out)[S]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) buf[64 * s + lane] = x[s];
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int j = 64 * s + lane + 1;
+    out[s] = buf[j > 64 * S - 1 ? 64 * S - 1 : j];
+  }
+  __syncthreads();
+}
+// value of the surface layer (k = nz-1) of the cell's column
+template <int S, class T>
+__device__ __forceinline__ void col_bottom(const Lay<S>& L, T* buf, const T (&x)[S], T (&out)[S]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) buf[64 * s + lane] = x[s];
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S; ++s) out[s] = buf[L.cl[s] * L.nz + L.nz - 1];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- scans
+// x[k] = a[k] x[k-1] + b[k] for k > 0, x[0] = b[0]   (top -> surface)
+template <int S>
+__device__ __forceinline__ void scan_fwd(const Lay<S>& L, const double (&a)[S], const double (&b)[S],
+                                         double (&x)[S]) {
+  const int lane = lane_id();
+  double carry = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double A = a[s], B = b[s];
+    const int k = L.k[s];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
+      if (lane >= d && k >= d) {
+        B = A * Bp + B;
+        A = A * Ap;
+      }
+    }
+    x[s] = (k <= lane) ? B : A * carry + B;
+    carry = readlane(x[s], 63);
+  }
+}
+// x[k] = x[k-1] + b[k] for k > 0, x[0] = b[0]
+template <int S>
+__device__ __forceinline__ void sum_fwd(const Lay<S>& L, const double (&b)[S], double (&x)[S]) {
+  const int lane = lane_id();
+  double carry = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double B = b[s];
+    const int k = L.k[s];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double Bp = __shfl_up(B, d);
+      if (lane >= d && k >= d) B = Bp + B;
+    }
+    x[s] = (k <= lane) ? B : carry + B;
+    carry = readlane(x[s], 63);
+  }
+}
+// x[k] = x[k+1] + b[k] for k < nz-1, x[nz-1] = b[nz-1]   (surface -> top)
+template <int S>
+__device__ __forceinline__ void sum_bwd(const Lay<S>& L, const double (&b)[S], double (&x)[S]) {
+  const int lane = lane_id();
+  double carry = 0.0;
+#pragma unroll
+  for (int s = S - 1; s >= 0; --s) {
+    double B = b[s];
+    const int kb = L.kb[s];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double Bp = __shfl_down(B, d);
+      if (lane + d < 64 && kb >= d) B = Bp + B;
+    }
+    x[s] = (kb <= 63 - lane) ? B : carry + B;
+    carry = readlane(x[s], 0);
+  }
+}
+
+}  // namespace shield

@@ -1,0 +1,214 @@
+// shield_api.hip -- C ABI entry points (include/shield_mp.h): host-side
+// set-up of the microphysics scalars, argument validation and launches.
+#include <math.h>
+#include <string.h>
+
+#include "mp_common.h"
+
+namespace shield {
+hipError_t launch_forward(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f, int mode,
+                          hipStream_t stream);
+struct LayoutChoice {
+  int S, P;
+};
+LayoutChoice choose_layout(int64_t nz);
+}  // namespace shield
+
+using namespace shield;
+
+namespace {
+
+// Microphysics._set_timestep (microphysics.py:1964-1979).  Python's round()
+// rounds half to even, as nearbyint does in the default rounding mode.
+void set_timestep(shield_mp_params* p, double timestep) {
+  const shield_mp_namelist& n = p->nml;
+  p->timestep = timestep;
+  p->mpdt = timestep < n.mp_time ? timestep : n.mp_time;
+  p->rdt = 1.0 / timestep;
+  p->ntimes = (int32_t)nearbyint(timestep / p->mpdt);
+  p->dts = timestep / p->ntimes;
+  p->dt_rain = p->dts * 0.5;
+  p->rdts = 1.0 / p->dts;
+  p->dt_evap = n.fast_sat_adj ? 0.5 * p->dts : p->dts;
+  p->fac_i2s = 1.0 - exp(-p->dts / n.tau_i2s);
+  p->fac_g2v = 1.0 - exp(-p->dts / n.tau_g2v);
+  p->fac_v2g = 1.0 - exp(-p->dts / n.tau_v2g);
+  p->fac_imlt = 1.0 - exp(-0.5 * p->dts / n.tau_imlt);
+  p->fac_l2v = 1.0 - exp(-p->dt_evap / n.tau_l2v);
+}
+
+// Microphysics.__init__ scalars (microphysics.py:1643-1659,1712-1713) and
+// gfdl_cloud_microphys_init / setupm (:1806-1954): collection, evaporation,
+// sublimation and melting coefficients of Lin et al. (1983).
+void setupm(shield_mp_params* p) {
+  const shield_mp_namelist& n = p->nml;
+  p->c_air = CP_AIR;
+  p->c_vap = CP_VAP;
+  p->d0_vap = p->c_vap - C_LIQ;
+  p->lv00 = HLV - p->d0_vap * TICE;
+  p->do_sedi_w = n.hydrostatic ? 0 : n.do_sedi_w;
+  p->cpaut = n.c_paut * 0.104 * GRAV / 1.717e-5;
+  p->use_ccn = n.prog_ccn ? 0 : 1;
+  p->so3 = 7.0 / 3.0;
+  p->zs = 0.0;
+  p->log_10 = log(10.0);
+  p->tice0 = n.tice - 0.01;
+  p->t_wfr = n.tice - 40.0;
+
+  // gamma-function values and intercept / density parameters
+  const double gam263 = 1.456943, gam275 = 1.608355, gam290 = 1.827363;
+  const double gam325 = 2.54925, gam350 = 3.323363, gam380 = 4.694155;
+  const double rnzs = 3.0e6, rnzr = 8.0e6, rnzg = 4.0e6;
+  const double rhos = 1.0e2, rhog = 4.0e2, rhor = RHOR;
+  const double acc[3] = {5.0, 2.0, 0.5};
+  const double pie = 4.0 * atan(1.0);
+  p->fac_rc = (4.0 / 3.0) * pie * rhor * pow(n.rthresh, 3.0);
+  const double vdifu = 2.11e-5, tcond = 2.36e-2, visk = 1.259e-5;
+  const double hlts = 2.8336e6, hltc = 2.5e6, hltf = 3.336e5, ch2o = 4.1855e3;
+  const double pisq = pie * pie;
+  const double scm3 = pow(visk / vdifu, 1.0 / 3.0);
+  p->cracs = pisq * rnzr * rnzs * rhos;
+  p->csacr = pisq * rnzr * rnzs * rhor;
+  p->cgacr = pisq * rnzr * rnzg * rhor;
+  p->cgacs = pisq * rnzg * rnzs * rhos;
+  p->cgacs = p->cgacs * n.c_pgacs;
+  // act[2k], act[2k+1]: the (collector, collected) slope factors of the four
+  // accretion pairs (snow-rain, rain-snow, graupel-rain, graupel-snow)
+  const double act_s = pie * rnzs * rhos, act_r = pie * rnzr * rhor, act_g = pie * rnzg * rhog;
+  const double act[8] = {act_s, act_r, act_r, act_s, act_r, act_g, act_s, act_g};
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 4; ++k)
+      p->acco[i][k] = acc[i] / (pow(act[2 * k], (6 - i) * 0.25) * pow(act[2 * k + 1], (i + 1) * 0.25));
+  const double gcon = 40.74 * sqrt(SFCRHO);
+  p->csacw = pie * rnzs * n.clin * gam325 / (4.0 * pow(act_s, 0.8125));
+  const double craci = pie * rnzr * n.alin * gam380 / (4.0 * pow(act_r, 0.95));
+  p->csaci = p->csacw * n.c_psaci;
+  p->cgacw = pie * rnzg * gam350 * gcon / (4.0 * pow(act_g, 0.875));
+  p->cgaci = p->cgacw * 0.05;
+  p->cracw = n.c_cracw * craci;
+  p->cssub[0] = 2.0 * pie * vdifu * tcond * RVGAS * rnzs;
+  p->cssub[1] = 0.78 / sqrt(act_s);
+  p->cssub[2] = 0.31 * scm3 * gam263 * sqrt(n.clin / visk) / pow(act_s, 0.65625);
+  p->cssub[3] = tcond * RVGAS;
+  p->cssub[4] = (hlts * hlts) * vdifu;
+  p->cgsub[0] = 2.0 * pie * vdifu * tcond * RVGAS * rnzg;
+  p->cgsub[1] = 0.78 / sqrt(act_g);
+  p->cgsub[2] = 0.31 * scm3 * gam275 * sqrt(gcon / visk) / pow(act_g, 0.6875);
+  p->cgsub[3] = p->cssub[3];
+  p->cgsub[4] = p->cssub[4];
+  p->crevp[0] = 2.0 * pie * vdifu * tcond * RVGAS * rnzr;
+  p->crevp[1] = 0.78 / sqrt(act_r);
+  p->crevp[2] = 0.31 * scm3 * gam290 * sqrt(n.alin / visk) / pow(act_r, 0.725);
+  p->crevp[3] = p->cssub[3];
+  p->crevp[4] = hltc * hltc * vdifu;
+  p->cgfr[0] = 20.0e2 * pisq * rnzr * rhor / pow(act_r, 1.75);
+  p->cgfr[1] = 0.66;
+  p->csmlt[0] = 2.0 * pie * tcond * rnzs / hltf;
+  p->csmlt[1] = 2.0 * pie * vdifu * rnzs * hltc / hltf;
+  p->csmlt[2] = p->cssub[1];
+  p->csmlt[3] = p->cssub[2];
+  p->csmlt[4] = ch2o / hltf;
+  p->cgmlt[0] = 2.0 * pie * tcond * rnzg / hltf;
+  p->cgmlt[1] = 2.0 * pie * vdifu * rnzg * hltc / hltf;
+  p->cgmlt[2] = p->cgsub[1];
+  p->cgmlt[3] = p->cgsub[2];
+  p->cgmlt[4] = ch2o / hltf;
+  const double es0 = 6.107799961e2;
+  p->ces0 = (RDGAS / RVGAS) * es0;
+}
+
+bool shape_ok(const shield_mp_shape* s, int64_t nk_min, int64_t nk_max) {
+  if (!s) return false;
+  if (s->ni < 0 || s->nj < 0) return false;
+  if (s->nk < nk_min || s->nk > nk_max) return false;
+  if (s->ni * s->nj > 0 && (s->si < 0 || s->sj < 0 || s->si2 < 0 || s->sj2 < 0)) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shield_mp_setup(const shield_mp_namelist* nml, double timestep, shield_mp_params* out) {
+  if (!nml || !out) return SHIELD_ERR_NULL;
+  if (!(timestep > 0.0) || !isfinite(timestep)) return SHIELD_ERR_TIMESTEP;
+  memset(out, 0, sizeof(*out));
+  out->nml = *nml;
+  setupm(out);
+  set_timestep(out, timestep);
+  out->accumulate = 1;
+  if (nml->use_ppm || nml->prog_ccn) return SHIELD_ERR_UNSUPPORTED;
+  return SHIELD_OK;
+}
+
+int shield_mp_update_timestep(shield_mp_params* p, double timestep) {
+  if (!p) return SHIELD_ERR_NULL;
+  if (!(timestep > 0.0) || !isfinite(timestep)) return SHIELD_ERR_TIMESTEP;
+  if (timestep != p->timestep) set_timestep(p, timestep);
+  return SHIELD_OK;
+}
+
+int shield_mp_layout(int64_t nk, int* cols_per_wave, int* slots) {
+  const LayoutChoice lc = choose_layout(nk);
+  if (cols_per_wave) *cols_per_wave = lc.P;
+  if (slots) *slots = lc.S;
+  return lc.S == 0 ? SHIELD_ERR_SHAPE : SHIELD_OK;
+}
+
+int shield_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* shield_strerror(int code) {
+  switch (code) {
+    case SHIELD_OK: return "success";
+    case SHIELD_ERR_NULL: return "a required pointer is NULL";
+    case SHIELD_ERR_SHAPE: return "shape or strides out of range (1 <= nk <= 320 for the microphysics kernel)";
+    case SHIELD_ERR_UNSUPPORTED: return "use_ppm and prog_ccn are not supported (both are broken in the reference)";
+    case SHIELD_ERR_MODE: return "unknown precision mode";
+    case SHIELD_ERR_ALIAS: return "an output buffer aliases another field";
+    case SHIELD_ERR_LAUNCH: return "HIP launch failed";
+    case SHIELD_ERR_TIMESTEP: return "timestep must be positive and finite";
+    case SHIELD_ERR_NOGPU: return "no HIP device is visible";
+    default: return "unknown error";
+  }
+}
+
+int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f, int mode,
+                      void* stream) {
+  if (!shape || !p || !f) return SHIELD_ERR_NULL;
+  if (mode != SHIELD_MP_FP64 && mode != SHIELD_MP_MIXED) return SHIELD_ERR_MODE;
+  if (p->nml.use_ppm || p->nml.prog_ccn) return SHIELD_ERR_UNSUPPORTED;
+  if (!(p->timestep > 0.0) || p->ntimes < 1) return SHIELD_ERR_TIMESTEP;
+  if (!shape_ok(shape, 2, 320)) return SHIELD_ERR_SHAPE;
+  if (shape->ni * shape->nj == 0) return SHIELD_OK;
+  const void* in[] = {f->pt, f->qv, f->ql, f->qr, f->qi, f->qs, f->qg, f->qa, f->ua, f->va, f->delp, f->delz,
+                      f->land, f->area};
+  for (const void* q : in)
+    if (!q) return SHIELD_ERR_NULL;
+  const void* out[] = {f->pt_dt, f->qv_dt, f->ql_dt, f->qr_dt, f->qi_dt, f->qs_dt,
+                       f->qg_dt, f->qa_dt, f->udt,   f->vdt,   f->qaz};
+  for (const void* q : out)
+    if (!q) return SHIELD_ERR_NULL;
+  if (p->do_sedi_w && !f->w) return SHIELD_ERR_NULL;
+  // outputs must be distinct from each other and from every input
+  const void* all_out[] = {f->pt_dt, f->qv_dt, f->ql_dt, f->qr_dt, f->qi_dt, f->qs_dt, f->qg_dt, f->qa_dt,
+                           f->udt,   f->vdt,   f->qaz,   f->rain,  f->snow,  f->ice,   f->graupel};
+  const int nout = (int)(sizeof(all_out) / sizeof(all_out[0]));
+  for (int a = 0; a < nout; ++a) {
+    if (!all_out[a]) continue;
+    for (int b = a + 1; b < nout; ++b)
+      if (all_out[a] == all_out[b]) return SHIELD_ERR_ALIAS;
+    for (const void* q : in)
+      if (all_out[a] == q) return SHIELD_ERR_ALIAS;
+    if (f->w && all_out[a] == f->w) return SHIELD_ERR_ALIAS;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SHIELD_ERR_NOGPU;
+  const hipError_t e = launch_forward(*shape, *p, *f, mode, (hipStream_t)stream);
+  return e == hipSuccess ? SHIELD_OK : SHIELD_ERR_LAUNCH;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+"""Drop-in ``Microphysics`` / ``MicrophysicsState`` on the MI355X kernels.
+
+Mirrors pySHiELD/stencils/microphysics.py:1556-2263: same class names,
+constructor and ``__call__(state, timestep)`` signatures, same in-place
+semantics (the ten tendencies are accumulated with ``+=``; ``qice``/``qsnow``
+and their tendencies are rewritten when ``de_ice``; ``wmp`` when
+``do_sedi_w``), same persistent cloud-fraction accumulator ``qaz`` carried
+across calls.  Fields are torch tensors on the GPU (float64, [X, Y, Z] with k
+contiguous) or objects exposing such a tensor as ``.data`` (a Quantity-like
+wrapper); halo offsets are taken from a view of the compute domain.
+
+The whole call runs as one fused HIP kernel through the C ABI
+``shield_mp_forward``; there is no CPU path.
+"""
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._config import PhysicsConfig
+
+
+def _tensor(x):
+    """The torch tensor behind a field (a tensor or an object with .data)."""
+    t = x if isinstance(x, torch.Tensor) else getattr(x, "data", None)
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"expected a torch tensor or an object with a tensor .data, got {type(x)}")
+    return t
+
+
+def _check3(name, t, shape):
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name}: expected float64, got {t.dtype}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: expected a GPU tensor")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
+    if t.dim() == 3 and t.shape[2] > 1 and t.stride(2) != 1:
+        raise ValueError(f"{name}: the k dimension must be contiguous (stride 1)")
+
+
+def _same_strides(ts, what):
+    s0 = ts[0].stride()
+    for t in ts[1:]:
+        if t.stride() != s0:
+            raise ValueError(f"all {what} fields must share the same strides "
+                             f"({s0} vs {t.stride()})")
+    return s0
+
+
+class MicrophysicsState:
+    """pt, qvapor, qliquid, qrain, qice, qsnow, qgraupel, qcld, ua, va, delp,
+    delz, omga: same as the physics state; ``tendency`` is the template the ten
+    tendency fields are copied from; delprsi is the layer pressure thickness,
+    wmp the vertical velocity and land the land mask
+    (microphysics.py:1556-1620)."""
+
+    def __init__(self, pt, qvapor, qliquid, qrain, qice, qsnow, qgraupel, qcld, ua, va,
+                 delp, delz, omga, delprsi, wmp, dz, tendency, land):
+        self.pt = pt
+        self.qvapor = qvapor
+        self.qliquid = qliquid
+        self.qrain = qrain
+        self.qice = qice
+        self.qsnow = qsnow
+        self.qgraupel = qgraupel
+        self.qcld = qcld
+        self.ua = ua
+        self.va = va
+        self.delp = delp
+        self.delz = delz
+        self.omga = omga
+        tend = _tensor(tendency)
+        self.qv_dt = tend.clone()
+        self.ql_dt = tend.clone()
+        self.qr_dt = tend.clone()
+        self.qi_dt = tend.clone()
+        self.qs_dt = tend.clone()
+        self.qg_dt = tend.clone()
+        self.qa_dt = tend.clone()
+        self.udt = tend.clone()
+        self.vdt = tend.clone()
+        self.pt_dt = tend.clone()
+        self.delprsi = delprsi
+        self.wmp = wmp
+        self.dz = dz
+        self.land = land
+
+
+class GridIndexing:
+    """The part of NDSL's GridIndexing the microphysics reads: the compute
+    domain (nx, ny, nz)."""
+
+    def __init__(self, domain):
+        self.domain = tuple(int(d) for d in domain)
+
+
+class StencilFactory:
+    """Minimal stand-in for NDSL's StencilFactory: carries the grid indexing
+    and the torch device the fields live on."""
+
+    def __init__(self, domain, device="cuda"):
+        self.grid_indexing = GridIndexing(domain)
+        self.device = torch.device(device)
+
+
+class QuantityFactory:
+    """Allocates zero-initialised float64 device fields (NDSL QuantityFactory.zeros)."""
+
+    def __init__(self, domain, device="cuda"):
+        self.domain = tuple(int(d) for d in domain)
+        self.device = torch.device(device)
+
+    def zeros(self, dims=None, units="unknown", dtype=torch.float64):
+        shape = self.domain if dims is None or len(dims) == 3 else self.domain[:2]
+        return torch.zeros(shape, dtype=torch.float64, device=self.device)
+
+
+class GridData:
+    """Grid metrics the microphysics reads: the cell area (microphysics.py:1659)."""
+
+    def __init__(self, area):
+        self.area = area
+
+
+def _domain(stencil_factory):
+    if hasattr(stencil_factory, "grid_indexing"):
+        return tuple(stencil_factory.grid_indexing.domain)
+    return tuple(stencil_factory)
+
+
+class Microphysics:
+    """GFDL single-moment cloud microphysics (microphysics.py:1623-2263).
+
+    ``precision`` (extension): ``"fp64"`` runs every operation in FP64, the
+    reference's arithmetic; ``"mixed"`` runs process rates in FP32 with FP64
+    state, transfers, sedimentation/flux sums and precipitation.
+    """
+
+    def __init__(self, stencil_factory, quantity_factory, grid_data, namelist: PhysicsConfig,
+                 precision: str = "fp64"):
+        if precision not in _lib.MODES:
+            raise ValueError(f"precision must be one of {sorted(_lib.MODES)}")
+        self.namelist = namelist
+        self._mode = _lib.MODES[precision]
+        self.precision = precision
+        nx, ny, nz = _domain(stencil_factory)[:3]
+        self._domain = (nx, ny, nz)
+        self._area = grid_data.area
+        device = getattr(quantity_factory, "device", None) or _tensor(grid_data.area).device
+        # persistent device state: the cloud-fraction accumulator (never reset
+        # between calls, microphysics.py:1674) and the surface precipitation
+        # of the last call in mm/day (kept private by the reference,
+        # microphysics.py:1664-1667; exposed here as 2D fields)
+        self._qaz = torch.zeros((nx, ny, nz), dtype=torch.float64, device=device)
+        self.rain = torch.zeros((nx, ny), dtype=torch.float64, device=device)
+        self.snow = torch.zeros_like(self.rain)
+        self.ice = torch.zeros_like(self.rain)
+        self.graupel = torch.zeros_like(self.rain)
+        self._params = _lib.setup(namelist.mp_namelist(), float(namelist.dt_atmos or 1.0))
+        self._timestep = float(self._params.timestep)
+
+    # microphysics.py:1959-1962
+    def _update_timestep_if_needed(self, timestep: float):
+        if timestep != self._timestep:
+            _lib.check(_lib.lib().shield_mp_update_timestep(ctypes.byref(self._params), float(timestep)),
+                       "shield_mp_update_timestep")
+            self._timestep = float(timestep)
+
+    @property
+    def params(self):
+        return self._params
+
+    @property
+    def qaz(self):
+        return self._qaz
+
+    def __call__(self, state: MicrophysicsState, timestep: float):
+        self._update_timestep_if_needed(timestep)
+        nx, ny, nz = self._domain
+        fields = {
+            "pt": state.pt, "qv": state.qvapor, "ql": state.qliquid, "qr": state.qrain,
+            "qi": state.qice, "qs": state.qsnow, "qg": state.qgraupel, "qa": state.qcld,
+            "ua": state.ua, "va": state.va, "delp": state.delprsi, "delz": state.delz,
+            "w": state.wmp,
+            "pt_dt": state.pt_dt, "qv_dt": state.qv_dt, "ql_dt": state.ql_dt, "qr_dt": state.qr_dt,
+            "qi_dt": state.qi_dt, "qs_dt": state.qs_dt, "qg_dt": state.qg_dt, "qa_dt": state.qa_dt,
+            "udt": state.udt, "vdt": state.vdt,
+            "qaz": self._qaz,
+            "land": state.land, "area": self._area,
+            "rain": self.rain, "snow": self.snow, "ice": self.ice, "graupel": self.graupel,
+        }
+        forward({k: _tensor(v) for k, v in fields.items()}, self._params, self._mode)
+
+
+FIELDS_3D = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz", "w",
+             "pt_dt", "qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa_dt", "udt", "vdt",
+             "qaz")
+FIELDS_2D = ("land", "area", "rain", "snow", "ice", "graupel")
+
+
+def forward(fields: dict, params, mode=_lib.SHIELD_MP_FP64, stream=None):
+    """One microphysics call on tensors: ``fields`` maps the names of
+    shield_mp_fields to float64 GPU tensors ([ni, nj, nk], k contiguous; 2D
+    [ni, nj]).  Launches on ``stream`` (default: torch's current stream)."""
+    if isinstance(mode, str):
+        mode = _lib.MODES[mode]
+    pt = fields["pt"]
+    ni, nj, nk = pt.shape
+    t3 = [fields[n] for n in FIELDS_3D if fields.get(n) is not None]
+    t2 = [fields[n] for n in FIELDS_2D if fields.get(n) is not None]
+    for n in FIELDS_3D:
+        if fields.get(n) is not None:
+            _check3(n, fields[n], (ni, nj, nk))
+    for n in FIELDS_2D:
+        if fields.get(n) is not None:
+            _check3(n, fields[n], (ni, nj))
+    si, sj, _ = _same_strides(t3, "3D")
+    si2, sj2 = _same_strides(t2, "2D")
+    shape = _lib.ShieldMpShape(ni, nj, nk, si, sj, si2, sj2)
+    f = _lib.ShieldMpFields()
+    for n in _lib.FIELD_NAMES:
+        t = fields.get(n)
+        setattr(f, n, t.data_ptr() if t is not None else None)
+    if stream is None:
+        stream = torch.cuda.current_stream(pt.device).cuda_stream
+    rc = _lib.lib().shield_mp_forward(ctypes.byref(shape), ctypes.byref(params), ctypes.byref(f),
+                                      int(mode), ctypes.c_void_p(stream))
+    _lib.check(rc, "shield_mp_forward")

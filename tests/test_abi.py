@@ -1,0 +1,99 @@
+"""The C ABI library loads, exports every entry point include/*.h declares,
+and rejects bad arguments before touching the GPU."""
+
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from pyshield_amd import _lib
+from pyshield_amd._config import PhysicsConfig
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(shield_\w+)\s*\(", text, flags=re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert {"shield_mp_setup", "shield_mp_forward", "shield_fill_gfs_delp"} <= names
+    for n in sorted(names):
+        assert hasattr(L, n), n
+    assert set(_lib.EXPORTS) == names
+
+
+def test_struct_sizes_match_layout():
+    # the ctypes mirrors must have the C sizes: 37 doubles + 20 int32 in the namelist
+    assert ctypes.sizeof(_lib.ShieldMpNamelist) == 37 * 8 + 20 * 4
+    assert ctypes.sizeof(_lib.ShieldMpShape) == 7 * 8
+    assert ctypes.sizeof(_lib.ShieldMpFields) == len(_lib.FIELD_NAMES) * 8
+
+
+def test_setup_errors():
+    nml = PhysicsConfig().mp_namelist()
+    with pytest.raises(_lib.ShieldError, match="timestep"):
+        _lib.setup(nml, 0.0)
+    with pytest.raises(_lib.ShieldError, match="use_ppm"):
+        _lib.setup(dict(nml, use_ppm=1), 150.0)
+    with pytest.raises(_lib.ShieldError, match="use_ppm and prog_ccn"):
+        _lib.setup(dict(nml, prog_ccn=1), 150.0)
+
+
+def _fake_fields(alias=None):
+    f = _lib.ShieldMpFields()
+    for i, n in enumerate(_lib.FIELD_NAMES):
+        setattr(f, n, 0x1000 * (i + 1))
+    if alias:
+        setattr(f, alias[0], getattr(f, alias[1]))
+    return f
+
+
+def test_forward_rejects_before_launch():
+    """Validation errors are returned before any device call, with the
+    documented codes (these calls never launch: the fake pointers are caught)."""
+    L = _lib.lib()
+    p = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
+    shape = _lib.ShieldMpShape(2, 2, 79, 2 * 79, 79, 2, 1)
+    fwd = L.shield_mp_forward
+    # unknown mode
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(_fake_fields()), 7, None) == -4
+    # nk out of range
+    bad = _lib.ShieldMpShape(2, 2, 1, 2, 1, 2, 1)
+    assert fwd(ctypes.byref(bad), ctypes.byref(p), ctypes.byref(_fake_fields()), 0, None) == -2
+    bad = _lib.ShieldMpShape(2, 2, 400, 800, 400, 2, 1)
+    assert fwd(ctypes.byref(bad), ctypes.byref(p), ctypes.byref(_fake_fields()), 0, None) == -2
+    # missing input
+    f = _fake_fields()
+    f.pt = None
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), 0, None) == -1
+    # an output aliasing an input or another output
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(_fake_fields(("qv_dt", "qv"))), 0, None) == -5
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(_fake_fields(("udt", "vdt"))), 0, None) == -5
+    # unsupported namelist switches
+    p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
+    p2.nml.use_ppm = 1
+    assert fwd(ctypes.byref(shape), ctypes.byref(p2), ctypes.byref(_fake_fields()), 0, None) == -3
+    # empty domain is a no-op
+    empty = _lib.ShieldMpShape(0, 4, 79, 0, 79, 0, 1)
+    assert fwd(ctypes.byref(empty), ctypes.byref(p), ctypes.byref(_fake_fields()), 0, None) == 0
+
+
+def test_strerror_and_layout():
+    L = _lib.lib()
+    for code in range(-8, 1):
+        assert L.shield_strerror(code)
+    assert _lib.layout(79) == (4, 5)       # 4 columns in 5 slots: 316 of 320 lanes busy
+    assert _lib.layout(127) == (1, 2)
+    assert _lib.layout(8) == (8, 1)
+    with pytest.raises(_lib.ShieldError):
+        _lib.layout(1000)

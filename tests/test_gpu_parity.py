@@ -1,0 +1,195 @@
+"""Parity of the HIP microphysics kernel (through the C ABI) with the FP64 oracle.
+
+FP64 mode must match the oracle to round-off: every tendency, the cloud-fraction
+accumulator and the surface precipitation within a normalised max error of
+FP64_TOL (max |gpu - oracle| / max |oracle| per field).  The kernel evaluates
+the vertical recurrences as parallel scans and uses the GPU's libm, so bitwise
+equality is not expected.
+
+Mixed mode must keep the updated state (physics.py:146-194) within 1e-3 K of
+the oracle's temperature and within 1e-5 relative (floor 1e-12 kg/kg) of its
+mixing ratios, and conserve total water to 1e-12.
+"""
+
+import numpy as np
+import pytest
+
+from pyshield_amd._config import PhysicsConfig
+from pyshield_amd.synthetic import make_state
+
+from helpers import (PRECIP, TEND, WATER, norm_err, run_gpu, run_oracle, small_state,
+                     updated_state, water_imbalance)
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9
+# Surface precipitation below 1e-6 mm/day is round-off of the column flux sum
+# (e.g. cloud ice that never reaches the ground); precipitation errors are
+# normalised by max(field max, PRECIP_FLOOR).
+PRECIP_FLOOR = 1e-6     # mm/day
+MIX_T_TOL = 1e-3        # K
+MIX_Q_RTOL = 1e-5       # relative
+MIX_Q_FLOOR = 1e-12     # kg/kg
+CONS_TOL = 1e-12
+
+
+def _fp64_report(gpu, ora):
+    errs = {}
+    for n in TEND + ("qaz",):
+        errs[n] = norm_err(gpu[n], ora[n], floor=1e-30)
+    for n in PRECIP:
+        errs[n] = norm_err(gpu[n], ora[n], floor=PRECIP_FLOOR)
+    return errs
+
+
+def _assert_fp64(gpu, ora, tol=FP64_TOL):
+    errs = _fp64_report(gpu, ora)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not (v <= tol)}
+    assert not bad, f"fields above {tol}: {bad}"
+
+
+@pytest.mark.parametrize("ni,nj,nz,dt,seed", [
+    (12, 12, 79, 150.0, 1000),   # the reference's 12x12 tile, one sub-step
+    (16, 16, 79, 225.0, 2000),   # dt 225 s -> ntimes = 2
+    (5, 3, 79, 150.0, 11),       # 15 columns: last wavefront partly empty
+    (4, 4, 127, 90.0, 5000),     # deep grid, two slots per lane
+    (7, 5, 8, 150.0, 3),         # shallow columns, many per wavefront
+])
+def test_fp64_matches_oracle(ni, nj, nz, dt, seed):
+    print(f"case ni={ni} nj={nj} nz={nz} dt={dt} seed={seed}", flush=True)
+    ptop = 100.0 if nz > 100 else 300.0
+    st = make_state(ni, nj, nz, seed, ptop=ptop)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=nz)
+    gpu = run_gpu(st, cfg, dt, "fp64")
+    ora = run_oracle(st, cfg, dt)
+    _assert_fp64(gpu, ora)
+
+
+def test_fp64_two_calls_carry_qaz_and_accumulate():
+    st = small_state(6, 6, 79, seed=21)
+    cfg = PhysicsConfig(dt_atmos=150, npz=79)
+    g1 = run_gpu(st, cfg, 150.0, "fp64")
+    o1 = run_oracle(st, cfg, 150.0)
+    tend_g = {n: g1[n] for n in TEND}
+    tend_o = {n: o1[n] for n in TEND}
+    g2 = run_gpu(st, cfg, 150.0, "fp64", qaz=g1["qaz"], tend=tend_g)
+    o2 = run_oracle(st, cfg, 150.0, qaz=o1["qaz"], tend=tend_o)
+    _assert_fp64(g2, o2)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_water_conservation(mode):
+    st = small_state(12, 12, 79, seed=1000)
+    cfg = PhysicsConfig(dt_atmos=150, npz=79)
+    out = run_gpu(st, cfg, 150.0, mode)
+    imb = water_imbalance(st, out, 150.0)
+    print(mode, "max column water imbalance", imb.max(), flush=True)
+    assert imb.max() <= CONS_TOL
+
+
+@pytest.mark.parametrize("dt,seed", [(150.0, 1000), (225.0, 2001)])
+def test_mixed_within_north_star_tolerance(dt, seed):
+    st = small_state(12, 12, 79, seed=seed)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79)
+    mix = run_gpu(st, cfg, dt, "mixed")
+    ora = run_oracle(st, cfg, dt)
+    nm = updated_state(st, mix, dt)
+    no = updated_state(st, ora, dt)
+    terr = float(np.max(np.abs(nm["pt"] - no["pt"])))
+    qerr = {q: float(np.max(np.abs(nm[q] - no[q]) / (MIX_Q_RTOL * np.abs(no[q]) + MIX_Q_FLOOR)))
+            for q in WATER}
+    print("T err", terr, "q err / tol", qerr, flush=True)
+    assert terr <= MIX_T_TOL
+    assert all(v <= 1.0 for v in qerr.values()), qerr
+
+
+def test_deterministic_and_column_independent():
+    st = small_state(8, 6, 79, seed=77)
+    cfg = PhysicsConfig(dt_atmos=150, npz=79)
+    a = run_gpu(st, cfg, 150.0, "fp64")
+    b = run_gpu(st, cfg, 150.0, "fp64")
+    for n in TEND + PRECIP:
+        assert np.array_equal(a[n], b[n]), n
+    # reverse the column order: outputs are the reversed outputs.  A column's
+    # position inside its wavefront sets the association order of the vertical
+    # scans, so agreement is to round-off rather than bitwise.
+    rev = {k: (v[::-1, ::-1].copy() if v.ndim >= 2 else v) for k, v in st.items()}
+    c = run_gpu(rev, cfg, 150.0, "fp64")
+    for n in TEND:
+        assert norm_err(c[n][::-1, ::-1], a[n], floor=1e-30) <= 1e-11, n
+    for n in PRECIP:
+        assert norm_err(c[n][::-1, ::-1], a[n], floor=1e-6) <= 1e-11, n
+
+
+from test_oracle import NAMELISTS  # noqa: E402
+from helpers import cold_top_state, dry_state, low_pressure_top_state  # noqa: E402
+
+
+@pytest.mark.parametrize("overrides", NAMELISTS)
+def test_fp64_namelist_branches(overrides):
+    print("namelist", overrides, flush=True)
+    st = small_state(6, 5, 79, seed=31)
+    cfg = PhysicsConfig(dt_atmos=225, **overrides)
+    gpu = run_gpu(st, cfg, 225.0, "fp64")
+    ora = run_oracle(st, cfg, 225.0)
+    _assert_fp64(gpu, ora)
+    if overrides.get("de_ice"):
+        assert norm_err(gpu["qi"], ora["qi"]) <= 1e-15 and norm_err(gpu["qs"], ora["qs"]) <= 1e-15
+    if overrides.get("do_sedi_w"):
+        assert norm_err(gpu["w"], ora["w"], floor=1e-30) <= FP64_TOL
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_special_columns(mode):
+    cfg = PhysicsConfig()
+    # dry column: no water tendency, no precipitation
+    out = run_gpu(dry_state(), cfg, 150.0, mode)
+    for q in WATER:
+        assert np.all(out[f"{q}_dt"] == 0.0), q
+    for p in PRECIP:
+        assert np.all(out[p] == 0.0), p
+    # freeze-dry below T_MIN
+    st = cold_top_state(qv_top=1.0e-8)
+    g, o = run_gpu(st, cfg, 150.0, mode), run_oracle(st, cfg, 150.0)
+    assert g["qv_dt"][..., 0] == pytest.approx(o["qv_dt"][..., 0], rel=1e-9)
+    assert g["qa_dt"][..., 0] == pytest.approx(1.0 / 150.0, rel=1e-12)
+    # p1 < P_MIN: no ice processes; at normal pressure: deposition
+    thin = run_gpu(low_pressure_top_state(True), cfg, 150.0, mode)
+    assert thin["qv_dt"][..., 0] == 0.0
+    thick = run_gpu(low_pressure_top_state(False), cfg, 150.0, mode)
+    assert thick["qv_dt"][..., 0] < 0.0
+
+
+def test_halo_strided_views():
+    """Fields given as interior views of halo-padded [X+2h, Y+2h, Z+1]
+    storages (the reference's Quantity layout) give the dense results."""
+    import torch
+    from pyshield_amd import _lib
+    from pyshield_amd.microphysics import forward
+
+    st = small_state(5, 7, 79, seed=3)
+    cfg = PhysicsConfig()
+    dense = run_gpu(st, cfg, 150.0, "fp64")
+    h, dev = 3, torch.device("cuda:0")
+    ni, nj, nz = st["pt"].shape
+
+    def padded(x):
+        if x.ndim == 3:
+            big = torch.full((ni + 2 * h + 1, nj + 2 * h + 1, nz + 1), float("nan"), dtype=torch.float64, device=dev)
+            big[h:h + ni, h:h + nj, :nz] = torch.as_tensor(x, device=dev)
+            return big[h:h + ni, h:h + nj, :nz]
+        big = torch.full((ni + 2 * h + 1, nj + 2 * h + 1), float("nan"), dtype=torch.float64, device=dev)
+        big[h:h + ni, h:h + nj] = torch.as_tensor(x, device=dev)
+        return big[h:h + ni, h:h + nj]
+
+    f = {n: padded(st[n]) for n in ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp",
+                                    "delz", "w", "land", "area")}
+    for n in TEND + ("qaz",):
+        f[n] = padded(np.zeros((ni, nj, nz)))
+    for n in PRECIP:
+        f[n] = padded(np.zeros((ni, nj)))
+    forward(f, _lib.setup(cfg.mp_namelist(), 150.0), "fp64")
+    torch.cuda.synchronize()
+    for n in TEND + PRECIP:
+        assert np.array_equal(f[n].cpu().numpy(), dense[n]), n
