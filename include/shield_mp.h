@@ -137,11 +137,13 @@ int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p,
                       const shield_mp_fields* f, int mode, void* stream);
 
 /* ------------------------------------------------- state-update path */
+/* For these entry points shape->si2 and shape->sj2 are the i/j strides of
+ * the nk+1-level interface field (phii or prsi), which has k stride 1 like
+ * every other field; the nk-level fields use shape->si and shape->sj.       */
 
 /* prepare_microphysics (pySHiELD/stencils/physics.py:105-143):
  * dz = (phii[k+1]-phii[k])/g, wmp = -omga(1+zvir qv) pt/delp Rd/g, and the
- * ten tendencies zeroed.  phii has nk+1 levels (interfaces) with the same
- * i/j strides as the 3D fields.  tend[10] = {udt, vdt, pt_dt, qv_dt, ql_dt,
+ * ten tendencies zeroed.  phii has nk+1 levels (interfaces).  tend[10] = {udt, vdt, pt_dt, qv_dt, ql_dt,
  * qr_dt, qi_dt, qs_dt, qg_dt, qa_dt}.                                       */
 int shield_prepare_microphysics(const shield_mp_shape* shape, double* dz,
                                 const double* phii, double* wmp,
@@ -168,7 +170,7 @@ int shield_fill_gfs_delp(const shield_mp_shape* shape, const double* delp,
  * all nk layers (the reference's interval(0, -1) on an nk+1-level domain):
  * u_dt, v_dt, pt_dt += (x_t1 - x_t0) * rdt; the GFS mixing ratios *_t1 are
  * converted to FV3 ones in *_t0 and delp is replaced by the new total mass.
- * prsi has nk+1 interface levels at the same i/j strides.                  */
+ * prsi has nk+1 interface levels.                                          */
 int shield_prepare_tendencies_and_update_tracers(
     const shield_mp_shape* shape, double* u_dt, double* v_dt, double* pt_dt,
     const double* u_t1, const double* v_t1, const double* pt_t1,

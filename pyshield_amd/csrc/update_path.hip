@@ -20,10 +20,16 @@ constexpr int kBlock = 256;
 
 struct Idx {
   int64_t ni, nj, nk, si, sj;
+  int64_t si_i, sj_i;  // strides of the nk+1-level interface field
   __device__ __forceinline__ int64_t off(int64_t cell) const {
     const int64_t col = cell / nk, k = cell - col * nk;
     const int64_t i = col / nj, j = col - i * nj;
     return i * si + j * sj + k;
+  }
+  __device__ __forceinline__ int64_t off_i(int64_t cell) const {
+    const int64_t col = cell / nk, k = cell - col * nk;
+    const int64_t i = col / nj, j = col - i * nj;
+    return i * si_i + j * sj_i + k;
   }
 };
 
@@ -34,8 +40,8 @@ __global__ __launch_bounds__(kBlock) void prepare_mp_kernel(Idx ix, double* dz, 
                                                             double* t7, double* t8, double* t9) {
   const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (cell >= ix.ni * ix.nj * ix.nk) return;
-  const int64_t o = ix.off(cell);
-  dz[o] = (phii[o + 1] - phii[o]) * RGRAV;
+  const int64_t o = ix.off(cell), oi = ix.off_i(cell);
+  dz[o] = (phii[oi + 1] - phii[oi]) * RGRAV;
   const double zvir = RVGAS / RDGAS - 1.0;
   wmp[o] = -omga[o] * (1.0 + zvir * qv[o]) * pt[o] / delp[o] * (RDGAS * RGRAV);
   t0[o] = 0.0; t1[o] = 0.0; t2[o] = 0.0; t3[o] = 0.0; t4[o] = 0.0;
@@ -87,11 +93,11 @@ struct TracerArgs {
 __global__ __launch_bounds__(kBlock) void update_tracers_kernel(Idx ix, TracerArgs a, double rdt) {
   const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (cell >= ix.ni * ix.nj * ix.nk) return;
-  const int64_t o = ix.off(cell);
+  const int64_t o = ix.off(cell), oi = ix.off_i(cell);
   a.u_dt[o] += (a.u_t1[o] - a.u_t0[o]) * rdt;
   a.v_dt[o] += (a.v_t1[o] - a.v_t0[o]) * rdt;
   a.pt_dt[o] += (a.pt_t1[o] - a.pt_t0[o]) * rdt;
-  const double dp = a.prsi[o + 1] - a.prsi[o];
+  const double dp = a.prsi[oi + 1] - a.prsi[oi];
   const double qwat_qv = dp * a.qv_t1[o];
   const double qwat_ql = dp * a.ql_t1[o];
   const double qwat_qr = dp * a.qr_t1[o];
@@ -111,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void update_tracers_kernel(Idx ix, TracerAr
 }
 
 bool shape_ok(const shield_mp_shape* s) {
-  return s && s->ni >= 0 && s->nj >= 0 && s->nk >= 1 && s->si >= 0 && s->sj >= 0;
+  return s && s->ni >= 0 && s->nj >= 0 && s->nk >= 1 && s->si >= 0 && s->sj >= 0 && s->si2 >= 0 && s->sj2 >= 0;
 }
 
 unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -131,7 +137,7 @@ int shield_prepare_microphysics(const shield_mp_shape* shape, double* dz, const 
     if (!tend[n]) return SHIELD_ERR_NULL;
   const int64_t n = shape->ni * shape->nj * shape->nk;
   if (n == 0) return SHIELD_OK;
-  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj};
+  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj, shape->si2, shape->sj2};
   hipLaunchKernelGGL(prepare_mp_kernel, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, ix, dz, phii, wmp,
                      omga, qvapor, pt, delp, tend[0], tend[1], tend[2], tend[3], tend[4], tend[5], tend[6], tend[7],
                      tend[8], tend[9]);
@@ -153,7 +159,7 @@ int shield_update_physics_state_with_tendencies(const shield_mp_shape* shape, co
   }
   const int64_t n = shape->ni * shape->nj * shape->nk;
   if (n == 0) return SHIELD_OK;
-  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj};
+  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj, shape->si2, shape->sj2};
   hipLaunchKernelGGL(forward_euler_kernel, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, ix, a, b, c, dt);
   return finish();
 }
@@ -163,7 +169,7 @@ int shield_fill_gfs_delp(const shield_mp_shape* shape, const double* delp, doubl
   if (!delp || !q) return SHIELD_ERR_NULL;
   const int64_t ncol = shape->ni * shape->nj;
   if (ncol == 0) return SHIELD_OK;
-  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj};
+  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj, shape->si2, shape->sj2};
   hipLaunchKernelGGL(fill_gfs_delp_kernel, dim3(grid_for(ncol)), dim3(kBlock), 0, (hipStream_t)stream, ix, delp, q,
                      q_min);
   return finish();
@@ -184,7 +190,7 @@ int shield_prepare_tendencies_and_update_tracers(
     if (!p) return SHIELD_ERR_NULL;
   const int64_t n = shape->ni * shape->nj * shape->nk;
   if (n == 0) return SHIELD_OK;
-  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj};
+  const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj, shape->si2, shape->sj2};
   hipLaunchKernelGGL(update_tracers_kernel, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, ix, a, rdt);
   return finish();
 }

@@ -1,0 +1,106 @@
+"""The column-local state-update path around the microphysics, on the GPU.
+
+Reference names and argument meaning:
+prepare_microphysics and update_physics_state_with_tendencies
+(pySHiELD/stencils/physics.py:105-194); fill_gfs_delp and
+prepare_tendencies_and_update_tracers (pySHiELD/update/update_atmos_state.py
+:21-92).  Fields are float64 GPU tensors [X, Y, Z] (k contiguous, k = 0 at the
+model top); interface fields (phii, prsi) have one more level.  Each function
+launches one kernel through the C ABI on torch's current stream and updates
+its outputs in place.
+"""
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+def _shape(t, iface=None):
+    """Shape of the nk-level fields like t; si2/sj2 carry the strides of the
+    nk+1-level interface field iface (phii or prsi) when there is one."""
+    if t.dim() != 3 or t.stride(2) != 1:
+        raise ValueError("expected a [X, Y, Z] tensor with contiguous k")
+    ni, nj, nk = t.shape
+    si2, sj2 = (iface.stride(0), iface.stride(1)) if iface is not None else (0, 0)
+    return _lib.ShieldMpShape(ni, nj, nk, t.stride(0), t.stride(1), si2, sj2)
+
+
+def _check_iface(iface, like):
+    if (iface.dtype != torch.float64 or not iface.is_cuda or iface.dim() != 3
+            or iface.shape[:2] != like.shape[:2] or iface.shape[2] != like.shape[2] + 1
+            or iface.stride(2) != 1):
+        raise ValueError("the interface field must be float64 on the GPU with nk+1 contiguous levels")
+
+
+def _check(*ts, like):
+    for t in ts:
+        if t.dtype != torch.float64 or not t.is_cuda:
+            raise TypeError("fields must be float64 GPU tensors")
+        if t.stride()[:2] != like.stride()[:2] or t.shape[:2] != like.shape[:2]:
+            raise ValueError("fields must share the i/j shape and strides")
+
+
+def _stream(t):
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def prepare_microphysics(dz, phii, wmp, omga, qvapor, pt, delp, u_dt, v_dt, pt_dt, qv_dt, ql_dt,
+                         qr_dt, qi_dt, qs_dt, qg_dt, qa_dt):
+    """dz, wmp from the geopotential and omega; the ten tendencies zeroed."""
+    tend = (u_dt, v_dt, pt_dt, qv_dt, ql_dt, qr_dt, qi_dt, qs_dt, qg_dt, qa_dt)
+    _check(dz, wmp, omga, qvapor, pt, delp, *tend, like=pt)
+    _check_iface(phii, pt)
+    arr = (ctypes.c_void_p * 10)(*[t.data_ptr() for t in tend])
+    rc = _lib.lib().shield_prepare_microphysics(ctypes.byref(_shape(pt, phii)), _p(dz), _p(phii), _p(wmp), _p(omga),
+                                                _p(qvapor), _p(pt), _p(delp), arr, _stream(pt))
+    _lib.check(rc, "shield_prepare_microphysics")
+
+
+def update_physics_state_with_tendencies(qvapor, qliquid, qrain, qice, qsnow, qgraupel, qcld, pt, ua, va,
+                                         qv_dt, ql_dt, qr_dt, qi_dt, qs_dt, qg_dt, qa_dt, pt_dt, udt, vdt,
+                                         physics_updated_specific_humidity, physics_updated_qliquid,
+                                         physics_updated_qrain, physics_updated_qice, physics_updated_qsnow,
+                                         physics_updated_qgraupel, physics_updated_cloud_fraction,
+                                         physics_updated_pt, physics_updated_ua, physics_updated_va, dt):
+    """physics_updated_X = X + X_dt * dt for the ten fields."""
+    ins = (qvapor, qliquid, qrain, qice, qsnow, qgraupel, qcld, pt, ua, va)
+    tds = (qv_dt, ql_dt, qr_dt, qi_dt, qs_dt, qg_dt, qa_dt, pt_dt, udt, vdt)
+    outs = (physics_updated_specific_humidity, physics_updated_qliquid, physics_updated_qrain,
+            physics_updated_qice, physics_updated_qsnow, physics_updated_qgraupel,
+            physics_updated_cloud_fraction, physics_updated_pt, physics_updated_ua, physics_updated_va)
+    _check(*ins, *tds, *outs, like=pt)
+    mk = lambda ts: (ctypes.c_void_p * 10)(*[t.data_ptr() for t in ts])  # noqa: E731
+    rc = _lib.lib().shield_update_physics_state_with_tendencies(ctypes.byref(_shape(pt)), mk(ins), mk(tds),
+                                                                mk(outs), float(dt), _stream(pt))
+    _lib.check(rc, "shield_update_physics_state_with_tendencies")
+
+
+def fill_gfs_delp(delp, q, q_min: float):
+    """Fill q below q_min from the layer below, in place (nk layers)."""
+    _check(delp, q, like=q)
+    rc = _lib.lib().shield_fill_gfs_delp(ctypes.byref(_shape(q)), _p(delp), _p(q), float(q_min), _stream(q))
+    _lib.check(rc, "shield_fill_gfs_delp")
+
+
+def prepare_tendencies_and_update_tracers(u_dt, v_dt, pt_dt, u_t1, v_t1, physics_updated_pt,
+                                          physics_updated_specific_humidity, physics_updated_qliquid,
+                                          physics_updated_qrain, physics_updated_qsnow, physics_updated_qice,
+                                          physics_updated_qgraupel, u_t0, v_t0, pt_t0, qvapor_t0, qliquid_t0,
+                                          qrain_t0, qsnow_t0, qice_t0, qgraupel_t0, prsi, delp, rdt: float):
+    """Accumulate u/v/pt tendencies and convert GFS to FV3 tracers in place."""
+    layer = (u_dt, v_dt, pt_dt, u_t1, v_t1, physics_updated_pt, physics_updated_specific_humidity,
+             physics_updated_qliquid, physics_updated_qrain, physics_updated_qsnow, physics_updated_qice,
+             physics_updated_qgraupel, u_t0, v_t0, pt_t0, qvapor_t0, qliquid_t0, qrain_t0, qsnow_t0, qice_t0,
+             qgraupel_t0)
+    _check(*layer, delp, like=delp)
+    _check_iface(prsi, delp)
+    args = [_p(t) for t in layer] + [_p(prsi), _p(delp)]
+    rc = _lib.lib().shield_prepare_tendencies_and_update_tracers(ctypes.byref(_shape(delp, prsi)), *args, float(rdt),
+                                                                 _stream(delp))
+    _lib.check(rc, "shield_prepare_tendencies_and_update_tracers")
