@@ -42,7 +42,21 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only-kernel", action="store_true",
                     help="launch only the timed kernel (for profiler runs)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="instead of the benchmark, copy one buffer of this many bytes "
+                         "(calibrates the HBM byte counters of a profiler run)")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8, "note": "one device-to-device copy; reads and writes n bytes"}))
 
 
 def load_state(name, device):
@@ -120,6 +134,9 @@ def cpu_baseline(name, seconds):
 
 def main():
     args = parse()
+    if args.copy_bytes:
+        copy_calibration(args.copy_bytes)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the microphysics shards by columns with no "
               f"exchange, so --gpus {args.gpus} is not run here", file=sys.stderr)
