@@ -102,6 +102,8 @@ struct Coef {
   R vs_fac, vg_fac, vi_fac, vr_fac, vs_max, vg_max, vi_max, vr_max;
   int32_t const_vi, const_vs, const_vg, const_vr, fast_sat_adj, do_qa, rad_snow, rad_rain;
   int32_t use_ccn, pad_;
+  // FP64 copies of the thresholds the state is tested against
+  double t_sub_d, t_wfr_d, qc_crt_d;
 };
 
 template <class R>
@@ -131,6 +133,7 @@ inline Coef<R> make_coef(const shield_mp_params& p) {
   c.const_vi = n.const_vi; c.const_vs = n.const_vs; c.const_vg = n.const_vg; c.const_vr = n.const_vr;
   c.fast_sat_adj = n.fast_sat_adj; c.do_qa = n.do_qa; c.rad_snow = n.rad_snow; c.rad_rain = n.rad_rain;
   c.use_ccn = p.use_ccn; c.pad_ = 0;
+  c.t_sub_d = n.t_sub; c.t_wfr_d = p.t_wfr; c.qc_crt_d = n.qc_crt;
   return c;
 }
 

@@ -5,9 +5,11 @@
 // [warm_rain(first), sedimentation, warm_rain(second), icloud], then
 // fields_update.  The reference runs five GT4Py stencils (~100 generated
 // kernels per sub-step, 43 persistent 3D scratch fields); here every
-// intermediate lives in registers and LDS, so HBM sees only the compulsory
-// traffic: 13 FP64 fields read and 11 written per cell (192 B/cell at the
-// default namelist) plus 6 FP64 values per column.
+// intermediate lives in registers and LDS.  HBM sees the compulsory traffic
+// (13 FP64 fields read and 11 written per cell at the default namelist, plus
+// 6 FP64 values per column) and re-reads of inputs that are cheaper to fetch
+// again than to hold: pt and delz in the sub-steps, and the state and winds
+// in fields_update.
 //
 // Layout: see mp_wave.h.  One wavefront (= one workgroup) owns `ncols`
 // consecutive columns spread over S slots of 64 lanes.
@@ -17,59 +19,114 @@
 namespace shield {
 
 enum { CV_HVAR, CV_RHADJ, CV_RHRAIN, CV_CCN, CV_CPRAUT, CV_RAIN, CV_SNOW, CV_ICE, CV_GRAUPEL, CV_N };
-constexpr int PMAX = 64;  // max columns per wavefront
+// per-cell values kept in LDS (each lane touches only its own cells)
+enum { CL_M1, CL_M1SOL, CL_DZ1, CL_TMP, CL_N };
 
+// most columns one wavefront owns for a slot count
+template <int S>
+constexpr int pmax() {
+  return S >= 5 ? 16 : 64;
+}
+
+// Per-cell state that stays in registers for the whole call.
 template <int S, class R>
 struct St {
-  double qv[S], ql[S], qr[S], qi[S], qs[S], qg[S], tz[S];
-  double t0[S], dz0[S], dp1[S], dp0[S], den[S], denfac[S], dz1[S];
-  double m1[S], m1_sol[S], qaz[S], w[S];
-  R vtr[S], vtg[S], vts[S];
+  double qv[S], ql[S], qr[S], qi[S], qs[S], qg[S], tz[S];  // prognostic (dry mixing ratios)
+  double qaz[S];                                            // cloud-fraction accumulator
+  double dp1[S];                                            // dry air mass (Pa)
+  double den[S], denfac[S];                                 // air density of the sub-step
+  R vtr[S], vtg[S], vts[S];                                 // fall speeds for icloud
 };
 
 template <int S>
 struct Shared {
-  double buf[64 * S];
-  double buf2[64 * S];
-  double colv[CV_N][PMAX];
+  double buf[64 * S];          // neighbour exchange
+  double cell[CL_N][64 * S];   // m1, m1_sol, dz1, a temporary
+  double colv[CV_N][pmax<S>()];
 };
 
+// ------------------------------------------------------- slot selection
+// The heavy point-wise stages run as a runtime loop over the S slots (one
+// copy of the process code instead of S).  Each iteration moves one slot's
+// cell out of the register arrays and back with select chains on static
+// indices.  launder() hides each element's origin from the optimiser (an
+// empty asm that claims to rewrite it), so the select chain cannot be folded
+// into a load through a computed address, which would move the array to
+// scratch memory.
+__device__ __forceinline__ double launder(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ float launder(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ int launder(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ int64_t launder(int64_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ bool launder(bool v) { return launder(int(v)) != 0; }
+template <int S, class T>
+__device__ __forceinline__ T pick(const T (&a)[S], int s) {
+  T v = launder(a[0]);
+#pragma unroll
+  for (int t = 1; t < S; ++t) v = (s == t) ? launder(a[t]) : v;
+  return v;
+}
+template <int S, class T>
+__device__ __forceinline__ void put(T (&a)[S], int s, T v) {
+#pragma unroll
+  for (int t = 0; t < S; ++t) a[t] = (s == t) ? v : launder(a[t]);
+}
 template <int S, class R>
-__device__ __forceinline__ Cell load_cell(const St<S, R>& X, int s) {
+__device__ __forceinline__ Cell pick_cell(const St<S, R>& X, int s) {
   Cell c;
-  c.qv = X.qv[s];
-  c.ql = X.ql[s];
-  c.qr = X.qr[s];
-  c.qi = X.qi[s];
-  c.qs = X.qs[s];
-  c.qg = X.qg[s];
-  c.tz = X.tz[s];
-  c.qa = X.qaz[s];
+  c.qv = pick(X.qv, s);
+  c.ql = pick(X.ql, s);
+  c.qr = pick(X.qr, s);
+  c.qi = pick(X.qi, s);
+  c.qs = pick(X.qs, s);
+  c.qg = pick(X.qg, s);
+  c.tz = pick(X.tz, s);
+  c.qa = pick(X.qaz, s);
   return c;
 }
 template <int S, class R>
-__device__ __forceinline__ void store_cell(St<S, R>& X, int s, const Cell& c) {
-  X.qv[s] = c.qv;
-  X.ql[s] = c.ql;
-  X.qr[s] = c.qr;
-  X.qi[s] = c.qi;
-  X.qs[s] = c.qs;
-  X.qg[s] = c.qg;
-  X.tz[s] = c.tz;
-  X.qaz[s] = c.qa;
+__device__ __forceinline__ void put_cell(St<S, R>& X, int s, const Cell& c) {
+  put(X.qv, s, c.qv);
+  put(X.ql, s, c.ql);
+  put(X.qr, s, c.qr);
+  put(X.qi, s, c.qi);
+  put(X.qs, s, c.qs);
+  put(X.qg, s, c.qg);
+  put(X.tz, s, c.tz);
+  put(X.qaz, s, c.qa);
 }
+
+// Global element offsets of this wavefront's cells.
+template <int S>
+struct Glob {
+  int64_t off[S];
+};
 
 // den0 = -dp1 / (g dz0) (microphysics.py:149), recomputed where needed
 __device__ __forceinline__ double dry_density(double dp1, double dz0) { return -dp1 / (GRAV * dz0); }
 
 // Layer-bottom heights ze (BACKWARD, microphysics.py:373-380 / :756-765) and
 // layer thicknesses dz = ze[k] - ze[k+1], dz[nz-1] = ze - zs (:445-452).
-template <int S, class R>
-__device__ __forceinline__ void layer_dz(const Lay<S>& L, const St<S, R>& X, Shared<S>& sh, double zs,
-                                         double (&dz)[S]) {
+template <int S>
+__device__ __forceinline__ void layer_dz(const Lay<S>& L, Shared<S>& sh, double zs, double (&dz)[S]) {
+  const int lane = lane_id();
   double b[S], ze[S], zn[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) b[s] = (L.kb[s] == 0) ? zs - X.dz1[s] : -X.dz1[s];
+  for (int s = 0; s < S; ++s) {
+    const double dz1 = sh.cell[CL_DZ1][64 * s + lane];
+    b[s] = (L.kb[s] == 0) ? zs - dz1 : -dz1;
+  }
   sum_bwd(L, b, ze);
   col_next(sh.buf, ze, zn);
 #pragma unroll
@@ -80,16 +137,16 @@ __device__ __forceinline__ void layer_dz(const Lay<S>& L, const St<S, R>& X, Sha
 // shared by microphysics.py:454-515, :825-900, :956-1035, :1089-1168).
 // q holds dry mixing ratios.  Only columns with fall[s] change; flux gets the
 // mass flux through each layer bottom (0 in non-falling columns) and surf
-// the surface flux of the cell's column.  w is transported when do_sedi_w.
+// the surface flux of the cell's column.  When do_sedi_w the vertical
+// velocity, which stays in HBM, is transported with the falling mass.
 template <int S, class R>
-__device__ __forceinline__ void implicit_fall(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const double (&dz)[S],
-                                              double (&q)[S], const R (&vt)[S], double dt, const bool (&fall)[S],
-                                              bool do_sedi_w, double (&flux)[S], double (&surf)[S]) {
-  double dm[S], dd[S], qq[S], a[S], b[S], qm[S], ddp[S];
+__device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                              const shield_mp_fields& f, const double (&dz)[S], double (&q)[S],
+                                              const R (&vt)[S], double dt, const bool (&fall)[S], bool do_sedi_w,
+                                              double (&flux)[S], double (&surf)[S]) {
+  double dd[S], qq[S], a[S], b[S], qm[S], ddp[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (do_sedi_w)
-      dm[s] = X.dp1[s] * (1.0 + X.qv[s] + X.ql[s] + X.qr[s] + X.qi[s] + X.qs[s] + X.qg[s]);
     dd[s] = dt * double(vt[s]);
     qq[s] = q[s] * X.dp1[s];
   }
@@ -108,6 +165,28 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, St<S, R>& X, Shar
   }
   sum_fwd(L, b, flux);
   col_bottom(L, sh.buf, flux, surf);
+  if (do_sedi_w) {
+    // dm uses the mixing ratios before the fall (:478-486)
+    double fp[S], vp[S], vd[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      vd[s] = double(vt[s]);
+      if (!fall[s]) flux[s] = 0.0;
+    }
+    col_prev(sh.buf, flux, fp);
+    col_prev(sh.buf, vd, vp);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!fall[s] || !L.v[s]) continue;
+      const double dm = X.dp1[s] * (1.0 + X.qv[s] + X.ql[s] + X.qr[s] + X.qi[s] + X.qs[s] + X.qg[s]);
+      const int64_t o = G.off[s];
+      const double w = f.w[o];
+      if (L.k[s] == 0)
+        f.w[o] = (dm * w + flux[s] * vd[s]) / (dm - flux[s]);
+      else
+        f.w[o] = (dm * w - fp[s] * vp[s] + flux[s] * vd[s]) / (dm + fp[s] - flux[s]);
+    }
+  }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (fall[s]) {
@@ -117,21 +196,6 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, St<S, R>& X, Shar
       surf[s] = 0.0;
     }
   }
-  if (do_sedi_w) {
-    double fp[S], vp[S], vd[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) vd[s] = double(vt[s]);
-    col_prev(sh.buf, flux, fp);
-    col_prev(sh.buf, vd, vp);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (!fall[s]) continue;
-      if (L.k[s] == 0)
-        X.w[s] = (dm[s] * X.w[s] + flux[s] * vd[s]) / (dm[s] - flux[s]);
-      else
-        X.w[s] = (dm[s] * X.w[s] - fp[s] * vp[s] + flux[s] * vd[s]) / (dm[s] + fp[s] - flux[s]);
-    }
-  }
 }
 
 // Sedimentation heating, explicit at the top and implicit below
@@ -139,11 +203,12 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, St<S, R>& X, Shar
 template <int S, class R>
 __device__ __forceinline__ void sedi_heat(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const double (&flux)[S],
                                           double cq, const bool (&apply)[S]) {
+  const int lane = lane_id();
   double fp[S], a[S], b[S], tn[S];
   col_prev(sh.buf, flux, fp);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const double dgz = -0.5 * GRAV * X.dz1[s];
+    const double dgz = -0.5 * GRAV * sh.cell[CL_DZ1][64 * s + lane];
     const double cvn = X.dp1[s] * (CV_AIR + X.qv[s] * CV_VAP + (X.qr[s] + X.ql[s]) * C_LIQ +
                                    (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE);
     if (L.k[s] == 0) {
@@ -190,16 +255,21 @@ __device__ __forceinline__ void slope_limiter(const Lay<S>& L, Shared<S>& sh, co
 
 // ------------------------------------------------------ warm_rain (:279-653)
 template <int S, class R>
-__device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const shield_mp_params& P,
-                                          const Coef<R>& c, bool is_first) {
-  const int nz = L.nz;
-  (void)nz;
+__device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                          const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c,
+                                          bool is_first) {
+  const int lane = lane_id();
   const double t_wfr = P.t_wfr;
   if (is_first) {
+    // air density from hydrostatic balance (:331-341); t0 and dz0 are the
+    // call's input pt and delz
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      X.dz1[s] = X.dz0[s] * X.tz[s] / X.t0[s];
-      X.den[s] = dry_density(X.dp1[s], X.dz0[s]) * X.dz0[s] / X.dz1[s];
+      const double t0 = L.v[s] ? f.pt[G.off[s]] : 250.0;
+      const double dz0 = L.v[s] ? f.delz[G.off[s]] : -100.0;
+      const double dz1 = dz0 * X.tz[s] / t0;
+      sh.cell[CL_DZ1][64 * s + lane] = dz1;
+      X.den[s] = dry_density(X.dp1[s], dz0) * dz0 / dz1;
       X.denfac[s] = sqrt(SFCRHO / X.den[s]);
     }
   }
@@ -217,72 +287,83 @@ __device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S
   }
   const R dt5 = R(0.5) * c.dt_rain;
   if (wave_any(fall)) {
-    double dz[S];
-    layer_dz(L, X, sh, P.zs, dz);
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < S; ++s) {
-      if (fall[s]) {
-        Cell cs = load_cell(X, s);
-        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][L.cl[s]]), cs, R(X.den[s]), R(X.denfac[s]));
-        store_cell(X, s, cs);
+      if (pick(fall, s)) {
+        Cell cs = pick_cell(X, s);
+        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][pick(L.cl, s)]), cs, pick(X.den, s),
+                      R(pick(X.denfac, s)));
+        put_cell(X, s, cs);
       }
     }
-    implicit_fall(L, X, sh, dz, X.qr, X.vtr, P.dt_rain, fall, P.do_sedi_w != 0, flux, surf);
+    double dz[S];
+    layer_dz(L, sh, P.zs, dz);
+    implicit_fall(L, X, sh, G, f, dz, X.qr, X.vtr, P.dt_rain, fall, P.do_sedi_w != 0, flux, surf);
     if (P.nml.do_sedi_heat) sedi_heat(L, X, sh, flux, C_LIQ, fall);
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < S; ++s) {
-      if (fall[s]) {
-        Cell cs = load_cell(X, s);
-        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][L.cl[s]]), cs, R(X.den[s]), R(X.denfac[s]));
-        store_cell(X, s, cs);
+      if (pick(fall, s)) {
+        Cell cs = pick_cell(X, s);
+        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][pick(L.cl, s)]), cs, pick(X.den, s),
+                      R(pick(X.denfac, s)));
+        put_cell(X, s, cs);
       }
     }
   }
   // auto-conversion (:587-645)
   if (P.nml.irain_f != 0) {
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < S; ++s) {
-      Cell cs = load_cell(X, s);
-      autoconv_no_subgrid_var<R>(c, t_wfr, cs, R(X.den[s]), R(sh.colv[CV_CCN][L.cl[s]]),
-                                 R(sh.colv[CV_CPRAUT][L.cl[s]]));
-      store_cell(X, s, cs);
+      const int cl = pick(L.cl, s);
+      Cell cs = pick_cell(X, s);
+      autoconv_no_subgrid_var<R>(c, t_wfr, cs, R(pick(X.den, s)), R(sh.colv[CV_CCN][cl]),
+                                 R(sh.colv[CV_CPRAUT][cl]));
+      put_cell(X, s, cs);
     }
   } else {
     double dl[S];
     if (P.nml.z_slope_liq) slope_limiter(L, sh, X.ql, dl);
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < S; ++s) {
-      const double hv = sh.colv[CV_HVAR][L.cl[s]];
+      const int cl = pick(L.cl, s);
+      const double hv = sh.colv[CV_HVAR][cl];
+      Cell cs = pick_cell(X, s);
       double d;
       if (P.nml.z_slope_liq)
-        d = mmax(dl[s], mmax(QVMIN, hv * X.ql[s]));
+        d = mmax(pick(dl, s), mmax(QVMIN, hv * cs.ql));
       else
-        d = mmax(QVMIN, hv * X.ql[s]);
-      Cell cs = load_cell(X, s);
-      autoconv_subgrid_var<R>(c, t_wfr, cs, R(X.den[s]), R(sh.colv[CV_CCN][L.cl[s]]),
-                              R(sh.colv[CV_CPRAUT][L.cl[s]]), R(d));
-      store_cell(X, s, cs);
+        d = mmax(QVMIN, hv * cs.ql);
+      autoconv_subgrid_var<R>(c, t_wfr, cs, R(pick(X.den, s)), R(sh.colv[CV_CCN][cl]),
+                              R(sh.colv[CV_CPRAUT][cl]), R(d));
+      put_cell(X, s, cs);
     }
   }
   // rain += r1; m1 += m1_rain (+ m1_sol)  (:647-653)
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (L.v[s] && L.kb[s] == 0) sh.colv[CV_RAIN][L.cl[s]] += surf[s];
-    X.m1[s] = is_first ? X.m1[s] + flux[s] : X.m1[s] + flux[s] + X.m1_sol[s];
+    double& m1 = sh.cell[CL_M1][64 * s + lane];
+    m1 = is_first ? m1 + flux[s] : m1 + flux[s] + sh.cell[CL_M1SOL][64 * s + lane];
   }
   __syncthreads();
 }
 
 // ---------------------------------------------------- sedimentation (:656-1217)
 template <int S, class R>
-__device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shared<S>& sh,
-                                              const shield_mp_params& P, const Coef<R>& c) {
+__device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                              const shield_mp_fields& f, const shield_mp_params& P,
+                                              const Coef<R>& c) {
+  const int lane = lane_id();
   const double tice_d = P.nml.tice;
   R vti[S];
-#pragma unroll
+#pragma unroll 1
   for (int s = 0; s < S; ++s) {
-    fall_speed<R>(c, R(X.qg[s]), R(X.qi[s]), R(X.qs[s]), R(X.tz[s]), R(X.den[s]), X.vtg[s], vti[s], X.vts[s]);
-    X.m1_sol[s] = 0.0;
+    R vg, vi, vs;
+    fall_speed<R>(c, R(pick(X.qg, s)), R(pick(X.qi, s)), R(pick(X.qs, s)), R(pick(X.tz, s)), R(pick(X.den, s)),
+                  vg, vi, vs);
+    put(X.vtg, s, vg);
+    put(vti, s, vi);
+    put(X.vts, s, vs);
   }
   // melting of cloud ice at and below the first level warmer than tice, and
   // always in the surface layer (:702-746)
@@ -310,21 +391,21 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
       X.tz[s] = X.tz[s] - double(R(sink) * lhi / cvm);
     }
   }
-  double dz[S], flux[S], surf[S], rsurf[S];
+  double dz[S], flux[S], surf[S], m1_sol[S];
   bool fall[S];
-  layer_dz(L, X, sh, P.zs, dz);
+#pragma unroll
+  for (int s = 0; s < S; ++s) m1_sol[s] = 0.0;
+  layer_dz(L, sh, P.zs, dz);
+  const bool sedi_w = P.do_sedi_w != 0;
   // cloud ice (:773-900)
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    pred[s] = L.v[s] && (X.qi[s] > QRMIN);
-    rsurf[s] = 0.0;
-  }
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qi[s] > QRMIN);
   col_any(L, pred, fall);
   if (!(P.nml.vi_fac < 1.0e-5) && wave_any(fall)) {
-    implicit_fall(L, X, sh, dz, X.qi, vti, P.dts, fall, P.do_sedi_w != 0, flux, surf);
+    implicit_fall(L, X, sh, G, f, dz, X.qi, vti, P.dts, fall, sedi_w, flux, surf);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      X.m1_sol[s] = flux[s];
+      m1_sol[s] = flux[s];
       if (L.v[s] && L.kb[s] == 0) sh.colv[CV_ICE][L.cl[s]] += surf[s];
     }
   }
@@ -333,10 +414,10 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
   for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qs[s] > QRMIN);
   col_any(L, pred, fall);
   if (wave_any(fall)) {
-    implicit_fall(L, X, sh, dz, X.qs, X.vts, P.dts, fall, P.do_sedi_w != 0, flux, surf);
+    implicit_fall(L, X, sh, G, f, dz, X.qs, X.vts, P.dts, fall, sedi_w, flux, surf);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      X.m1_sol[s] = X.m1_sol[s] + flux[s];
+      m1_sol[s] = m1_sol[s] + flux[s];
       if (L.v[s] && L.kb[s] == 0) sh.colv[CV_SNOW][L.cl[s]] += surf[s];
     }
   }
@@ -345,10 +426,10 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
   for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qg[s] > QRMIN);
   col_any(L, pred, fall);
   if (wave_any(fall)) {
-    implicit_fall(L, X, sh, dz, X.qg, X.vtg, P.dts, fall, P.do_sedi_w != 0, flux, surf);
+    implicit_fall(L, X, sh, G, f, dz, X.qg, X.vtg, P.dts, fall, sedi_w, flux, surf);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      X.m1_sol[s] = X.m1_sol[s] + flux[s];
+      m1_sol[s] = m1_sol[s] + flux[s];
       if (L.v[s] && L.kb[s] == 0) sh.colv[CV_GRAUPEL][L.cl[s]] += surf[s];
     }
   }
@@ -357,67 +438,67 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
     bool all[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) all[s] = L.v[s];
-    sedi_heat(L, X, sh, X.m1_sol, C_ICE, all);
+    sedi_heat(L, X, sh, m1_sol, C_ICE, all);
   }
+#pragma unroll
+  for (int s = 0; s < S; ++s) sh.cell[CL_M1SOL][64 * s + lane] = m1_sol[s];
 }
 
 // ------------------------------------------------------------ icloud (:1220-1449)
 template <int S, class R>
-__device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const shield_mp_params& P,
-                                       const Coef<R>& c) {
-  const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr;
-  R q_liq[S], q_sol[S], cvm[S];
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const R lhi = R(LI00) + R(DC_ICE) * R(X.tz[s]);
-    q_liq[s] = R(X.ql[s]) + R(X.qr[s]);
-    q_sol[s] = R(X.qi[s]) + R(X.qs[s]) + R(X.qg[s]);
-    cvm[s] = c.c_air + R(X.qv[s]) * c.c_vap + q_liq[s] * R(C_LIQ) + q_sol[s] * R(C_ICE);
-    const R icpk = lhi / cvm[s];
-    if ((X.tz[s] > tice_d) && (X.qi[s] > QCMIN)) {
-      // pimlt: instant melting of cloud ice
-      const double melt = mmin(X.qi[s], double(c.fac_imlt * R(X.tz[s] - tice_d) / icpk));
-      const double tmp = mmin(melt, dim(double(P.nml.ql_mlt), X.ql[s]));
-      X.ql[s] = X.ql[s] + tmp;
-      X.qr[s] = X.qr[s] + melt - tmp;
-      X.qi[s] = X.qi[s] - melt;
-      q_liq[s] = q_liq[s] + R(melt);
-      q_sol[s] = q_sol[s] - R(melt);
-      cvm[s] = c.c_air + R(X.qv[s]) * c.c_vap + q_liq[s] * R(C_LIQ) + q_sol[s] * R(C_ICE);
-      X.tz[s] = X.tz[s] - double(R(melt) * lhi / cvm[s]);
-    } else if ((X.tz[s] < t_wfr_d) && (X.ql[s] > QCMIN)) {
-      // pihom: homogeneous freezing of cloud water
-      const R dtmp = R(t_wfr_d - X.tz[s]);
-      const R factor = mmin(R(1.0), dtmp / R(DT_FR));
-      const double sink = mmin(X.ql[s] * double(factor), double(dtmp / icpk));
-      const R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * R(tice_d - X.tz[s])) / R(X.den[s]);
-      const double tmp = mmin(sink, dim(double(qi_crt), X.qi[s]));
-      X.ql[s] = X.ql[s] - sink;
-      X.qs[s] = X.qs[s] + sink - tmp;
-      X.qi[s] = X.qi[s] + tmp;
-      q_liq[s] = q_liq[s] - R(sink);
-      q_sol[s] = q_sol[s] + R(sink);
-      cvm[s] = c.c_air + R(X.qv[s]) * c.c_vap + q_liq[s] * R(C_LIQ) + q_sol[s] * R(C_ICE);
-      X.tz[s] = X.tz[s] + double(R(sink) * lhi / cvm[s]);
-    }
-  }
+__device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                       const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c) {
+  const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr, ql_mlt = P.nml.ql_mlt;
+  const bool slope = P.nml.z_slope_ice != 0;
+  const int lane = lane_id();
+  // With the ice slope limiter the whole column's melt / freeze must be done
+  // before the limiter sees cloud ice; the heat-capacity terms it leaves are
+  // parked in LDS cells that are free during icloud (m1_sol is consumed by
+  // the second warm_rain, dz1 is rebuilt by the next sub-step).
+  double* q_liq_l = sh.cell[CL_M1SOL];
+  double* q_sol_l = sh.cell[CL_DZ1];
+  double* cvm_l = sh.cell[CL_TMP];
   double dl[S];
-  if (P.nml.z_slope_ice) slope_limiter(L, sh, X.qi, dl);
-#pragma unroll
+  if (slope) {
+#pragma unroll 1
+    for (int s = 0; s < S; ++s) {
+      Cell cs = pick_cell(X, s);
+      R q_liq, q_sol, cvm;
+      icloud_melt_freeze<R>(c, tice_d, t_wfr_d, ql_mlt, cs, pick(X.den, s), q_liq, q_sol, cvm);
+      put_cell(X, s, cs);
+      q_liq_l[64 * s + lane] = double(q_liq);
+      q_sol_l[64 * s + lane] = double(q_sol);
+      cvm_l[64 * s + lane] = double(cvm);
+    }
+    slope_limiter(L, sh, X.qi, dl);
+  }
+#pragma unroll 1
   for (int s = 0; s < S; ++s) {
-    const int cl = L.cl[s];
+    const int cl = pick(L.cl, s);
     const double hv = sh.colv[CV_HVAR][cl];
+    const double den = pick(X.den, s);
+    Cell cs = pick_cell(X, s);
+    R q_liq, q_sol, cvm;
     double di;
-    if (P.nml.z_slope_ice)
-      di = mmax(dl[s], mmax(QVMIN, hv * X.qi[s]));
-    else
-      di = mmax(QVMIN, hv * X.qi[s]);
-    const double p1 = dry_density(X.dp1[s], X.dz0[s]) * RDGAS * X.t0[s];
-    Cell cs = load_cell(X, s);
-    icloud_main<R>(c, tice_d, P.tice0, cs, R(hv), R(sh.colv[CV_RHADJ][cl]), R(sh.colv[CV_RHRAIN][cl]),
-                   R(X.den[s]), R(X.denfac[s]), X.vtg[s], X.vtr[s], X.vts[s], p1, R(di), q_liq[s], q_sol[s],
-                   cvm[s]);
-    store_cell(X, s, cs);
+    if (slope) {
+      q_liq = R(q_liq_l[64 * s + lane]);
+      q_sol = R(q_sol_l[64 * s + lane]);
+      cvm = R(cvm_l[64 * s + lane]);
+      di = mmax(pick(dl, s), mmax(QVMIN, hv * cs.qi));
+    } else {
+      icloud_melt_freeze<R>(c, tice_d, t_wfr_d, ql_mlt, cs, den, q_liq, q_sol, cvm);
+      di = mmax(QVMIN, hv * cs.qi);
+    }
+    // p1 = den0 rdgas t0 from the call's input pt, delz (:149-151)
+    const int64_t o = pick(G.off, s);
+    const bool v = pick(L.v, s);
+    const double t0 = v ? f.pt[o] : 250.0;
+    const double dz0 = v ? f.delz[o] : -100.0;
+    const double p1 = dry_density(pick(X.dp1, s), dz0) * RDGAS * t0;
+    icloud_main<R>(c, tice_d, P.tice0, cs, R(hv), sh.colv[CV_RHADJ][cl], sh.colv[CV_RHRAIN][cl], den,
+                   R(pick(X.denfac, s)), pick(X.vtg, s), pick(X.vtr, s), pick(X.vts, s), p1, R(di), q_liq, q_sol,
+                   cvm);
+    put_cell(X, s, cs);
   }
 }
 
@@ -434,9 +515,9 @@ __global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shi
   const shield_mp_namelist& n = P.nml;
 
   Lay<S> L;
+  Glob<S> G;
   L.nz = nz;
   L.ncols = ncols;
-  int64_t off[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int idx = 64 * s + lane;
@@ -449,7 +530,7 @@ __global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shi
     L.cl[s] = v ? cl : 0;
     const int64_t col = c0 + L.cl[s];
     const int64_t i = col / sh3.nj, j = col - i * sh3.nj;
-    off[s] = i * sh3.si + j * sh3.sj + L.k[s];
+    G.off[s] = i * sh3.si + j * sh3.sj + L.k[s];
   }
 
   // ------------------------------------------- per-column scalars (:173-199)
@@ -476,69 +557,50 @@ __global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shi
   __syncthreads();
 
   // ---------------------------------------------------- fields_init (:101-276)
+  // de_ice (:110-120) only acts when the incoming tendencies are kept: with
+  // overwritten (zero) tendencies qi never exceeds max(qi, qi0_max).  It
+  // rewrites state.qice, state.qsnow and both tendencies in place, as the
+  // reference does; fields_update then reads them back.
+  const bool de_ice = n.de_ice && P.accumulate;
   St<S, R> X;
-  double qv0[S], ql0[S], qr0[S], qi0[S], qs0[S], qg0[S], qa0[S], u0[S], v0[S];
-  double qi_dt_in[S], qs_dt_in[S];  // set only where de_ice rewrote them
-  bool de_iced[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    de_iced[s] = false;
-    qi_dt_in[s] = qs_dt_in[s] = 0.0;
     if (L.v[s]) {
-      const int64_t o = off[s];
+      const int64_t o = G.off[s];
       double qi = f.qi[o], qs = f.qs[o];
-      if (n.de_ice) {
-        const double qi_dt = P.accumulate ? f.qi_dt[o] : 0.0;
-        const double qs_dt = P.accumulate ? f.qs_dt[o] : 0.0;
+      if (de_ice) {
+        const double qi_dt = f.qi_dt[o];
         const double qio = qi - P.timestep * qi_dt;
         const double qin = mmax(qio, n.qi0_max);
         if (qi > qin) {
           qs = qs + qi - qin;
           qi = qin;
           const double dqi = (qin - qio) * P.rdt;
-          // the de_ice tendencies are the starting values the tendency
-          // update below accumulates onto
-          qs_dt_in[s] = qs_dt + qi_dt - dqi;
-          qi_dt_in[s] = dqi;
+          f.qs_dt[o] = f.qs_dt[o] + qi_dt - dqi;
+          f.qi_dt[o] = dqi;
           f.qs[o] = qs;
           f.qi[o] = qi;
-          de_iced[s] = true;
         }
       }
-      X.t0[s] = f.pt[o];
-      X.tz[s] = X.t0[s];
-      X.dp0[s] = f.delp[o];
+      X.tz[s] = f.pt[o];
+      const double dp0 = f.delp[o];
       const double qv = f.qv[o];
-      X.dp1[s] = X.dp0[s] * (1.0 - qv);
-      const double omq = X.dp0[s] / X.dp1[s];
+      X.dp1[s] = dp0 * (1.0 - qv);
+      const double omq = dp0 / X.dp1[s];
       X.qv[s] = qv * omq;
       X.ql[s] = f.ql[o] * omq;
       X.qr[s] = f.qr[o] * omq;
       X.qi[s] = qi * omq;
       X.qs[s] = qs * omq;
       X.qg[s] = f.qg[o] * omq;
-      qa0[s] = f.qa[o];
-      X.dz0[s] = f.delz[o];
-      u0[s] = f.ua[o];
-      v0[s] = f.va[o];
       X.qaz[s] = n.reset_qaz ? 0.0 : f.qaz[o];
-      X.w[s] = P.do_sedi_w ? f.w[o] : 0.0;
     } else {
-      X.t0[s] = X.tz[s] = 250.0;
-      X.dp0[s] = X.dp1[s] = 100.0;
-      X.qv[s] = X.ql[s] = X.qr[s] = X.qi[s] = X.qs[s] = X.qg[s] = 0.0;
-      qa0[s] = u0[s] = v0[s] = X.qaz[s] = X.w[s] = 0.0;
-      X.dz0[s] = -100.0;
+      X.tz[s] = 250.0;
+      X.dp1[s] = 100.0;
+      X.qv[s] = X.ql[s] = X.qr[s] = X.qi[s] = X.qs[s] = X.qg[s] = X.qaz[s] = 0.0;
     }
-    qv0[s] = X.qv[s];
-    ql0[s] = X.ql[s];
-    qr0[s] = X.qr[s];
-    qi0[s] = X.qi[s];
-    qs0[s] = X.qs[s];
-    qg0[s] = X.qg[s];
-    X.m1[s] = 0.0;
-    X.m1_sol[s] = 0.0;
-    X.den[s] = X.denfac[s] = X.dz1[s] = 1.0;
+    X.den[s] = X.denfac[s] = 1.0;
+    X.vtr[s] = X.vtg[s] = X.vts[s] = R(VF_MIN);
   }
   // ccn scaled by the surface-layer density, then c_praut (:174-186)
 #pragma unroll
@@ -547,7 +609,8 @@ __global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shi
       const int cl = L.cl[s];
       double ccn = sh.colv[CV_CCN][cl];
       if (P.use_ccn) {
-        const double p1 = dry_density(X.dp1[s], X.dz0[s]) * RDGAS * X.t0[s];
+        const int64_t o = G.off[s];
+        const double p1 = dry_density(X.dp1[s], f.delz[o]) * RDGAS * f.pt[o];
         ccn = ccn * RDGAS * X.tz[s] / p1;
       }
       sh.colv[CV_CCN][cl] = ccn;
@@ -586,16 +649,16 @@ __global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shi
       }
     }
     // vapour borrowing is a non-linear recurrence: one lane per column walks
-    // its column in LDS (:244-276)
+    // its column in LDS (:244-276); dp1 is staged where m1 lives later
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       sh.buf[64 * s + lane] = X.qv[s];
-      sh.buf2[64 * s + lane] = X.dp1[s];
+      sh.cell[CL_M1][64 * s + lane] = X.dp1[s];
     }
     __syncthreads();
     if (lane < ncols) {
       double* q = sh.buf + lane * nz;
-      const double* dp = sh.buf2 + lane * nz;
+      const double* dp = sh.cell[CL_M1] + lane * nz;
       for (int k = 1; k < nz; ++k)
         if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * dp[k - 1] / dp[k];
       for (int k = 0; k < nz - 1; ++k)
@@ -612,78 +675,92 @@ __global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shi
     for (int s = 0; s < S; ++s) X.qv[s] = sh.buf[64 * s + lane];
     __syncthreads();
   }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    sh.cell[CL_M1][64 * s + lane] = 0.0;
+    sh.cell[CL_M1SOL][64 * s + lane] = 0.0;
+    sh.cell[CL_DZ1][64 * s + lane] = 1.0;
+  }
 
   // --------------------------------------------------------- sub-steps (:2045)
   for (int it = 0; it < P.ntimes; ++it) {
-    warm_rain(L, X, sh, P, c, true);
-    sedimentation(L, X, sh, P, c);
-    warm_rain(L, X, sh, P, c, false);
-    icloud(L, X, sh, P, c);
+    warm_rain(L, X, sh, G, f, P, c, true);
+    sedimentation(L, X, sh, G, f, P, c);
+    warm_rain(L, X, sh, G, f, P, c, false);
+    icloud(L, X, sh, G, f, P, c);
   }
 
   // ---------------------------------------------------- fields_update (:1452-1553)
+  // The initial values (q0 = q_in dp0 / dp1, qa0, u0, v0, t0, dp0) are read
+  // again from the inputs and recomputed with the operations fields_init used.
   const double rdt = P.rdt;
-  if (!n.sedi_transport && !P.accumulate) {
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-      if (L.v[s]) {
-        f.udt[off[s]] = 0.0;
-        f.vdt[off[s]] = 0.0;
-      }
-  }
+  const bool acc = P.accumulate != 0;
   if (n.sedi_transport) {
-    double mp[S], a[S], bu[S], bv[S], u1[S], v1[S];
-    col_prev(sh.buf, X.m1, mp);
+    double m1[S], mp[S], a[S], bu[S], bv[S], u1[S], v1[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) m1[s] = sh.cell[CL_M1][64 * s + lane];
+    col_prev(sh.buf, m1, mp);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+      const int64_t o = G.off[s];
+      const double u0 = L.v[s] ? f.ua[o] : 0.0;
+      const double v0 = L.v[s] ? f.va[o] : 0.0;
       if (L.k[s] == 0) {
         a[s] = 0.0;
-        bu[s] = u0[s];
-        bv[s] = v0[s];
+        bu[s] = u0;
+        bv[s] = v0;
       } else {
-        const double r = 1.0 / (X.dp0[s] + mp[s]);
+        const double dp0 = L.v[s] ? f.delp[o] : 100.0;
+        const double r = 1.0 / (dp0 + mp[s]);
         a[s] = mp[s] * r;
-        bu[s] = X.dp0[s] * u0[s] * r;
-        bv[s] = X.dp0[s] * v0[s] * r;
+        bu[s] = dp0 * u0 * r;
+        bv[s] = dp0 * v0 * r;
       }
     }
     scan_fwd(L, a, bu, u1);
     scan_fwd(L, a, bv, v1);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (L.v[s] && L.k[s] > 0) {
-        const int64_t o = off[s];
-        f.udt[o] = (P.accumulate ? f.udt[o] : 0.0) + (u1[s] - u0[s]) * rdt;
-        f.vdt[o] = (P.accumulate ? f.vdt[o] : 0.0) + (v1[s] - v0[s]) * rdt;
-      } else if (L.v[s] && !P.accumulate) {
-        f.udt[off[s]] = 0.0;
-        f.vdt[off[s]] = 0.0;
+      if (!L.v[s]) continue;
+      const int64_t o = G.off[s];
+      if (L.k[s] > 0) {
+        const double u0 = f.ua[o], v0 = f.va[o];
+        f.udt[o] = (acc ? f.udt[o] : 0.0) + (u1[s] - u0) * rdt;
+        f.vdt[o] = (acc ? f.vdt[o] : 0.0) + (v1[s] - v0) * rdt;
+      } else if (!acc) {
+        f.udt[o] = 0.0;
+        f.vdt[o] = 0.0;
       }
     }
+  } else if (!acc) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (L.v[s]) {
+        f.udt[G.off[s]] = 0.0;
+        f.vdt[G.off[s]] = 0.0;
+      }
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!L.v[s]) continue;
-    const int64_t o = off[s];
-    const double omq = X.dp1[s] / X.dp0[s];
-    const bool acc = P.accumulate != 0;
-    const double qi_dt0 = de_iced[s] ? qi_dt_in[s] : (acc ? f.qi_dt[o] : 0.0);
-    const double qs_dt0 = de_iced[s] ? qs_dt_in[s] : (acc ? f.qs_dt[o] : 0.0);
-    f.qv_dt[o] = (acc ? f.qv_dt[o] : 0.0) + rdt * (X.qv[s] - qv0[s]) * omq;
-    f.ql_dt[o] = (acc ? f.ql_dt[o] : 0.0) + rdt * (X.ql[s] - ql0[s]) * omq;
-    f.qr_dt[o] = (acc ? f.qr_dt[o] : 0.0) + rdt * (X.qr[s] - qr0[s]) * omq;
-    f.qi_dt[o] = qi_dt0 + rdt * (X.qi[s] - qi0[s]) * omq;
-    f.qs_dt[o] = qs_dt0 + rdt * (X.qs[s] - qs0[s]) * omq;
-    f.qg_dt[o] = (acc ? f.qg_dt[o] : 0.0) + rdt * (X.qg[s] - qg0[s]) * omq;
+    const int64_t o = G.off[s];
+    const double dp0 = f.delp[o];
+    const double omq0 = dp0 / X.dp1[s];
+    const double omq = X.dp1[s] / dp0;
+    f.qv_dt[o] = (acc ? f.qv_dt[o] : 0.0) + rdt * (X.qv[s] - f.qv[o] * omq0) * omq;
+    f.ql_dt[o] = (acc ? f.ql_dt[o] : 0.0) + rdt * (X.ql[s] - f.ql[o] * omq0) * omq;
+    f.qr_dt[o] = (acc ? f.qr_dt[o] : 0.0) + rdt * (X.qr[s] - f.qr[o] * omq0) * omq;
+    f.qi_dt[o] = (acc ? f.qi_dt[o] : 0.0) + rdt * (X.qi[s] - f.qi[o] * omq0) * omq;
+    f.qs_dt[o] = (acc ? f.qs_dt[o] : 0.0) + rdt * (X.qs[s] - f.qs[o] * omq0) * omq;
+    f.qg_dt[o] = (acc ? f.qg_dt[o] : 0.0) + rdt * (X.qg[s] - f.qg[o] * omq0) * omq;
     const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
                        (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
-    f.pt_dt[o] = (acc ? f.pt_dt[o] : 0.0) + rdt * (X.tz[s] - X.t0[s]) * cvm / CP_AIR;
+    f.pt_dt[o] = (acc ? f.pt_dt[o] : 0.0) + rdt * (X.tz[s] - f.pt[o]) * cvm / CP_AIR;
     if (n.do_qa)
       f.qa_dt[o] = 0.0;
     else
-      f.qa_dt[o] = (acc ? f.qa_dt[o] : 0.0) + rdt * (X.qaz[s] / P.ntimes - qa0[s]);
+      f.qa_dt[o] = (acc ? f.qa_dt[o] : 0.0) + rdt * (X.qaz[s] / P.ntimes - f.qa[o]);
     f.qaz[o] = X.qaz[s];
-    if (P.do_sedi_w) f.w[o] = X.w[s];
   }
   if (lane < ncols) {
     const int64_t col = c0 + lane;
@@ -705,13 +782,15 @@ struct LayoutChoice {
 // slot counts with a compiled kernel instance
 static const int kSlots[] = {1, 2, 5};
 
+static int pmax_of(int S) { return S == 5 ? pmax<5>() : S == 2 ? pmax<2>() : pmax<1>(); }
+
 LayoutChoice choose_layout(int64_t nz) {
   LayoutChoice best{0, 0};
   double best_util = -1.0;
   for (int S : kSlots) {
     const int64_t P = (64 * S) / nz;
     if (P < 1) continue;
-    const int64_t Pc = P > PMAX ? PMAX : P;
+    const int64_t Pc = P > pmax_of(S) ? pmax_of(S) : P;
     const double util = double(Pc * nz) / double(64 * S);
     if (util > best_util + 1e-9) {
       best_util = util;

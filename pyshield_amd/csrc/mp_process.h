@@ -4,9 +4,15 @@
 // (pySHiELD/functions/microphysics_funcs.py) statement by statement.  The
 // cell state `Cell` is FP64; rates are computed in the rate type R and
 // applied as FP64 transfers (s.q_a -= x; s.q_b += x).  With R = double every
-// expression has the reference's operation order.  With R = float, limiters
-// that bound a sink by the species it drains are evaluated against the FP64
-// state (`cap`), so no species is driven below zero by FP32 rounding.
+// expression has the reference's operation order.  With R = float:
+//  * saturation mixing ratios and their gradients are evaluated in FP64 from
+//    the FP64 temperature and density, and every difference between a
+//    saturation value and the state (qsat - qv, ...) is formed in FP64 before
+//    it is narrowed; FP32 would put ~1e-6 of qsat (1e-8 kg/kg) into every
+//    saturation adjustment;
+//  * limiters that bound a sink by the species it drains are evaluated
+//    against the FP64 state (`cap`), so no species is driven below zero by
+//    FP32 rounding.
 #pragma once
 
 #include "mp_common.h"
@@ -110,34 +116,35 @@ __device__ __forceinline__ R gmlt(R tc, R dqs, R qgrho, R pgacw, R pgacr, const 
 
 // ------------------------------------------------ f:183-265
 template <class R>
-__device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, Cell& s, R den,
+__device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, Cell& s, double den_d,
                                            R denfac) {
   if ((s.tz > t_wfr) && (s.qr > QRMIN)) {
+    const R den = R(den_d);
     R lhl = c.lv00 + c.d0_vap * R(s.tz);
     R q_liq = R(s.ql) + R(s.qr);
     R q_sol = R(s.qi) + R(s.qs) + R(s.qg);
     R cvm = SH_CVM(c, s, q_liq, q_sol);
     R lcpk = lhl / cvm;
-    R tin = R(s.tz) - lcpk * R(s.ql);
-    R qpz = R(s.qv) + R(s.ql);
-    R dqsdt;
-    R qsat = wqs2(tin, den, dqsdt);
-    R dqh = mmax(R(s.ql), h_var * mmax(qpz, R(QCMIN)));
-    dqh = mmin(dqh, R(0.2) * qpz);
-    R dqv = qsat - R(s.qv);
-    R q_minus = qpz - dqh;
-    R q_plus = qpz + dqh;
-    if ((dqv > R(QVMIN)) && (qsat > q_minus)) {
+    const double tin = s.tz - double(lcpk * R(s.ql));
+    const double qpz = s.qv + s.ql;
+    double dqsdt;
+    const double qsat = wqs2<double>(tin, den_d, dqsdt);
+    R dqh = mmax(R(s.ql), h_var * mmax(R(qpz), R(QCMIN)));
+    dqh = mmin(dqh, R(0.2) * R(qpz));
+    const double dqv = qsat - s.qv;
+    const double q_minus = qpz - double(dqh);
+    const double q_plus = qpz + double(dqh);
+    if ((dqv > QVMIN) && (qsat > q_minus)) {
       R dq;
       if (qsat > q_plus)
-        dq = qsat - qpz;
+        dq = R(qsat - qpz);
       else
-        dq = R(0.25) * ((q_minus - qsat) * (q_minus - qsat)) / dqh;
+        dq = R(0.25) * R((q_minus - qsat) * (q_minus - qsat)) / dqh;
       R qden = R(s.qr) * den;
-      R t2 = tin * tin;
+      R t2 = R(tin) * R(tin);
       R evap = c.crevp[0] * t2 * dq * (c.crevp[1] * msqrt(qden) + c.crevp[2] * mexp(R(0.725) * mlog(qden))) /
-               (c.crevp[3] * t2 + c.crevp[4] * qsat * den);
-      double ev = mmin(s.qr, double(mmin(dt * evap, dqv / (R(1.0) + lcpk * dqsdt))));
+               (c.crevp[3] * t2 + c.crevp[4] * R(qsat) * den);
+      double ev = mmin(s.qr, double(mmin(dt * evap, R(dqv) / (R(1.0) + lcpk * R(dqsdt)))));
       s.qr = s.qr - ev;
       s.qv = s.qv + ev;
       q_liq = q_liq - R(ev);
@@ -235,9 +242,10 @@ __device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_
 
 // ------------------------------------------------ f:431-850
 template <class R>
-__device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, Cell& s, R h_var, R rh_adj,
-                                               R rh_rain, R den, R denfac, double p1) {
+__device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, Cell& s, R h_var, double rh_adj,
+                                               double rh_rain, double den_d, R denfac, double p1) {
   const R tice = c.tice, t_wfr = c.t_wfr, t_sub = c.t_sub, dts = c.dts;
+  const R den = R(den_d);
   R lhl = c.lv00 + c.d0_vap * R(s.tz);
   R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
   R q_liq = R(s.ql) + R(s.qr);
@@ -266,37 +274,34 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   tcp3 = lcpk + icpk * mmin(R(1.0), dim(tice, R(s.tz)) / (tice - t_wfr));
 
   // instant evaporation / sublimation of all clouds when rh < rh_adj
-  R qpz = R(s.qv) + R(s.ql) + R(s.qi);
+  const double qpz = s.qv + s.ql + s.qi;
   const R dtin = (lhl * (R(s.ql) + R(s.qi)) + lhi * R(s.qi)) /
-                 (c.c_air + qpz * c.c_vap + R(s.qr) * R(C_LIQ) + (R(s.qs) + R(s.qg)) * R(C_ICE));
-  R tin = R(s.tz) - dtin;
-  R rh = R(0.0);
-  if (tin > t_sub + R(6.0)) {
-    rh = qpz / iqs1(tin, den);
+                 (c.c_air + R(qpz) * c.c_vap + R(s.qr) * R(C_LIQ) + (R(s.qs) + R(s.qg)) * R(C_ICE));
+  const double tin = s.tz - double(dtin);
+  const double t_sub6 = c.t_sub_d + 6.0;
+  double rh = 0.0;
+  if (tin > t_sub6) {
+    rh = qpz / iqs1<double>(tin, den_d);
     if (rh < rh_adj) {
-      // all vapour, liquid and ice of the cell collapse into vapour; the
-      // temperature change is applied as an FP64 increment
-      s.tz = s.tz - double(dtin);
-      if constexpr (is_f32<R>::value)
-        s.qv = s.qv + s.ql + s.qi;
-      else
-        s.qv = double(qpz);
+      // all vapour, liquid and ice of the cell collapse into vapour
+      s.tz = tin;
+      s.qv = qpz;
       s.ql = 0.0;
       s.qi = 0.0;
     }
   }
-  if (!(((tin > t_sub + R(6.0)) && (rh >= rh_adj)) || (tin <= t_sub + R(6.0)))) return;
+  if (!(((tin > t_sub6) && (rh >= rh_adj)) || (tin <= t_sub6))) return;
 
   // cloud water <-> vapour adjustment
-  R dwsdt;
-  R qsw = wqs2(R(s.tz), den, dwsdt);
-  R dq0 = qsw - R(s.qv);
+  double dwsdt;
+  const double qsw = wqs2<double>(s.tz, den_d, dwsdt);
+  const double dq0 = qsw - s.qv;
   double evap;
-  if (dq0 > R(0.0)) {
-    R factor = mmin(R(1.0), c.fac_l2v * (R(10.0) * dq0 / qsw));
-    evap = mmin(s.ql, double(factor * dq0 / (R(1.0) + tcp3 * dwsdt)));
+  if (dq0 > 0.0) {
+    R factor = mmin(R(1.0), c.fac_l2v * R(10.0 * dq0 / qsw));
+    evap = mmin(s.ql, double(factor * R(dq0) / (R(1.0) + tcp3 * R(dwsdt))));
   } else {
-    evap = double(dq0 / (R(1.0) + tcp3 * dwsdt));
+    evap = double(R(dq0) / (R(1.0) + tcp3 * R(dwsdt)));
   }
   s.qv = s.qv + evap;
   s.ql = s.ql - evap;
@@ -346,19 +351,19 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // sublimation / deposition of cloud ice
   if (s.tz < tice_d) {
-    R dqsdt;
-    R qsi = iqs2(R(s.tz), den, dqsdt);
-    R dq = R(s.qv) - qsi;
-    R sink = dq / (R(1.0) + tcpk * dqsdt);
+    double dqsdt;
+    const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
+    const double dq = s.qv - qsi;
+    R sink = R(dq) / (R(1.0) + tcpk * R(dqsdt));
     R pidep;
     if (s.qi > QRMIN) {
-      pidep = dt_pisub * dq * R(349138.78) * mexp(R(0.875) * mlog(R(s.qi) * den)) /
-              (qsi * den * R(LAT2) / (R(0.0243) * R(RVGAS) * (R(s.tz) * R(s.tz))) + R(4.42478e4));
+      pidep = dt_pisub * R(dq) * R(349138.78) * mexp(R(0.875) * mlog(R(s.qi) * den)) /
+              (R(qsi) * den * R(LAT2) / (R(0.0243) * R(RVGAS) * (R(s.tz) * R(s.tz))) + R(4.42478e4));
     } else {
       pidep = R(0.0);
     }
     double sk;
-    if (dq > R(0.0)) {
+    if (dq > 0.0) {
       R tmp = tice - R(s.tz);
       R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * tmp) / den;
       sk = double(mmin(sink, mmin(mmax(qi_crt - R(s.qi), pidep), tmp / tcpk)));
@@ -381,15 +386,16 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // sublimation / deposition of snow
   if (s.qs > QRMIN) {
-    R dqsdt;
-    R qsi = iqs2(R(s.tz), den, dqsdt);
+    double dqsdt;
+    const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
     R qden = R(s.qs) * den;
     R tmp = mexp(R(0.65625) * mlog(qden));
     R tsq = R(s.tz) * R(s.tz);
-    R dq = (qsi - R(s.qv)) / (R(1.0) + tcpk * dqsdt);
+    const R dsat = R(qsi - s.qv);
+    R dq = dsat / (R(1.0) + tcpk * R(dqsdt));
     R pssub = c.cssub[0] * tsq * (c.cssub[1] * msqrt(qden) + c.cssub[2] * tmp * msqrt(denfac)) /
-              (c.cssub[3] * tsq + c.cssub[4] * qsi * den);
-    pssub = (qsi - R(s.qv)) * dts * pssub;
+              (c.cssub[3] * tsq + c.cssub[4] * R(qsi) * den);
+    pssub = dsat * dts * pssub;
     double ps;
     if (pssub > R(0.0)) {
       ps = mmin(double(pssub * mmin(R(1.0), dim(R(s.tz), t_sub) * R(0.2))), s.qs);
@@ -413,10 +419,10 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // two-way graupel sublimation / deposition
   if (s.qg > QRMIN) {
-    R dqsdt;
-    R qsi = iqs2(R(s.tz), den, dqsdt);
-    R dq = (R(s.qv) - qsi) / (R(1.0) + tcpk * dqsdt);
-    R pgsub = (R(s.qv) / qsi - R(1.0)) * R(s.qg);
+    double dqsdt;
+    const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
+    R dq = R(s.qv - qsi) / (R(1.0) + tcpk * R(dqsdt));
+    R pgsub = R(s.qv / qsi - 1.0) * R(s.qg);
     double pg;
     if (pgsub > R(0.0)) {
       if (s.tz > tice_d)
@@ -439,9 +445,9 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   lhl = c.lv00 + c.d0_vap * R(s.tz);
   lcpk = lhl / cvm;
   if (s.qr > QCMIN) {
-    R dqsdt;
-    R qsw2 = wqs2(R(s.tz), den, dqsdt);
-    double sink = mmin(s.qr, double(dim(rh_rain * qsw2, R(s.qv)) / (R(1.0) + lcpk * dqsdt)));
+    double dqsdt;
+    const double qsw2 = wqs2<double>(s.tz, den_d, dqsdt);
+    double sink = mmin(s.qr, double(R(dim(rh_rain * qsw2, s.qv)) / (R(1.0) + lcpk * R(dqsdt))));
     s.qv = s.qv + sink;
     s.qr = s.qr - sink;
     q_liq = q_liq - R(sink);
@@ -454,44 +460,83 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // cloud fraction
   if (!c.do_qa) {
-    q_sol = c.rad_snow ? R(s.qi) + R(s.qs) : R(s.qi);
-    q_liq = c.rad_rain ? R(s.ql) + R(s.qr) : R(s.ql);
-    R q_cond = q_liq + q_sol;
-    R qpz2 = R(s.qv) + q_cond;
-    R tin2 = R(s.tz) - (lcpk * q_cond + icpk * q_sol);
-    R qstar;
-    if (tin2 <= t_wfr) {
-      qstar = iqs1(tin2, den);
-    } else if (tin2 >= tice) {
-      qstar = wqs1(tin2, den);
+    const double q_sol2 = c.rad_snow ? s.qi + s.qs : s.qi;
+    const double q_liq2 = c.rad_rain ? s.ql + s.qr : s.ql;
+    const double q_cond = q_liq2 + q_sol2;
+    const double qpz2 = s.qv + q_cond;
+    const double tin2 = s.tz - double(lcpk * R(q_cond) + icpk * R(q_sol2));
+    double qstar;
+    if (tin2 <= c.t_wfr_d) {
+      qstar = iqs1<double>(tin2, den_d);
+    } else if (tin2 >= tice_d) {
+      qstar = wqs1<double>(tin2, den_d);
     } else {
-      R qsi = iqs1(tin2, den);
-      R qsw3 = wqs1(tin2, den);
-      R rqi;
-      if (q_cond > R(3.0e-6))
-        rqi = q_sol / q_cond;
+      const double qsi = iqs1<double>(tin2, den_d);
+      const double qsw3 = wqs1<double>(tin2, den_d);
+      double rqi;
+      if (q_cond > 3.0e-6)
+        rqi = q_sol2 / q_cond;
       else
-        rqi = (tice - tin2) / (tice - t_wfr);
-      qstar = rqi * qsi + (R(1.0) - rqi) * qsw3;
+        rqi = (tice_d - tin2) / (tice_d - c.t_wfr_d);
+      qstar = rqi * qsi + (1.0 - rqi) * qsw3;
     }
-    if (qpz2 > R(QRMIN)) {
-      R dq = mmax(R(QCMIN), h_var * qpz2);
-      R q_plus = qpz2 + dq;
-      R q_minus = qpz2 - dq;
+    if (qpz2 > QRMIN) {
+      const double dq = mmax(QCMIN, double(h_var) * qpz2);
+      const double q_plus = qpz2 + dq;
+      const double q_minus = qpz2 - dq;
       if (qstar < q_minus)
         s.qa = s.qa + 1.0;
-      else if ((qstar < q_plus) && (q_cond > c.qc_crt))
-        s.qa = s.qa + double((q_plus - qstar) / (dq + dq));
+      else if ((qstar < q_plus) && (q_cond > c.qc_crt_d))
+        s.qa = s.qa + (q_plus - qstar) / (dq + dq);
     }
+  }
+}
+
+// ------------------------------------------------ microphysics.py:1290-1338
+// Instant melting of cloud ice above tice (pimlt) or homogeneous freezing of
+// cloud water below t_wfr (pihom); returns the heat-capacity terms the rest of
+// icloud continues from.
+template <class R>
+__device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice_d, double t_wfr_d, double ql_mlt,
+                                                   Cell& s, double den, R& q_liq, R& q_sol, R& cvm) {
+  const R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  q_liq = R(s.ql) + R(s.qr);
+  q_sol = R(s.qi) + R(s.qs) + R(s.qg);
+  cvm = SH_CVM(c, s, q_liq, q_sol);
+  const R icpk = lhi / cvm;
+  if ((s.tz > tice_d) && (s.qi > QCMIN)) {
+    const double melt = mmin(s.qi, double(c.fac_imlt * R(s.tz - tice_d) / icpk));
+    const double tmp = mmin(melt, dim(ql_mlt, s.ql));
+    s.ql = s.ql + tmp;
+    s.qr = s.qr + melt - tmp;
+    s.qi = s.qi - melt;
+    q_liq = q_liq + R(melt);
+    q_sol = q_sol - R(melt);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz - double(R(melt) * lhi / cvm);
+  } else if ((s.tz < t_wfr_d) && (s.ql > QCMIN)) {
+    const R dtmp = R(t_wfr_d - s.tz);
+    const R factor = mmin(R(1.0), dtmp / R(DT_FR));
+    const double sink = mmin(s.ql * double(factor), double(dtmp / icpk));
+    const R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * R(tice_d - s.tz)) / R(den);
+    const double tmp = mmin(sink, dim(double(qi_crt), s.qi));
+    s.ql = s.ql - sink;
+    s.qs = s.qs + sink - tmp;
+    s.qi = s.qi + tmp;
+    q_liq = q_liq - R(sink);
+    q_sol = q_sol + R(sink);
+    cvm = SH_CVM(c, s, q_liq, q_sol);
+    s.tz = s.tz + double(R(sink) * lhi / cvm);
   }
 }
 
 // ------------------------------------------------ f:853-1260
 template <class R>
 __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, Cell& s, R h_var,
-                                            R rh_adj, R rh_rain, R den, R denfac, R vtgz, R vtrz, R vtsz,
-                                            double p1d, R di, R q_liq, R q_sol, R cvm) {
+                                            double rh_adj, double rh_rain, double den_d, R denfac, R vtgz, R vtrz,
+                                            R vtsz, double p1d, R di, R q_liq, R q_sol, R cvm) {
   const R tice = c.tice, dts = c.dts, rdts = c.rdts;
+  const R den = R(den_d);
   R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
   R icpk = lhi / cvm;
   if (p1d >= P_MIN) {
@@ -499,7 +544,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
     R pgacr = R(0.0), pgacw = R(0.0);
     R tc = R(s.tz) - tice;
     if (s.tz - tice_d >= 0.0) {
-      R dqs0 = c.ces0 / p1 - R(s.qv);
+      R dqs0 = R(double(c.ces0) / p1d - s.qv);
       if (s.qs > QCMIN) {
         R psacw, psacr, pracs;
         if (s.ql > QRMIN) {
@@ -658,7 +703,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
       }
     }
   }
-  subgrid_z_proc<R>(c, tice_d, s, h_var, rh_adj, rh_rain, den, denfac, p1d);
+  subgrid_z_proc<R>(c, tice_d, s, h_var, rh_adj, rh_rain, den_d, denfac, p1d);
 }
 
 }  // namespace shield
