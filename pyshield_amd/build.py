@@ -22,6 +22,9 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+    # keep loop-invariant constants of the sub-step loop from being hoisted
+    # into registers that stay live across the whole loop body
+    "-mllvm", "-disable-machine-licm",
     "-I", os.path.join(HERE, os.pardir, "include"),
     "-Wall", "-Wno-unused-function",
 ]

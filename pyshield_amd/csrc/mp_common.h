@@ -77,6 +77,34 @@ __device__ __forceinline__ float mlog(float x) { return __logf(x); }
 __device__ __forceinline__ float msqrt(float x) { return __fsqrt_rn(x); }
 __device__ __forceinline__ float mpow(float x, float y) { return __expf(y * __logf(x)); }
 __device__ __forceinline__ float mabs(float x) { return fabsf(x); }
+// Powers x^p (x > 0, p a literal).  pw_el is the reference's exp(p*log(x)),
+// pw its x**p.  In FP64 they are exactly those expressions.  In FP32 the power
+// is split as x^n * 2^((p-n) log2 x) with n the integer nearest p: rounding
+// the exponent then costs |p-n| |log2 x| ulp instead of |p| |ln x| ulp (for
+// x^(7/3) at x = 1e-4, 3 ulp instead of 21).
+__device__ __forceinline__ float pw_split(float x, double p) {
+  const double nd = __builtin_rint(p);
+  const float r = float(p - nd);
+  float y = __builtin_amdgcn_exp2f(r * __builtin_amdgcn_logf(x));
+  const int n = int(nd);
+  if (n >= 1) y *= x;
+  if (n >= 2) y *= x;
+  if (n >= 3) y *= x;
+  return y;
+}
+template <class R>
+__device__ __forceinline__ R pw_el(R x, double p);
+template <>
+__device__ __forceinline__ double pw_el<double>(double x, double p) { return exp(p * log(x)); }
+template <>
+__device__ __forceinline__ float pw_el<float>(float x, double p) { return pw_split(x, p); }
+template <class R>
+__device__ __forceinline__ R pw(R x, double p);
+template <>
+__device__ __forceinline__ double pw<double>(double x, double p) { return pow(x, p); }
+template <>
+__device__ __forceinline__ float pw<float>(float x, double p) { return pw_split(x, p); }
+
 template <class T>
 __device__ __forceinline__ T mmin(T a, T b) { return a < b ? a : b; }
 template <class T>
@@ -87,6 +115,10 @@ __device__ __forceinline__ T dim(T x, T y) {
   T diff = x - y;
   return diff > T(0) ? diff : T(0);
 }
+
+// most columns one wavefront of the forward kernel owns, per slot count (the
+// size of its per-column LDS tables)
+constexpr int pmax_for(int S) { return S >= 3 ? 16 : 64; }
 
 // --------------------------------------------------- coefficient block
 // The scalars the kernels read, in the rate type R.  Filled on the host from

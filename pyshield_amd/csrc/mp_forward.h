@@ -1,0 +1,771 @@
+// mp_forward.h -- the fused GFDL microphysics forward kernel for gfx950.
+//
+// One launch = one Microphysics.__call__(state, timestep)
+// (pySHiELD/stencils/microphysics.py:1981-2263): fields_init, ntimes x
+// [warm_rain(first), sedimentation, warm_rain(second), icloud], then
+// fields_update.  The reference runs five GT4Py stencils (~100 generated
+// kernels per sub-step, 43 persistent 3D scratch fields); here every
+// intermediate lives in registers and LDS.  HBM sees the compulsory traffic
+// (13 FP64 fields read and 11 written per cell at the default namelist, plus
+// 6 FP64 values per column) and re-reads of inputs that are cheaper to fetch
+// again than to hold: pt and delz in the sub-steps, and the state and winds
+// in fields_update.
+//
+// Layout: see mp_wave.h.  One wavefront (= one workgroup) owns `ncols`
+// consecutive columns spread over S slots of 64 lanes.  The kernel is
+// instantiated per slot count in mp_forward_s<S>.hip.
+#pragma once
+
+#include "mp_process.h"
+#include "mp_wave.h"
+
+namespace shield {
+
+enum { CV_HVAR, CV_RHADJ, CV_RHRAIN, CV_CCN, CV_CPRAUT, CV_RAIN, CV_SNOW, CV_ICE, CV_GRAUPEL, CV_N };
+// per-cell values kept in LDS (each lane touches only its own cells)
+enum { CL_M1, CL_M1SOL, CL_DZ1, CL_TMP, CL_DEN, CL_DENFAC, CL_N };
+
+// Per-cell state that stays in registers for the whole call.
+template <int S, class R>
+struct St {
+  double qv[S], ql[S], qr[S], qi[S], qs[S], qg[S], tz[S];  // prognostic (dry mixing ratios)
+  double qaz[S];                                            // cloud-fraction accumulator
+  double dp1[S];                                            // dry air mass (Pa)
+  R vtr[S], vtg[S], vts[S];                                 // fall speeds for icloud
+};
+
+template <int S>
+struct Shared {
+  double buf[64 * S];          // neighbour exchange
+  double cell[CL_N][64 * S];   // per-cell values, CL_*
+  double colv[CV_N][pmax_for(S)];
+  // this lane's cell of slot s in per-cell array `which`
+  __device__ __forceinline__ double& at(int which, int s) { return cell[which][64 * s + lane_id()]; }
+};
+
+// launder() hides a value's origin from the optimiser (an empty asm that
+// claims to rewrite it in place), so the value is materialised exactly as
+// rounded and cannot be fused into a neighbouring multiply-add.
+__device__ __forceinline__ double launder(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// The point-wise stages are unrolled over the S slots with static indices, so
+// every slot's state stays in registers.
+template <int S, class R>
+__device__ __forceinline__ Cell cell_of(const St<S, R>& X, int s) {
+  return Cell{X.qv[s], X.ql[s], X.qr[s], X.qi[s], X.qs[s], X.qg[s], X.tz[s], X.qaz[s]};
+}
+template <int S, class R>
+__device__ __forceinline__ void set_cell(St<S, R>& X, int s, const Cell& c) {
+  X.qv[s] = c.qv;
+  X.ql[s] = c.ql;
+  X.qr[s] = c.qr;
+  X.qi[s] = c.qi;
+  X.qs[s] = c.qs;
+  X.qg[s] = c.qg;
+  X.tz[s] = c.tz;
+  X.qaz[s] = c.qa;
+}
+
+// Global element offsets of this wavefront's cells (the host guarantees
+// every addressed element index fits in 31 bits).
+template <int S>
+struct Glob {
+  int32_t off[S];
+};
+
+// den0 = -dp1 / (g dz0) (microphysics.py:149), recomputed where needed
+__device__ __forceinline__ double dry_density(double dp1, double dz0) { return -dp1 / (GRAV * dz0); }
+
+// Layer-bottom heights ze (BACKWARD, microphysics.py:373-380 / :756-765) and
+// layer thicknesses dz = ze[k] - ze[k+1], dz[nz-1] = ze - zs (:445-452).
+template <int S>
+__device__ __forceinline__ void layer_dz(const Lay<S>& L, Shared<S>& sh, double zs, double (&dz)[S]) {
+  const int lane = lane_id();
+  double b[S], ze[S], zn[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double dz1 = sh.cell[CL_DZ1][64 * s + lane];
+    b[s] = (L.kb[s] == 0) ? zs - dz1 : -dz1;
+  }
+  sum_bwd(L, b, ze);
+  col_next(sh.buf, ze, zn);
+#pragma unroll
+  for (int s = 0; s < S; ++s) dz[s] = (L.kb[s] == 0) ? ze[s] - zs : ze[s] - zn[s];
+}
+
+// Implicit upwind fall of one species over dt with speeds vt (the pattern
+// shared by microphysics.py:454-515, :825-900, :956-1035, :1089-1168).
+// q holds dry mixing ratios.  Only columns with fall[s] change; flux gets the
+// mass flux through each layer bottom (0 in non-falling columns) and surf
+// the surface flux of the cell's column.  When do_sedi_w the vertical
+// velocity, which stays in HBM, is transported with the falling mass.
+template <int S, class R>
+__device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                              const shield_mp_fields& f, const double (&dz)[S], double (&q)[S],
+                                              const R (&vt)[S], double dt, const bool (&fall)[S], bool do_sedi_w,
+                                              double (&flux)[S], double (&surf)[S]) {
+  double dd[S], qq[S], a[S], b[S], qm[S], ddp[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    dd[s] = dt * double(vt[s]);
+    qq[s] = q[s] * X.dp1[s];
+  }
+  col_prev(sh.buf, dd, ddp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double r = 1.0 / (dz[s] + dd[s]);
+    a[s] = ddp[s] * r;
+    b[s] = qq[s] * r;
+  }
+  scan_fwd(L, a, b, qm);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    qm[s] = qm[s] * dz[s];
+    b[s] = qq[s] - qm[s];
+  }
+  sum_fwd(L, b, flux);
+  col_bottom(L, sh.buf, flux, surf);
+  if (do_sedi_w) {
+    // dm uses the mixing ratios before the fall (:478-486)
+    double fp[S], vp[S], vd[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      vd[s] = double(vt[s]);
+      if (!fall[s]) flux[s] = 0.0;
+    }
+    col_prev(sh.buf, flux, fp);
+    col_prev(sh.buf, vd, vp);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!fall[s] || !L.v[s]) continue;
+      const double dm = X.dp1[s] * (1.0 + X.qv[s] + X.ql[s] + X.qr[s] + X.qi[s] + X.qs[s] + X.qg[s]);
+      const int32_t o = G.off[s];
+      const double w = f.w[o];
+      if (L.k[s] == 0)
+        f.w[o] = (dm * w + flux[s] * vd[s]) / (dm - flux[s]);
+      else
+        f.w[o] = (dm * w - fp[s] * vp[s] + flux[s] * vd[s]) / (dm + fp[s] - flux[s]);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (fall[s]) {
+      q[s] = qm[s] / X.dp1[s];
+    } else {
+      flux[s] = 0.0;
+      surf[s] = 0.0;
+    }
+  }
+}
+
+// Sedimentation heating, explicit at the top and implicit below
+// (microphysics.py:516-557 with cq = C_LIQ, :1176-1217 with cq = C_ICE).
+template <int S, class R>
+__device__ __forceinline__ void sedi_heat(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const double (&flux)[S],
+                                          double cq, const bool (&apply)[S]) {
+  const int lane = lane_id();
+  double fp[S], a[S], b[S], tn[S];
+  col_prev(sh.buf, flux, fp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double dgz = -0.5 * GRAV * sh.cell[CL_DZ1][64 * s + lane];
+    const double cvn = X.dp1[s] * (CV_AIR + X.qv[s] * CV_VAP + (X.qr[s] + X.ql[s]) * C_LIQ +
+                                   (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE);
+    if (L.k[s] == 0) {
+      a[s] = 0.0;
+      b[s] = X.tz[s] + flux[s] * dgz / (cvn + flux[s] * cq);
+    } else {
+      const double r = 1.0 / (cvn + cq * flux[s]);
+      a[s] = fp[s] * cq * r;
+      b[s] = ((cvn + cq * (flux[s] - fp[s])) * X.tz[s] + dgz * (fp[s] + flux[s])) * r;
+    }
+  }
+  scan_fwd(L, a, b, tn);
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    if (apply[s]) X.tz[s] = tn[s];
+}
+
+// Vertical subgrid variability limiter (microphysics.py:595-620 for liquid,
+// :1342-1367 for ice): returns dl (0 at the top and surface layers).
+template <int S>
+__device__ __forceinline__ void slope_limiter(const Lay<S>& L, Shared<S>& sh, const double (&q)[S],
+                                              double (&dl)[S]) {
+  double qp[S], dq[S], dqn[S];
+  col_prev(sh.buf, q, qp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) dq[s] = 0.5 * (q[s] - qp[s]);
+  col_next(sh.buf, dq, dqn);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (L.k[s] == 0 || L.kb[s] == 0) {
+      dl[s] = 0.0;
+    } else {
+      double d = 0.5 * mmin(fabs(dq[s] + dqn[s]), 0.5 * q[s]);
+      if (dq[s] * dqn[s] <= 0.0) {
+        if (dq[s] > 0.0)
+          d = mmin(d, mmin(dq[s], -dqn[s]));
+        else
+          d = 0.0;
+      }
+      dl[s] = d;
+    }
+  }
+}
+
+// ------------------------------------------------------ warm_rain (:279-653)
+template <int S, class R>
+__device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                          const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c,
+                                          bool is_first) {
+  const int lane = lane_id();
+  const double t_wfr = P.t_wfr;
+  if (is_first) {
+    // air density from hydrostatic balance (:331-341); t0 and dz0 are the
+    // call's input pt and delz
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double t0 = L.v[s] ? f.pt[G.off[s]] : 250.0;
+      const double dz0 = L.v[s] ? f.delz[G.off[s]] : -100.0;
+      const double dz1 = dz0 * X.tz[s] / t0;
+      sh.cell[CL_DZ1][64 * s + lane] = dz1;
+      sh.at(CL_DEN, s) = dry_density(X.dp1[s], dz0) * dz0 / dz1;
+      sh.at(CL_DENFAC, s) = sqrt(SFCRHO / sh.at(CL_DEN, s));
+    }
+  }
+  bool pred[S], fall[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qr[s] > QRMIN);
+  col_any(L, pred, fall);
+#pragma unroll
+  for (int s = 0; s < S; ++s) X.vtr[s] = rain_fspeed<R>(c, !fall[s], R(X.qr[s]), R(sh.at(CL_DEN, s)));
+  double flux[S], surf[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    flux[s] = 0.0;
+    surf[s] = 0.0;
+  }
+  const R dt5 = R(0.5) * c.dt_rain;
+  if (wave_any(fall)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (fall[s]) {
+        Cell cs = cell_of(X, s);
+        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][L.cl[s]]), cs, sh.at(CL_DEN, s),
+                      R(sh.at(CL_DENFAC, s)));
+        set_cell(X, s, cs);
+      }
+    }
+    double dz[S];
+    layer_dz(L, sh, P.zs, dz);
+    implicit_fall(L, X, sh, G, f, dz, X.qr, X.vtr, P.dt_rain, fall, P.do_sedi_w != 0, flux, surf);
+    if (P.nml.do_sedi_heat) sedi_heat(L, X, sh, flux, C_LIQ, fall);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (fall[s]) {
+        Cell cs = cell_of(X, s);
+        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][L.cl[s]]), cs, sh.at(CL_DEN, s),
+                      R(sh.at(CL_DENFAC, s)));
+        set_cell(X, s, cs);
+      }
+    }
+  }
+  // auto-conversion (:587-645)
+  if (P.nml.irain_f != 0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int cl = L.cl[s];
+      Cell cs = cell_of(X, s);
+      autoconv_no_subgrid_var<R>(c, t_wfr, cs, R(sh.at(CL_DEN, s)), R(sh.colv[CV_CCN][cl]),
+                                 R(sh.colv[CV_CPRAUT][cl]));
+      set_cell(X, s, cs);
+    }
+  } else {
+    double dl[S];
+    if (P.nml.z_slope_liq) slope_limiter(L, sh, X.ql, dl);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int cl = L.cl[s];
+      const double hv = sh.colv[CV_HVAR][cl];
+      Cell cs = cell_of(X, s);
+      double d;
+      if (P.nml.z_slope_liq)
+        d = mmax(dl[s], mmax(QVMIN, hv * cs.ql));
+      else
+        d = mmax(QVMIN, hv * cs.ql);
+      autoconv_subgrid_var<R>(c, t_wfr, cs, R(sh.at(CL_DEN, s)), R(sh.colv[CV_CCN][cl]),
+                              R(sh.colv[CV_CPRAUT][cl]), R(d));
+      set_cell(X, s, cs);
+    }
+  }
+  // rain += r1; m1 += m1_rain (+ m1_sol)  (:647-653)
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (L.v[s] && L.kb[s] == 0) sh.colv[CV_RAIN][L.cl[s]] += surf[s];
+    double& m1 = sh.cell[CL_M1][64 * s + lane];
+    m1 = is_first ? m1 + flux[s] : m1 + flux[s] + sh.cell[CL_M1SOL][64 * s + lane];
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------- sedimentation (:656-1217)
+template <int S, class R>
+__device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                              const shield_mp_fields& f, const shield_mp_params& P,
+                                              const Coef<R>& c) {
+  const int lane = lane_id();
+  const double tice_d = P.nml.tice;
+  R vti[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    R vg, vi, vs;
+    fall_speed<R>(c, R(X.qg[s]), R(X.qi[s]), R(X.qs[s]), R(X.tz[s] - tice_d),
+                  R(sh.at(CL_DEN, s)), vg, vi, vs);
+    X.vtg[s] = vg;
+    vti[s] = vi;
+    X.vts[s] = vs;
+  }
+  // melting of cloud ice at and below the first level warmer than tice, and
+  // always in the surface layer (:702-746)
+  bool pred[S], stop[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.tz[s] > tice_d);
+  col_from_first(L, pred, stop);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double tcd = X.tz[s] - tice_d;
+    if (stop[s] && (X.qi[s] > QCMIN) && (tcd > 0.0)) {
+      const R lhi = R(LI00) + R(DC_ICE) * R(X.tz[s]);
+      R q_liq = R(X.ql[s]) + R(X.qr[s]);
+      R q_sol = R(X.qi[s]) + R(X.qs[s]) + R(X.qg[s]);
+      R cvm = c.c_air + R(X.qv[s]) * c.c_vap + q_liq * R(C_LIQ) + q_sol * R(C_ICE);
+      const R icpk = lhi / cvm;
+      const double sink = mmin(X.qi[s], double(c.fac_imlt * R(tcd) / icpk));
+      const double tmp = mmin(sink, dim(double(P.nml.ql_mlt), X.ql[s]));
+      X.ql[s] = X.ql[s] + tmp;
+      X.qr[s] = X.qr[s] + sink - tmp;
+      X.qi[s] = X.qi[s] - sink;
+      q_liq = q_liq + R(sink);
+      q_sol = q_sol - R(sink);
+      cvm = c.c_air + R(X.qv[s]) * c.c_vap + q_liq * R(C_LIQ) + q_sol * R(C_ICE);
+      X.tz[s] = X.tz[s] - double(R(sink) * lhi / cvm);
+    }
+  }
+  double dz[S], flux[S], surf[S], m1_sol[S];
+  bool fall[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) m1_sol[s] = 0.0;
+  layer_dz(L, sh, P.zs, dz);
+  const bool sedi_w = P.do_sedi_w != 0;
+  // cloud ice (:773-900)
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qi[s] > QRMIN);
+  col_any(L, pred, fall);
+  if (!(P.nml.vi_fac < 1.0e-5) && wave_any(fall)) {
+    implicit_fall(L, X, sh, G, f, dz, X.qi, vti, P.dts, fall, sedi_w, flux, surf);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      m1_sol[s] = flux[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_ICE][L.cl[s]] += surf[s];
+    }
+  }
+  // snow (:902-1035)
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qs[s] > QRMIN);
+  col_any(L, pred, fall);
+  if (wave_any(fall)) {
+    implicit_fall(L, X, sh, G, f, dz, X.qs, X.vts, P.dts, fall, sedi_w, flux, surf);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      m1_sol[s] = m1_sol[s] + flux[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_SNOW][L.cl[s]] += surf[s];
+    }
+  }
+  // graupel (:1037-1168)
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qg[s] > QRMIN);
+  col_any(L, pred, fall);
+  if (wave_any(fall)) {
+    implicit_fall(L, X, sh, G, f, dz, X.qg, X.vtg, P.dts, fall, sedi_w, flux, surf);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      m1_sol[s] = m1_sol[s] + flux[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_GRAUPEL][L.cl[s]] += surf[s];
+    }
+  }
+  __syncthreads();
+  if (P.nml.do_sedi_heat) {
+    bool all[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) all[s] = L.v[s];
+    sedi_heat(L, X, sh, m1_sol, C_ICE, all);
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) sh.cell[CL_M1SOL][64 * s + lane] = m1_sol[s];
+}
+
+// ------------------------------------------------------------ icloud (:1220-1449)
+template <int S, class R>
+__device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                       const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c) {
+  const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr, ql_mlt = P.nml.ql_mlt;
+  const bool slope = P.nml.z_slope_ice != 0;
+  const int lane = lane_id();
+  // With the ice slope limiter the whole column's melt / freeze must be done
+  // before the limiter sees cloud ice; the heat-capacity terms it leaves are
+  // parked in LDS cells that are free during icloud (m1_sol is consumed by
+  // the second warm_rain, dz1 is rebuilt by the next sub-step).
+  double* q_liq_l = sh.cell[CL_M1SOL];
+  double* q_sol_l = sh.cell[CL_DZ1];
+  double* cvm_l = sh.cell[CL_TMP];
+  double dl[S];
+  if (slope) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      Cell cs = cell_of(X, s);
+      R q_liq, q_sol, cvm;
+      icloud_melt_freeze<R>(c, tice_d, t_wfr_d, ql_mlt, cs, sh.at(CL_DEN, s), q_liq, q_sol, cvm);
+      set_cell(X, s, cs);
+      q_liq_l[64 * s + lane] = double(q_liq);
+      q_sol_l[64 * s + lane] = double(q_sol);
+      cvm_l[64 * s + lane] = double(cvm);
+    }
+    slope_limiter(L, sh, X.qi, dl);
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int cl = L.cl[s];
+    const double hv = sh.colv[CV_HVAR][cl];
+    const double den = sh.at(CL_DEN, s);
+    Cell cs = cell_of(X, s);
+    R q_liq, q_sol, cvm;
+    double di;
+    if (slope) {
+      q_liq = R(q_liq_l[64 * s + lane]);
+      q_sol = R(q_sol_l[64 * s + lane]);
+      cvm = R(cvm_l[64 * s + lane]);
+      di = mmax(dl[s], mmax(QVMIN, hv * cs.qi));
+    } else {
+      icloud_melt_freeze<R>(c, tice_d, t_wfr_d, ql_mlt, cs, den, q_liq, q_sol, cvm);
+      di = mmax(QVMIN, hv * cs.qi);
+    }
+    // p1 = den0 rdgas t0 from the call's input pt, delz (:149-151)
+    const int32_t o = G.off[s];
+    const bool v = L.v[s];
+    const double t0 = v ? f.pt[o] : 250.0;
+    const double dz0 = v ? f.delz[o] : -100.0;
+    const double p1 = dry_density(X.dp1[s], dz0) * RDGAS * t0;
+    icloud_main<R>(c, tice_d, P.tice0, cs, R(hv), sh.colv[CV_RHADJ][cl], sh.colv[CV_RHRAIN][cl], den,
+                   R(sh.at(CL_DENFAC, s)), X.vtg[s], X.vtr[s], X.vts[s], p1, R(di), q_liq, q_sol,
+                   cvm);
+    set_cell(X, s, cs);
+  }
+}
+
+// One sub-step (microphysics.py:2045-2220).
+template <int S, class R>
+__device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
+                                        const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c) {
+  warm_rain(L, X, sh, G, f, P, c, true);
+  sedimentation(L, X, sh, G, f, P, c);
+  warm_rain(L, X, sh, G, f, P, c, false);
+  icloud(L, X, sh, G, f, P, c);
+}
+
+// ------------------------------------------------------------------ kernel
+// MULTI: more than one sub-step.  With a single sub-step the body is not
+// wrapped in a loop: loop-invariant code motion would otherwise keep every
+// hoisted constant of the process code in a register across the whole body.
+template <int S, class R, bool MULTI>
+__global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<R> c,
+                                                        shield_mp_fields f, int cols_per_wave) {
+  __shared__ Shared<S> sh;
+  const int lane = lane_id();
+  const int nz = (int)sh3.nk;
+  const int64_t ncol = sh3.ni * sh3.nj;
+  const int64_t c0 = (int64_t)blockIdx.x * cols_per_wave;
+  const int ncols = (int)((ncol - c0) < cols_per_wave ? (ncol - c0) : cols_per_wave);
+  const shield_mp_namelist& n = P.nml;
+
+  Lay<S> L;
+  Glob<S> G;
+  L.nz = nz;
+  L.ncols = ncols;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int idx = 64 * s + lane;
+    const int cl = idx / nz;
+    const int k = idx - cl * nz;
+    const bool v = cl < ncols;
+    L.v[s] = v;
+    L.k[s] = v ? k : 0;
+    L.kb[s] = v ? nz - 1 - k : 0;
+    L.cl[s] = v ? cl : 0;
+    const int64_t col = c0 + L.cl[s];
+    const int64_t i = col / sh3.nj, j = col - i * sh3.nj;
+    G.off[s] = (int32_t)(i * sh3.si + j * sh3.sj + L.k[s]);
+  }
+
+  // ------------------------------------------- per-column scalars (:173-199)
+  if (lane < ncols) {
+    const int64_t col = c0 + lane;
+    const int64_t i = col / sh3.nj, j = col - i * sh3.nj;
+    const int64_t o2 = i * sh3.si2 + j * sh3.sj2;
+    const double land = f.land[o2], area = f.area[o2];
+    const double s_leng = sqrt(sqrt(area * 1.0e-10));
+    const double t_land = n.dw_land * s_leng;
+    const double t_ocean = n.dw_ocean * s_leng;
+    double h_var = t_land * land + t_ocean * (1.0 - land);
+    h_var = mmin(0.2, mmax(0.01, h_var));
+    const double rh_adj = 1.0 - h_var - n.rh_inc;
+    sh.colv[CV_HVAR][lane] = h_var;
+    sh.colv[CV_RHADJ][lane] = rh_adj;
+    sh.colv[CV_RHRAIN][lane] = mmax(0.35, rh_adj - n.rh_inr);
+    sh.colv[CV_CCN][lane] = (n.ccn_l * land + n.ccn_o * (1.0 - land)) * 1.0e6;
+    sh.colv[CV_RAIN][lane] = 0.0;
+    sh.colv[CV_SNOW][lane] = 0.0;
+    sh.colv[CV_ICE][lane] = 0.0;
+    sh.colv[CV_GRAUPEL][lane] = 0.0;
+  }
+  __syncthreads();
+
+  // ---------------------------------------------------- fields_init (:101-276)
+  // de_ice (:110-120) only acts when the incoming tendencies are kept: with
+  // overwritten (zero) tendencies qi never exceeds max(qi, qi0_max).  It
+  // rewrites state.qice, state.qsnow and both tendencies in place, as the
+  // reference does; fields_update then reads them back.
+  const bool de_ice = n.de_ice && P.accumulate;
+  St<S, R> X;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (L.v[s]) {
+      const int32_t o = G.off[s];
+      double qi = f.qi[o], qs = f.qs[o];
+      if (de_ice) {
+        const double qi_dt = f.qi_dt[o];
+        const double qio = qi - P.timestep * qi_dt;
+        const double qin = mmax(qio, n.qi0_max);
+        if (qi > qin) {
+          qs = qs + qi - qin;
+          qi = qin;
+          const double dqi = (qin - qio) * P.rdt;
+          f.qs_dt[o] = f.qs_dt[o] + qi_dt - dqi;
+          f.qi_dt[o] = dqi;
+          f.qs[o] = qs;
+          f.qi[o] = qi;
+        }
+      }
+      X.tz[s] = f.pt[o];
+      const double dp0 = f.delp[o];
+      const double qv = f.qv[o];
+      X.dp1[s] = dp0 * (1.0 - qv);
+      const double omq = dp0 / X.dp1[s];
+      // the products are materialised exactly (launder) because
+      // fields_update forms the same products again and subtracts
+      X.qv[s] = launder(qv * omq);
+      X.ql[s] = launder(f.ql[o] * omq);
+      X.qr[s] = launder(f.qr[o] * omq);
+      X.qi[s] = launder(qi * omq);
+      X.qs[s] = launder(qs * omq);
+      X.qg[s] = launder(f.qg[o] * omq);
+      X.qaz[s] = n.reset_qaz ? 0.0 : f.qaz[o];
+    } else {
+      X.tz[s] = 250.0;
+      X.dp1[s] = 100.0;
+      X.qv[s] = X.ql[s] = X.qr[s] = X.qi[s] = X.qs[s] = X.qg[s] = X.qaz[s] = 0.0;
+    }
+    X.vtr[s] = X.vtg[s] = X.vts[s] = R(VF_MIN);
+  }
+  // ccn scaled by the surface-layer density, then c_praut (:174-186)
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (L.v[s] && L.kb[s] == 0) {
+      const int cl = L.cl[s];
+      double ccn = sh.colv[CV_CCN][cl];
+      if (P.use_ccn) {
+        const int32_t o = G.off[s];
+        const double p1 = dry_density(X.dp1[s], f.delz[o]) * RDGAS * f.pt[o];
+        ccn = ccn * RDGAS * X.tz[s] / p1;
+      }
+      sh.colv[CV_CCN][cl] = ccn;
+      sh.colv[CV_CPRAUT][cl] = P.cpaut * pow(ccn * RHOR, -1.0 / 3.0);
+    }
+  }
+  __syncthreads();
+  if (n.fix_negative) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
+                         (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
+      const double lcpk = (P.lv00 + P.d0_vap * X.tz[s]) / cvm;
+      const double icpk = (LI00 + DC_ICE * X.tz[s]) / cvm;
+      if (X.qi[s] < 0.0) {
+        X.qs[s] = X.qs[s] + X.qi[s];
+        X.qi[s] = 0.0;
+      }
+      if (X.qs[s] < 0.0) {
+        X.qg[s] = X.qg[s] + X.qs[s];
+        X.qs[s] = 0.0;
+      }
+      if (X.qg[s] < 0.0) {
+        X.qr[s] = X.qr[s] + X.qg[s];
+        X.tz[s] = X.tz[s] - X.qg[s] * icpk;
+        X.qg[s] = 0.0;
+      }
+      if (X.qr[s] < 0.0) {
+        X.ql[s] = X.ql[s] + X.qr[s];
+        X.qr[s] = 0.0;
+      }
+      if (X.ql[s] < 0.0) {
+        X.qv[s] = X.qv[s] + X.ql[s];
+        X.tz[s] = X.tz[s] - X.ql[s] * lcpk;
+        X.ql[s] = 0.0;
+      }
+    }
+    // vapour borrowing is a non-linear recurrence: one lane per column walks
+    // its column in LDS (:244-276); dp1 is staged where m1 lives later
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      sh.buf[64 * s + lane] = X.qv[s];
+      sh.cell[CL_M1][64 * s + lane] = X.dp1[s];
+    }
+    __syncthreads();
+    if (lane < ncols) {
+      double* q = sh.buf + lane * nz;
+      const double* dp = sh.cell[CL_M1] + lane * nz;
+      for (int k = 1; k < nz; ++k)
+        if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * dp[k - 1] / dp[k];
+      for (int k = 0; k < nz - 1; ++k)
+        if (q[k] < 0.0) q[k] = 0.0;
+      const int kb = nz - 1, ka = nz - 2;
+      if ((q[kb] < 0.0) && (q[ka] > 0.0)) {
+        const double dq = mmin(-q[kb] * dp[kb], q[ka] * dp[ka]);
+        q[ka] = q[ka] - dq / dp[ka];
+        q[kb] = q[kb] + dq / dp[kb];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; ++s) X.qv[s] = sh.buf[64 * s + lane];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    sh.cell[CL_M1][64 * s + lane] = 0.0;
+    sh.cell[CL_M1SOL][64 * s + lane] = 0.0;
+    sh.cell[CL_DZ1][64 * s + lane] = 1.0;
+  }
+
+  // --------------------------------------------------------- sub-steps (:2045)
+  if constexpr (MULTI) {
+    for (int it = 0; it < P.ntimes; ++it) substep(L, X, sh, G, f, P, c);
+  } else {
+    substep(L, X, sh, G, f, P, c);
+  }
+
+  // ---------------------------------------------------- fields_update (:1452-1553)
+  // The initial values (q0 = q_in dp0 / dp1, qa0, u0, v0, t0, dp0) are read
+  // again from the inputs and recomputed with the operations fields_init used.
+  const double rdt = P.rdt;
+  const bool acc = P.accumulate != 0;
+  if (n.sedi_transport) {
+    double m1[S], mp[S], a[S], bu[S], bv[S], u1[S], v1[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) m1[s] = sh.cell[CL_M1][64 * s + lane];
+    col_prev(sh.buf, m1, mp);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int32_t o = G.off[s];
+      const double u0 = L.v[s] ? f.ua[o] : 0.0;
+      const double v0 = L.v[s] ? f.va[o] : 0.0;
+      if (L.k[s] == 0) {
+        a[s] = 0.0;
+        bu[s] = u0;
+        bv[s] = v0;
+      } else {
+        const double dp0 = L.v[s] ? f.delp[o] : 100.0;
+        const double r = 1.0 / (dp0 + mp[s]);
+        a[s] = mp[s] * r;
+        bu[s] = dp0 * u0 * r;
+        bv[s] = dp0 * v0 * r;
+      }
+    }
+    scan_fwd(L, a, bu, u1);
+    scan_fwd(L, a, bv, v1);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!L.v[s]) continue;
+      const int32_t o = G.off[s];
+      if (L.k[s] > 0) {
+        const double u0 = f.ua[o], v0 = f.va[o];
+        f.udt[o] = (acc ? f.udt[o] : 0.0) + (u1[s] - u0) * rdt;
+        f.vdt[o] = (acc ? f.vdt[o] : 0.0) + (v1[s] - v0) * rdt;
+      } else if (!acc) {
+        f.udt[o] = 0.0;
+        f.vdt[o] = 0.0;
+      }
+    }
+  } else if (!acc) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (L.v[s]) {
+        f.udt[G.off[s]] = 0.0;
+        f.vdt[G.off[s]] = 0.0;
+      }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s]) continue;
+    const int32_t o = G.off[s];
+    const double dp0 = f.delp[o];
+    const double omq0 = dp0 / X.dp1[s];
+    const double omq = X.dp1[s] / dp0;
+    // q0 is rounded exactly as fields_init rounded it (launder keeps the
+    // product out of a fused multiply-add with the difference), so an
+    // untouched species has a zero tendency
+    f.qv_dt[o] = (acc ? f.qv_dt[o] : 0.0) + rdt * (X.qv[s] - launder(f.qv[o] * omq0)) * omq;
+    f.ql_dt[o] = (acc ? f.ql_dt[o] : 0.0) + rdt * (X.ql[s] - launder(f.ql[o] * omq0)) * omq;
+    f.qr_dt[o] = (acc ? f.qr_dt[o] : 0.0) + rdt * (X.qr[s] - launder(f.qr[o] * omq0)) * omq;
+    f.qi_dt[o] = (acc ? f.qi_dt[o] : 0.0) + rdt * (X.qi[s] - launder(f.qi[o] * omq0)) * omq;
+    f.qs_dt[o] = (acc ? f.qs_dt[o] : 0.0) + rdt * (X.qs[s] - launder(f.qs[o] * omq0)) * omq;
+    f.qg_dt[o] = (acc ? f.qg_dt[o] : 0.0) + rdt * (X.qg[s] - launder(f.qg[o] * omq0)) * omq;
+    const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
+                       (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
+    f.pt_dt[o] = (acc ? f.pt_dt[o] : 0.0) + rdt * (X.tz[s] - f.pt[o]) * cvm / CP_AIR;
+    if (n.do_qa)
+      f.qa_dt[o] = 0.0;
+    else
+      f.qa_dt[o] = (acc ? f.qa_dt[o] : 0.0) + rdt * (X.qaz[s] / P.ntimes - f.qa[o]);
+    f.qaz[o] = X.qaz[s];
+  }
+  if (lane < ncols) {
+    const int64_t col = c0 + lane;
+    const int64_t i = col / sh3.nj, j = col - i * sh3.nj;
+    const int64_t o2 = i * sh3.si2 + j * sh3.sj2;
+    const double convt = 86400.0 * rdt * RGRAV;
+    if (f.rain) f.rain[o2] = sh.colv[CV_RAIN][lane] * convt;
+    if (f.snow) f.snow[o2] = sh.colv[CV_SNOW][lane] * convt;
+    if (f.ice) f.ice[o2] = sh.colv[CV_ICE][lane] * convt;
+    if (f.graupel) f.graupel[o2] = sh.colv[CV_GRAUPEL][lane] * convt;
+  }
+}
+
+// Launch the kernel for one slot count: a grid of ceil(ncol / P) one-wave
+// workgroups.
+template <int S, class R>
+hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f, int P,
+                        hipStream_t stream) {
+  const int64_t ncol = sh.ni * sh.nj;
+  const int64_t grid = (ncol + P - 1) / P;
+  const Coef<R> c = make_coef<R>(p);
+  if (p.ntimes > 1)
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, true>), dim3((unsigned)grid), dim3(64), 0, stream, sh, p, c, f, P);
+  else
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, false>), dim3((unsigned)grid), dim3(64), 0, stream, sh, p, c, f, P);
+  return hipGetLastError();
+}
+
+}  // namespace shield

@@ -1,0 +1,11 @@
+// mp_forward_s1.hip -- the forward kernel for 1 slot(s) of 64 cells per
+// wavefront, FP64 and mixed precision (one file per slot count so the
+// instantiations compile in parallel).
+#include "mp_forward.h"
+
+namespace shield {
+template hipError_t launch_slots<1, float>(const shield_mp_shape&, const shield_mp_params&,
+                                            const shield_mp_fields&, int, hipStream_t);
+template hipError_t launch_slots<1, double>(const shield_mp_shape&, const shield_mp_params&,
+                                             const shield_mp_fields&, int, hipStream_t);
+}  // namespace shield

@@ -105,12 +105,12 @@ __device__ __forceinline__ R acr3d(R v1, R v2, R q1, R q2, R c, R cac_ik, R cac_
 }
 template <class R>
 __device__ __forceinline__ R smlt(R tc, R dqs, R qsrho, R psacw, R psacr, const R* k, R rho, R rhofac) {
-  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qsrho) + k[3] * mpow(qsrho, R(0.65625)) * msqrt(rhofac)) +
+  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qsrho) + k[3] * pw<R>(qsrho, 0.65625) * msqrt(rhofac)) +
          k[4] * tc * (psacw + psacr);
 }
 template <class R>
 __device__ __forceinline__ R gmlt(R tc, R dqs, R qgrho, R pgacw, R pgacr, const R* k, R rho) {
-  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qgrho) + k[3] * mpow(qgrho, R(0.6875)) / mpow(rho, R(0.25))) +
+  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qgrho) + k[3] * pw<R>(qgrho, 0.6875) / pw<R>(rho, 0.25)) +
          k[4] * tc * (pgacw + pgacr);
 }
 
@@ -142,7 +142,7 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
         dq = R(0.25) * R((q_minus - qsat) * (q_minus - qsat)) / dqh;
       R qden = R(s.qr) * den;
       R t2 = R(tin) * R(tin);
-      R evap = c.crevp[0] * t2 * dq * (c.crevp[1] * msqrt(qden) + c.crevp[2] * mexp(R(0.725) * mlog(qden))) /
+      R evap = c.crevp[0] * t2 * dq * (c.crevp[1] * msqrt(qden) + c.crevp[2] * pw_el<R>(qden, 0.725)) /
                (c.crevp[3] * t2 + c.crevp[4] * R(qsat) * den);
       double ev = mmin(s.qr, double(mmin(dt * evap, R(dqv) / (R(1.0) + lcpk * R(dqsdt)))));
       s.qr = s.qr - ev;
@@ -152,7 +152,7 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
       s.tz = s.tz - double(R(ev) * lhl / cvm);
     }
     if ((s.qr > QRMIN) && (s.ql > 1.0e-6) && (qsat < q_minus)) {
-      R sink = dt * denfac * c.cracw * mexp(R(0.95) * mlog(R(s.qr) * den));
+      R sink = dt * denfac * c.cracw * pw_el<R>(R(s.qr) * den, 0.95);
       double sk = cap<R>(double(sink / (R(1.0) + sink) * R(s.ql)), s.ql);
       s.ql = s.ql - sk;
       s.qr = s.qr + sk;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
 
 // ------------------------------------------------ f:269-341
 template <class R>
-__device__ __forceinline__ void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R tk, R den, R& vtg, R& vti,
+__device__ __forceinline__ void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R tc, R den, R& vtg, R& vti,
                                            R& vts) {
   R rhof = msqrt(mmin(R(10.0), R(SFCRHO) / den));
   if (c.const_vi) {
@@ -172,7 +172,6 @@ __device__ __forceinline__ void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R
     if (qi < R(THI)) {
       vti = R(VF_MIN);
     } else {
-      R tc = tk - c.tice;
       R v = (R(3.0) + mlog(qi * den) / c.log_10) * (tc * (R(AA) * tc + R(BB)) + R(CC)) + R(DD_FS) * tc + R(EE);
       v = vi0 * mexp(c.log_10 * v) * R(0.8);
       vti = mmin(c.vi_max, mmax(R(VF_MIN), v));
@@ -183,7 +182,7 @@ __device__ __forceinline__ void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R
   } else if (qs < R(THS)) {
     vts = R(VF_MIN);
   } else {
-    R v = c.vs_fac * R(VCONS) * rhof * mexp(R(0.0625) * mlog(qs * den / R(NORMS)));
+    R v = c.vs_fac * R(VCONS) * rhof * pw_el<R>(qs * den / R(NORMS), 0.0625);
     vts = mmin(c.vs_max, mmax(R(VF_MIN), v));
   }
   if (c.const_vg) {
@@ -203,7 +202,7 @@ __device__ __forceinline__ R rain_fspeed(const Coef<R>& c, bool no_fall, R qrz, 
   if (c.const_vr) return c.vr_fac;
   R qden = qrz * den;
   if (qrz < R(THR)) return R(VR_MIN);
-  R v = c.vr_fac * R(VCONR) * msqrt(mmin(R(10.0), R(SFCRHO) / den)) * mexp(R(0.2) * mlog(qden / R(NORMR)));
+  R v = c.vr_fac * R(VCONR) * msqrt(mmin(R(10.0), R(SFCRHO) / den)) * pw_el<R>(qden / R(NORMR), 0.2);
   return mmin(c.vr_max, mmax(R(VR_MIN), v));
 }
 
@@ -214,9 +213,9 @@ __device__ __forceinline__ void autoconv_no_subgrid_var(const Coef<R>& c, double
   R qc0 = c.fac_rc * ccn;
   if (s.tz > t_wfr) {
     R qc = c.use_ccn ? qc0 : qc0 / den;
-    R dq = R(s.ql) - qc;
+    R dq = R(s.ql - double(qc));
     if (dq > R(0.0)) {
-      double sink = mmin(double(dq), double(c.dt_rain * c_praut * den * mexp(c.so3 * mlog(R(s.ql)))));
+      double sink = mmin(double(dq), double(c.dt_rain * c_praut * den * pw_el<R>(R(s.ql), 7.0 / 3.0)));
       sink = cap<R>(sink, s.ql);
       s.ql = s.ql - sink;
       s.qr = s.qr + sink;
@@ -230,9 +229,9 @@ __device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_
   if (s.tz > t_wfr + DT_FR) {
     dl = mmin(mmax(R(1.0e-6), dl), R(0.5) * R(s.ql));
     R qc = c.use_ccn ? qc0 : qc0 / den;
-    R dq = R(0.5) * (R(s.ql) + dl - qc);
+    R dq = R(0.5 * (s.ql + double(dl) - double(qc)));
     if (dq > R(0.0)) {
-      R sink = mmin(R(1.0), dq / dl) * c.dt_rain * c_praut * den * mexp(c.so3 * mlog(R(s.ql)));
+      R sink = mmin(R(1.0), dq / dl) * c.dt_rain * c_praut * den * pw_el<R>(R(s.ql), 7.0 / 3.0);
       double sk = cap<R>(double(sink), s.ql);
       s.ql = s.ql - sk;
       s.qr = s.qr + sk;
@@ -244,7 +243,7 @@ __device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_
 template <class R>
 __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, Cell& s, R h_var, double rh_adj,
                                                double rh_rain, double den_d, R denfac, double p1) {
-  const R tice = c.tice, t_wfr = c.t_wfr, t_sub = c.t_sub, dts = c.dts;
+  const R tice = c.tice, t_wfr = c.t_wfr, dts = c.dts;
   const R den = R(den_d);
   R lhl = c.lv00 + c.d0_vap * R(s.tz);
   R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
@@ -254,7 +253,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   R lcpk = lhl / cvm;
   R icpk = lhi / cvm;
   R tcpk = lcpk + icpk;
-  R tcp3 = lcpk + icpk * mmin(R(1.0), dim(tice, R(s.tz)) / (tice - t_wfr));
+  R tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
   if (!(p1 >= P_MIN)) return;
   if (s.tz < T_MIN) {
     double sink = dim(1.0e-7, s.qv);
@@ -271,7 +270,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   lcpk = lhl / cvm;
   icpk = lhi / cvm;
   tcpk = lcpk + icpk;
-  tcp3 = lcpk + icpk * mmin(R(1.0), dim(tice, R(s.tz)) / (tice - t_wfr));
+  tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
 
   // instant evaporation / sublimation of all clouds when rh < rh_adj
   const double qpz = s.qv + s.ql + s.qi;
@@ -312,7 +311,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   icpk = lhi / cvm;
 
   // complete freezing below t_wfr
-  R dtmp = t_wfr - R(s.tz);
+  R dtmp = R(c.t_wfr_d - s.tz);
   if ((dtmp > R(0.0)) && (s.ql > QCMIN)) {
     double sink = mmin(s.ql, double(mmin(R(s.ql) * dtmp * R(0.125), dtmp / icpk)));
     s.ql = s.ql - sink;
@@ -331,7 +330,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     dt_pisub = R(0.5) * dts;
   } else {
     dt_pisub = dts;
-    R tc = tice - R(s.tz);
+    R tc = R(tice_d - s.tz);
     if ((s.ql > QRMIN) && (tc > R(0.0))) {
       R sink = R(3.3333e-10) * dts * (mexp(R(0.66) * tc) - R(1.0)) * den * R(s.ql) * R(s.ql);
       double sk = mmin(s.ql, double(mmin(tc / icpk, sink)));
@@ -357,19 +356,19 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     R sink = R(dq) / (R(1.0) + tcpk * R(dqsdt));
     R pidep;
     if (s.qi > QRMIN) {
-      pidep = dt_pisub * R(dq) * R(349138.78) * mexp(R(0.875) * mlog(R(s.qi) * den)) /
+      pidep = dt_pisub * R(dq) * R(349138.78) * pw_el<R>(R(s.qi) * den, 0.875) /
               (R(qsi) * den * R(LAT2) / (R(0.0243) * R(RVGAS) * (R(s.tz) * R(s.tz))) + R(4.42478e4));
     } else {
       pidep = R(0.0);
     }
     double sk;
     if (dq > 0.0) {
-      R tmp = tice - R(s.tz);
+      R tmp = R(tice_d - s.tz);
       R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * tmp) / den;
-      sk = double(mmin(sink, mmin(mmax(qi_crt - R(s.qi), pidep), tmp / tcpk)));
+      sk = double(mmin(sink, mmin(mmax(R(double(qi_crt) - s.qi), pidep), tmp / tcpk)));
       sk = cap<R>(sk, s.qv);
     } else {
-      pidep = pidep * mmin(R(1.0), dim(R(s.tz), t_sub) * R(0.2));
+      pidep = pidep * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.2));
       sk = mmax(double(mmax(pidep, sink)), -s.qi);
     }
     s.qv = s.qv - sk;
@@ -389,7 +388,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     double dqsdt;
     const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
     R qden = R(s.qs) * den;
-    R tmp = mexp(R(0.65625) * mlog(qden));
+    R tmp = pw_el<R>(qden, 0.65625);
     R tsq = R(s.tz) * R(s.tz);
     const R dsat = R(qsi - s.qv);
     R dq = dsat / (R(1.0) + tcpk * R(dqsdt));
@@ -398,12 +397,12 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     pssub = dsat * dts * pssub;
     double ps;
     if (pssub > R(0.0)) {
-      ps = mmin(double(pssub * mmin(R(1.0), dim(R(s.tz), t_sub) * R(0.2))), s.qs);
+      ps = mmin(double(pssub * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.2))), s.qs);
     } else {
       if (s.tz > tice_d)
         ps = 0.0;
       else
-        ps = cap<R>(-double(mmax(pssub, mmax(dq, (R(s.tz) - tice) / tcpk))), s.qv) * -1.0;
+        ps = cap<R>(-double(mmax(pssub, mmax(dq, R(s.tz - tice_d) / tcpk))), s.qv) * -1.0;
     }
     s.qs = s.qs - ps;
     s.qv = s.qv + ps;
@@ -429,10 +428,10 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
         pg = 0.0;
       else
         pg = cap<R>(double(mmin(mmin(c.fac_v2g * pgsub, R(0.2) * dq),
-                                mmin(R(s.ql) + R(s.qr), (tice - R(s.tz)) / tcpk))),
+                                mmin(R(s.ql) + R(s.qr), R(tice_d - s.tz) / tcpk))),
                     s.qv);
     } else {
-      pg = -cap<R>(-double(mmax(c.fac_g2v * pgsub, dq) * mmin(R(1.0), dim(R(s.tz), t_sub) * R(0.1))), s.qg);
+      pg = -cap<R>(-double(mmax(c.fac_g2v * pgsub, dq) * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.1))), s.qg);
     }
     s.qg = s.qg + pg;
     s.qv = s.qv - pg;
@@ -535,20 +534,19 @@ template <class R>
 __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, Cell& s, R h_var,
                                             double rh_adj, double rh_rain, double den_d, R denfac, R vtgz, R vtrz,
                                             R vtsz, double p1d, R di, R q_liq, R q_sol, R cvm) {
-  const R tice = c.tice, dts = c.dts, rdts = c.rdts;
+  const R dts = c.dts, rdts = c.rdts;
   const R den = R(den_d);
   R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
   R icpk = lhi / cvm;
   if (p1d >= P_MIN) {
-    const R p1 = R(p1d);
     R pgacr = R(0.0), pgacw = R(0.0);
-    R tc = R(s.tz) - tice;
+    R tc = R(s.tz - tice_d);
     if (s.tz - tice_d >= 0.0) {
       R dqs0 = R(double(c.ces0) / p1d - s.qv);
       if (s.qs > QCMIN) {
         R psacw, psacr, pracs;
         if (s.ql > QRMIN) {
-          R factor = denfac * c.csacw * mexp(R(0.8125) * mlog(R(s.qs) * den));
+          R factor = denfac * c.csacw * pw_el<R>(R(s.qs) * den, 0.8125);
           psacw = factor / (R(1.0) + dts * factor) * R(s.ql);
         } else {
           psacw = R(0.0);
@@ -571,7 +569,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         q_sol = q_sol - R(sink);
         cvm = SH_CVM(c, s, q_liq, q_sol);
         s.tz = s.tz - double(R(sink) * lhi / cvm);
-        tc = R(s.tz) - tice;
+        tc = R(s.tz - tice_d);
       }
       lhi = R(LI00) + R(DC_ICE) * R(s.tz);
       icpk = lhi / cvm;
@@ -597,7 +595,12 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
       if (s.qi > 3.0e-7) {
         R psaci;
         if (s.qs > 1.0e-7) {
-          R factor = dts * denfac * c.csaci * mexp(R(0.05) * tc + R(0.8125) * mlog(R(s.qs) * den));
+          R e;
+          if constexpr (is_f32<R>::value)
+            e = mexp(R(0.05) * tc) * pw_split(R(s.qs) * den, 0.8125);
+          else
+            e = mexp(R(0.05) * tc + R(0.8125) * mlog(R(s.qs) * den));
+          R factor = dts * denfac * c.csaci * e;
           psaci = factor / (R(1.0) + factor) * R(s.qi);
         } else {
           psaci = R(0.0);
@@ -605,14 +608,15 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         R qim = c.qi0_crt / den;
         R tmp = c.const_vi ? c.fac_i2s : c.fac_i2s * mexp(R(0.025) * tc);
         di = mmax(di, R(QRMIN));
-        R q_plus = R(s.qi) + di;
+        const double q_plus = s.qi + double(di);
+        const double qimd = double(qim);
         R psaut;
-        if (q_plus > (qim + R(QRMIN))) {
+        if (q_plus > (qimd + QRMIN)) {
           R dq;
-          if (qim > (R(s.qi) - di))
-            dq = (R(0.25) * ((q_plus - qim) * (q_plus - qim))) / di;
+          if (qimd > (s.qi - double(di)))
+            dq = (R(0.25) * R((q_plus - qimd) * (q_plus - qimd))) / di;
           else
-            dq = R(s.qi) - qim;
+            dq = R(s.qi - qimd);
           psaut = tmp * dq;
         } else {
           psaut = R(0.0);
@@ -627,19 +631,19 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           s.qg = s.qg + pgaci;
         }
       }
-      tc = R(s.tz) - tice;
+      tc = R(s.tz - tice_d);
       if ((s.qr > 1e-7) && (s.tz - tice_d < 0.0)) {
         R psacr;
         if (s.qs > 1.0e-7)
           psacr = dts * acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, c.acco[0][1], c.acco[1][1], c.acco[2][1], den);
         else
           psacr = R(0.0);
-        R pgfr = dts * c.cgfr[0] / den * (mexp(-c.cgfr[1] * tc) - R(1.0)) * mexp(R(1.75) * mlog(R(s.qr) * den));
-        R sink = psacr + pgfr;
-        R factor = mmin(sink, mmin(R(s.qr), -tc / icpk)) / mmax(sink, R(QRMIN));
-        psacr = factor * psacr;
-        pgfr = factor * pgfr;
-        double ps = double(psacr), pg = double(pgfr);
+        R pgfr = dts * c.cgfr[0] / den * (mexp(-c.cgfr[1] * tc) - R(1.0)) * pw_el<R>(R(s.qr) * den, 1.75);
+        // the limiter that can drain all rain is applied in FP64, so a fully
+        // frozen layer keeps no FP32 residue
+        const double sink = double(psacr) + double(pgfr);
+        const double factor = mmin(sink, mmin(s.qr, double(-tc / icpk))) / mmax(sink, QRMIN);
+        double ps = factor * double(psacr), pg = factor * double(pgfr);
         if constexpr (is_f32<R>::value) {
           double tot = ps + pg;
           if (tot > s.qr) {
@@ -666,9 +670,9 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         else
           sink = R(0.0);
         R qsm = c.qs0_crt / den;
-        if (R(s.qs) > qsm) {
-          R factor = dts * R(1.0e-3) * mexp(R(0.09) * (R(s.tz) - tice));
-          sink = sink + factor / (R(1.0) + factor) * (R(s.qs) - qsm);
+        if (s.qs > double(qsm)) {
+          R factor = dts * R(1.0e-3) * mexp(R(0.09) * R(s.tz - tice_d));
+          sink = sink + factor / (R(1.0) + factor) * R(s.qs - double(qsm));
         }
         double sk = mmin(s.qs, double(sink));
         s.qs = s.qs - sk;
@@ -687,15 +691,13 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
                        R(s.qr));
         else
           pgacr = R(0.0);
-        R sink = pgacr + pgacw;
-        R factor = mmin(sink, dim(tice, R(s.tz)) / icpk) / mmax(sink, R(QRMIN));
-        pgacr = factor * pgacr;
-        pgacw = factor * pgacw;
-        double pr = cap<R>(double(pgacr), s.qr), pw = cap<R>(double(pgacw), s.ql);
-        double sk = pr + pw;
+        const double sink = double(pgacr) + double(pgacw);
+        const double factor = mmin(sink, double(R(dim(tice_d, s.tz)) / icpk)) / mmax(sink, QRMIN);
+        const double pr = cap<R>(factor * double(pgacr), s.qr), pl = cap<R>(factor * double(pgacw), s.ql);
+        double sk = pr + pl;
         s.qg = s.qg + sk;
         s.qr = s.qr - pr;
-        s.ql = s.ql - pw;
+        s.ql = s.ql - pl;
         q_liq = q_liq - R(sk);
         q_sol = q_sol + R(sk);
         cvm = SH_CVM(c, s, q_liq, q_sol);

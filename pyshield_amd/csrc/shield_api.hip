@@ -184,6 +184,10 @@ int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p, c
   if (!(p->timestep > 0.0) || p->ntimes < 1) return SHIELD_ERR_TIMESTEP;
   if (!shape_ok(shape, 2, 320)) return SHIELD_ERR_SHAPE;
   if (shape->ni * shape->nj == 0) return SHIELD_OK;
+  // the kernel addresses cells with 32-bit element offsets
+  const int64_t last3 = (shape->ni - 1) * shape->si + (shape->nj - 1) * shape->sj + (shape->nk - 1);
+  const int64_t last2 = (shape->ni - 1) * shape->si2 + (shape->nj - 1) * shape->sj2;
+  if (last3 > INT32_MAX || last2 > INT32_MAX) return SHIELD_ERR_SHAPE;
   const void* in[] = {f->pt, f->qv, f->ql, f->qr, f->qi, f->qs, f->qg, f->qa, f->ua, f->va, f->delp, f->delz,
                       f->land, f->area};
   for (const void* q : in)
