@@ -152,7 +152,7 @@ def main():
     params = _lib.setup(cfg.mp_namelist(), w.timestep)
     params.accumulate = 0
     print(f"# workload {w.name}: {ncol} columns x {w.nz} levels, dt {w.timestep} s, "
-          f"layout {_lib.layout(w.nz)}", file=sys.stderr, flush=True)
+          f"layout {_lib.layout(w.nz, args.mode, params.ntimes)}", file=sys.stderr, flush=True)
 
     wall, kms = time_mode(f, params, args.mode, args.steps, args.warmup)
     ms_per_step = 1e3 * wall / args.steps

@@ -184,8 +184,11 @@ int shield_prepare_tendencies_and_update_tracers(
 const char* shield_strerror(int code);
 /* Number of HIP devices visible (0 without a GPU; never fails).            */
 int shield_device_count(void);
-/* Columns per wavefront and slots per lane the forward kernel picks for nk. */
-int shield_mp_layout(int64_t nk, int* cols_per_wave, int* slots);
+/* Columns per wavefront and slots per lane the forward kernel picks for nk
+ * levels in `mode` with `ntimes` sub-steps on the current device (the
+ * busiest-lanes layout when no device is visible).  No reference
+ * counterpart: a diagnostic of this implementation.                         */
+int shield_mp_layout(int64_t nk, int mode, int ntimes, int* cols_per_wave, int* slots);
 
 #ifdef __cplusplus
 }

@@ -114,7 +114,8 @@ def lib():
             [ctypes.POINTER(ShieldMpShape)] + [_VP] * 23 + [_D, _VP])
         L.shield_strerror.argtypes = [ctypes.c_int]
         L.shield_strerror.restype = ctypes.c_char_p
-        L.shield_mp_layout.argtypes = [_I64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.shield_mp_layout.argtypes = [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int)]
         for fn in EXPORTS:
             if fn != "shield_strerror":
                 getattr(L, fn).restype = ctypes.c_int
@@ -145,9 +146,12 @@ def setup(values: dict, timestep: float) -> ShieldMpParams:
     return p
 
 
-def layout(nk):
+def layout(nk, mode="mixed", ntimes=1):
+    """(columns per wavefront, slots) the forward kernel uses for nk levels."""
     P, S = ctypes.c_int(), ctypes.c_int()
-    check(lib().shield_mp_layout(int(nk), ctypes.byref(P), ctypes.byref(S)), "shield_mp_layout")
+    m = MODES[mode] if isinstance(mode, str) else int(mode)
+    check(lib().shield_mp_layout(int(nk), m, int(ntimes), ctypes.byref(P), ctypes.byref(S)),
+          "shield_mp_layout")
     return P.value, S.value
 
 

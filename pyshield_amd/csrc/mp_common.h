@@ -134,8 +134,10 @@ struct Coef {
   R vs_fac, vg_fac, vi_fac, vr_fac, vs_max, vg_max, vi_max, vr_max;
   int32_t const_vi, const_vs, const_vg, const_vr, fast_sat_adj, do_qa, rad_snow, rad_rain;
   int32_t use_ccn, pad_;
-  // FP64 copies of the thresholds the state is tested against
+  // FP64 copies of the thresholds the state is tested against and of the
+  // heat-capacity coefficients (latent heats and cvm are FP64 in both modes)
   double t_sub_d, t_wfr_d, qc_crt_d;
+  double c_air_d, c_vap_d, d0_vap_d, lv00_d;
 };
 
 template <class R>
@@ -166,6 +168,7 @@ inline Coef<R> make_coef(const shield_mp_params& p) {
   c.fast_sat_adj = n.fast_sat_adj; c.do_qa = n.do_qa; c.rad_snow = n.rad_snow; c.rad_rain = n.rad_rain;
   c.use_ccn = p.use_ccn; c.pad_ = 0;
   c.t_sub_d = n.t_sub; c.t_wfr_d = p.t_wfr; c.qc_crt_d = n.qc_crt;
+  c.c_air_d = p.c_air; c.c_vap_d = p.c_vap; c.d0_vap_d = p.d0_vap; c.lv00_d = p.lv00;
   return c;
 }
 

@@ -338,20 +338,21 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
   for (int s = 0; s < S; ++s) {
     const double tcd = X.tz[s] - tice_d;
     if (stop[s] && (X.qi[s] > QCMIN) && (tcd > 0.0)) {
-      const R lhi = R(LI00) + R(DC_ICE) * R(X.tz[s]);
-      R q_liq = R(X.ql[s]) + R(X.qr[s]);
-      R q_sol = R(X.qi[s]) + R(X.qs[s]) + R(X.qg[s]);
-      R cvm = c.c_air + R(X.qv[s]) * c.c_vap + q_liq * R(C_LIQ) + q_sol * R(C_ICE);
-      const R icpk = lhi / cvm;
-      const double sink = mmin(X.qi[s], double(c.fac_imlt * R(tcd) / icpk));
+      // latent heat and heat capacity in FP64 (both modes)
+      const double lhi = LI00 + DC_ICE * X.tz[s];
+      double q_liq = X.ql[s] + X.qr[s];
+      double q_sol = X.qi[s] + X.qs[s] + X.qg[s];
+      double cvm = c.c_air_d + X.qv[s] * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
+      const double icpk = lhi / cvm;
+      const double sink = mmin(X.qi[s], double(c.fac_imlt) * tcd / icpk);
       const double tmp = mmin(sink, dim(double(P.nml.ql_mlt), X.ql[s]));
       X.ql[s] = X.ql[s] + tmp;
       X.qr[s] = X.qr[s] + sink - tmp;
       X.qi[s] = X.qi[s] - sink;
-      q_liq = q_liq + R(sink);
-      q_sol = q_sol - R(sink);
-      cvm = c.c_air + R(X.qv[s]) * c.c_vap + q_liq * R(C_LIQ) + q_sol * R(C_ICE);
-      X.tz[s] = X.tz[s] - double(R(sink) * lhi / cvm);
+      q_liq = q_liq + sink;
+      q_sol = q_sol - sink;
+      cvm = c.c_air_d + X.qv[s] * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
+      X.tz[s] = X.tz[s] - sink * lhi / cvm;
     }
   }
   double dz[S], flux[S], surf[S], m1_sol[S];
@@ -426,12 +427,12 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       Cell cs = cell_of(X, s);
-      R q_liq, q_sol, cvm;
+      double q_liq, q_sol, cvm;
       icloud_melt_freeze<R>(c, tice_d, t_wfr_d, ql_mlt, cs, sh.at(CL_DEN, s), q_liq, q_sol, cvm);
       set_cell(X, s, cs);
-      q_liq_l[64 * s + lane] = double(q_liq);
-      q_sol_l[64 * s + lane] = double(q_sol);
-      cvm_l[64 * s + lane] = double(cvm);
+      q_liq_l[64 * s + lane] = q_liq;
+      q_sol_l[64 * s + lane] = q_sol;
+      cvm_l[64 * s + lane] = cvm;
     }
     slope_limiter(L, sh, X.qi, dl);
   }
@@ -441,12 +442,12 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
     const double hv = sh.colv[CV_HVAR][cl];
     const double den = sh.at(CL_DEN, s);
     Cell cs = cell_of(X, s);
-    R q_liq, q_sol, cvm;
+    double q_liq, q_sol, cvm;
     double di;
     if (slope) {
-      q_liq = R(q_liq_l[64 * s + lane]);
-      q_sol = R(q_sol_l[64 * s + lane]);
-      cvm = R(cvm_l[64 * s + lane]);
+      q_liq = q_liq_l[64 * s + lane];
+      q_sol = q_sol_l[64 * s + lane];
+      cvm = cvm_l[64 * s + lane];
       di = mmax(dl[s], mmax(QVMIN, hv * cs.qi));
     } else {
       icloud_melt_freeze<R>(c, tice_d, t_wfr_d, ql_mlt, cs, den, q_liq, q_sol, cvm);
@@ -766,6 +767,21 @@ hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, co
   else
     hipLaunchKernelGGL((mp_forward_kernel<S, R, false>), dim3((unsigned)grid), dim3(64), 0, stream, sh, p, c, f, P);
   return hipGetLastError();
+}
+
+// Resident wavefronts per SIMD of one kernel instance on the current device
+// (0 when the query fails, e.g. without a device).
+template <int S, class R>
+int waves_per_simd(bool multi) {
+  int blocks = 0;
+  const hipError_t e =
+      multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, true>, 64, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, false>, 64, 0);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return blocks / 4;  // one-wave workgroups over the 4 SIMDs of a CU
 }
 
 }  // namespace shield

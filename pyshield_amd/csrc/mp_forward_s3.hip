@@ -8,4 +8,6 @@ template hipError_t launch_slots<3, float>(const shield_mp_shape&, const shield_
                                             const shield_mp_fields&, int, hipStream_t);
 template hipError_t launch_slots<3, double>(const shield_mp_shape&, const shield_mp_params&,
                                              const shield_mp_fields&, int, hipStream_t);
+template int waves_per_simd<3, float>(bool);
+template int waves_per_simd<3, double>(bool);
 }  // namespace shield

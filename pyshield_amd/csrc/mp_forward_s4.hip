@@ -8,4 +8,6 @@ template hipError_t launch_slots<4, float>(const shield_mp_shape&, const shield_
                                             const shield_mp_fields&, int, hipStream_t);
 template hipError_t launch_slots<4, double>(const shield_mp_shape&, const shield_mp_params&,
                                              const shield_mp_fields&, int, hipStream_t);
+template int waves_per_simd<4, float>(bool);
+template int waves_per_simd<4, double>(bool);
 }  // namespace shield

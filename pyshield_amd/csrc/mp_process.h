@@ -2,14 +2,21 @@
 //
 // Each function follows the pySHiELD process library
 // (pySHiELD/functions/microphysics_funcs.py) statement by statement.  The
-// cell state `Cell` is FP64; rates are computed in the rate type R and
+// cell state `Cell` is FP64; process rates are computed in the rate type R and
 // applied as FP64 transfers (s.q_a -= x; s.q_b += x).  With R = double every
-// expression has the reference's operation order.  With R = float:
-//  * saturation mixing ratios and their gradients are evaluated in FP64 from
-//    the FP64 temperature and density, and every difference between a
-//    saturation value and the state (qsat - qv, ...) is formed in FP64 before
-//    it is narrowed; FP32 would put ~1e-6 of qsat (1e-8 kg/kg) into every
-//    saturation adjustment;
+// expression has the reference's operation order.  With R = float (mixed
+// mode) the rate formulas (collection, evaporation, deposition, melting,
+// fall-speed and autoconversion laws) run in FP32, and what decides how much
+// of a species survives stays FP64:
+//  * saturation mixing ratios and their gradients, from the FP64 temperature
+//    and density, and every difference between a saturation value and the
+//    state (qsat - qv, ...); FP32 would put ~1e-6 of qsat (1e-8 kg/kg) into
+//    every saturation adjustment;
+//  * latent heats, the moist heat capacity cvm and their ratios, so a
+//    transfer limited by the heat that brings a cell to tice or t_wfr is
+//    accurate to FP64;
+//  * temperature differences (tz - tice, ...) and the limiters that scale a
+//    set of sinks to what is available;
 //  * limiters that bound a sink by the species it drains are evaluated
 //    against the FP64 state (`cap`), so no species is driven below zero by
 //    FP32 rounding.
@@ -40,8 +47,8 @@ __device__ __forceinline__ double cap(double x, double q) {
   return x;
 }
 
-#define SH_CVM(c, s, q_liq, q_sol) \
-  ((c).c_air + R((s).qv) * (c).c_vap + (q_liq) * R(C_LIQ) + (q_sol) * R(C_ICE))
+// moist heat capacity (FP64 in both modes)
+#define SH_CVM(c, s, q_liq, q_sol) ((c).c_air_d + (s).qv * (c).c_vap_d + (q_liq) * C_LIQ + (q_sol) * C_ICE)
 
 // ------------------------------------------------ saturation (f:53-145)
 template <class R>
@@ -120,11 +127,11 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
                                            R denfac) {
   if ((s.tz > t_wfr) && (s.qr > QRMIN)) {
     const R den = R(den_d);
-    R lhl = c.lv00 + c.d0_vap * R(s.tz);
-    R q_liq = R(s.ql) + R(s.qr);
-    R q_sol = R(s.qi) + R(s.qs) + R(s.qg);
-    R cvm = SH_CVM(c, s, q_liq, q_sol);
-    R lcpk = lhl / cvm;
+    double lhl = c.lv00_d + c.d0_vap_d * s.tz;
+    double q_liq = s.ql + s.qr;
+    double q_sol = s.qi + s.qs + s.qg;
+    double cvm = SH_CVM(c, s, q_liq, q_sol);
+    double lcpk = lhl / cvm;
     const double tin = s.tz - double(lcpk * R(s.ql));
     const double qpz = s.qv + s.ql;
     double dqsdt;
@@ -144,12 +151,12 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
       R t2 = R(tin) * R(tin);
       R evap = c.crevp[0] * t2 * dq * (c.crevp[1] * msqrt(qden) + c.crevp[2] * pw_el<R>(qden, 0.725)) /
                (c.crevp[3] * t2 + c.crevp[4] * R(qsat) * den);
-      double ev = mmin(s.qr, double(mmin(dt * evap, R(dqv) / (R(1.0) + lcpk * R(dqsdt)))));
+      double ev = mmin(s.qr, mmin(double(dt * evap), dqv / (1.0 + lcpk * dqsdt)));
       s.qr = s.qr - ev;
       s.qv = s.qv + ev;
-      q_liq = q_liq - R(ev);
+      q_liq = q_liq - ev;
       cvm = SH_CVM(c, s, q_liq, q_sol);
-      s.tz = s.tz - double(R(ev) * lhl / cvm);
+      s.tz = s.tz - ev * lhl / cvm;
     }
     if ((s.qr > QRMIN) && (s.ql > 1.0e-6) && (qsat < q_minus)) {
       R sink = dt * denfac * c.cracw * pw_el<R>(R(s.qr) * den, 0.95);
@@ -245,28 +252,28 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
                                                double rh_rain, double den_d, R denfac, double p1) {
   const R tice = c.tice, t_wfr = c.t_wfr, dts = c.dts;
   const R den = R(den_d);
-  R lhl = c.lv00 + c.d0_vap * R(s.tz);
-  R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
-  R q_liq = R(s.ql) + R(s.qr);
-  R q_sol = R(s.qi) + R(s.qs) + R(s.qg);
-  R cvm = SH_CVM(c, s, q_liq, q_sol);
-  R lcpk = lhl / cvm;
-  R icpk = lhi / cvm;
-  R tcpk = lcpk + icpk;
-  R tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
+  double lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  double lhi = LI00 + DC_ICE * s.tz;
+  double q_liq = s.ql + s.qr;
+  double q_sol = s.qi + s.qs + s.qg;
+  double cvm = SH_CVM(c, s, q_liq, q_sol);
+  double lcpk = lhl / cvm;
+  double icpk = lhi / cvm;
+  double tcpk = lcpk + icpk;
+  double tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
   if (!(p1 >= P_MIN)) return;
   if (s.tz < T_MIN) {
     double sink = dim(1.0e-7, s.qv);
     s.qv = s.qv - sink;
     s.qi = s.qi + sink;
-    q_sol = q_sol + R(sink);
+    q_sol = q_sol + sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + double(R(sink) * (lhl + lhi) / cvm);
+    s.tz = s.tz + sink * (lhl + lhi) / cvm;
     if (!c.do_qa) s.qa = s.qa + 1.0;
     return;
   }
-  lhl = c.lv00 + c.d0_vap * R(s.tz);
-  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  lhi = LI00 + DC_ICE * s.tz;
   lcpk = lhl / cvm;
   icpk = lhi / cvm;
   tcpk = lcpk + icpk;
@@ -274,9 +281,9 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // instant evaporation / sublimation of all clouds when rh < rh_adj
   const double qpz = s.qv + s.ql + s.qi;
-  const R dtin = (lhl * (R(s.ql) + R(s.qi)) + lhi * R(s.qi)) /
-                 (c.c_air + R(qpz) * c.c_vap + R(s.qr) * R(C_LIQ) + (R(s.qs) + R(s.qg)) * R(C_ICE));
-  const double tin = s.tz - double(dtin);
+  const double dtin = (lhl * (s.ql + s.qi) + lhi * s.qi) /
+                      (c.c_air_d + qpz * c.c_vap_d + s.qr * C_LIQ + (s.qs + s.qg) * C_ICE);
+  const double tin = s.tz - dtin;
   const double t_sub6 = c.t_sub_d + 6.0;
   double rh = 0.0;
   if (tin > t_sub6) {
@@ -298,30 +305,30 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   double evap;
   if (dq0 > 0.0) {
     R factor = mmin(R(1.0), c.fac_l2v * R(10.0 * dq0 / qsw));
-    evap = mmin(s.ql, double(factor * R(dq0) / (R(1.0) + tcp3 * R(dwsdt))));
+    evap = mmin(s.ql, double(factor) * dq0 / (1.0 + tcp3 * dwsdt));
   } else {
-    evap = double(R(dq0) / (R(1.0) + tcp3 * R(dwsdt)));
+    evap = dq0 / (1.0 + tcp3 * dwsdt);
   }
   s.qv = s.qv + evap;
   s.ql = s.ql - evap;
-  q_liq = q_liq - R(evap);
+  q_liq = q_liq - evap;
   cvm = SH_CVM(c, s, q_liq, q_sol);
-  s.tz = s.tz - double(R(evap) * lhl / cvm);
-  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  s.tz = s.tz - evap * lhl / cvm;
+  lhi = LI00 + DC_ICE * s.tz;
   icpk = lhi / cvm;
 
   // complete freezing below t_wfr
-  R dtmp = R(c.t_wfr_d - s.tz);
-  if ((dtmp > R(0.0)) && (s.ql > QCMIN)) {
-    double sink = mmin(s.ql, double(mmin(R(s.ql) * dtmp * R(0.125), dtmp / icpk)));
+  const double dtmp = c.t_wfr_d - s.tz;
+  if ((dtmp > 0.0) && (s.ql > QCMIN)) {
+    double sink = mmin(s.ql, mmin(double(R(s.ql) * R(dtmp) * R(0.125)), dtmp / icpk));
     s.ql = s.ql - sink;
     s.qi = s.qi + sink;
-    q_liq = q_liq - R(sink);
-    q_sol = q_sol + R(sink);
+    q_liq = q_liq - sink;
+    q_sol = q_sol + sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + double(R(sink) * lhi / cvm);
+    s.tz = s.tz + sink * lhi / cvm;
   }
-  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lhi = LI00 + DC_ICE * s.tz;
   icpk = lhi / cvm;
 
   // Bigg mechanism
@@ -330,20 +337,21 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     dt_pisub = R(0.5) * dts;
   } else {
     dt_pisub = dts;
-    R tc = R(tice_d - s.tz);
-    if ((s.ql > QRMIN) && (tc > R(0.0))) {
+    const double tcd = tice_d - s.tz;
+    const R tc = R(tcd);
+    if ((s.ql > QRMIN) && (tcd > 0.0)) {
       R sink = R(3.3333e-10) * dts * (mexp(R(0.66) * tc) - R(1.0)) * den * R(s.ql) * R(s.ql);
-      double sk = mmin(s.ql, double(mmin(tc / icpk, sink)));
+      double sk = mmin(s.ql, mmin(tcd / icpk, double(sink)));
       s.ql = s.ql - sk;
       s.qi = s.qi + sk;
-      q_liq = q_liq - R(sk);
-      q_sol = q_sol + R(sk);
+      q_liq = q_liq - sk;
+      q_sol = q_sol + sk;
       cvm = SH_CVM(c, s, q_liq, q_sol);
-      s.tz = s.tz + double(R(sk) * lhi / cvm);
+      s.tz = s.tz + sk * lhi / cvm;
     }
   }
-  lhl = c.lv00 + c.d0_vap * R(s.tz);
-  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  lhi = LI00 + DC_ICE * s.tz;
   lcpk = lhl / cvm;
   icpk = lhi / cvm;
   tcpk = lcpk + icpk;
@@ -353,7 +361,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     double dqsdt;
     const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
     const double dq = s.qv - qsi;
-    R sink = R(dq) / (R(1.0) + tcpk * R(dqsdt));
+    const double sink = dq / (1.0 + tcpk * dqsdt);
     R pidep;
     if (s.qi > QRMIN) {
       pidep = dt_pisub * R(dq) * R(349138.78) * pw_el<R>(R(s.qi) * den, 0.875) /
@@ -363,22 +371,23 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     }
     double sk;
     if (dq > 0.0) {
-      R tmp = R(tice_d - s.tz);
+      const double tmpd = tice_d - s.tz;
+      const R tmp = R(tmpd);
       R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * tmp) / den;
-      sk = double(mmin(sink, mmin(mmax(R(double(qi_crt) - s.qi), pidep), tmp / tcpk)));
+      sk = mmin(sink, mmin(double(mmax(R(double(qi_crt) - s.qi), pidep)), tmpd / tcpk));
       sk = cap<R>(sk, s.qv);
     } else {
       pidep = pidep * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.2));
-      sk = mmax(double(mmax(pidep, sink)), -s.qi);
+      sk = mmax(mmax(double(pidep), sink), -s.qi);
     }
     s.qv = s.qv - sk;
     s.qi = s.qi + sk;
-    q_sol = q_sol + R(sk);
+    q_sol = q_sol + sk;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + double(R(sk) * (lhl + lhi) / cvm);
+    s.tz = s.tz + sk * (lhl + lhi) / cvm;
   }
-  lhl = c.lv00 + c.d0_vap * R(s.tz);
-  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  lhi = LI00 + DC_ICE * s.tz;
   lcpk = lhl / cvm;
   icpk = lhi / cvm;
   tcpk = lcpk + icpk;
@@ -390,8 +399,9 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     R qden = R(s.qs) * den;
     R tmp = pw_el<R>(qden, 0.65625);
     R tsq = R(s.tz) * R(s.tz);
-    const R dsat = R(qsi - s.qv);
-    R dq = dsat / (R(1.0) + tcpk * R(dqsdt));
+    const double dsat_d = qsi - s.qv;
+    const R dsat = R(dsat_d);
+    const double dq = dsat_d / (1.0 + tcpk * dqsdt);
     R pssub = c.cssub[0] * tsq * (c.cssub[1] * msqrt(qden) + c.cssub[2] * tmp * msqrt(denfac)) /
               (c.cssub[3] * tsq + c.cssub[4] * R(qsi) * den);
     pssub = dsat * dts * pssub;
@@ -402,16 +412,16 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
       if (s.tz > tice_d)
         ps = 0.0;
       else
-        ps = cap<R>(-double(mmax(pssub, mmax(dq, R(s.tz - tice_d) / tcpk))), s.qv) * -1.0;
+        ps = cap<R>(-mmax(double(pssub), mmax(dq, (s.tz - tice_d) / tcpk)), s.qv) * -1.0;
     }
     s.qs = s.qs - ps;
     s.qv = s.qv + ps;
-    q_sol = q_sol - R(ps);
+    q_sol = q_sol - ps;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz - double(R(ps) * (lhl + lhi) / cvm);
+    s.tz = s.tz - ps * (lhl + lhi) / cvm;
   }
-  lhl = c.lv00 + c.d0_vap * R(s.tz);
-  lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+  lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  lhi = LI00 + DC_ICE * s.tz;
   lcpk = lhl / cvm;
   icpk = lhi / cvm;
   tcpk = lcpk + icpk;
@@ -420,41 +430,41 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   if (s.qg > QRMIN) {
     double dqsdt;
     const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
-    R dq = R(s.qv - qsi) / (R(1.0) + tcpk * R(dqsdt));
+    const double dq = (s.qv - qsi) / (1.0 + tcpk * dqsdt);
     R pgsub = R(s.qv / qsi - 1.0) * R(s.qg);
     double pg;
     if (pgsub > R(0.0)) {
       if (s.tz > tice_d)
         pg = 0.0;
       else
-        pg = cap<R>(double(mmin(mmin(c.fac_v2g * pgsub, R(0.2) * dq),
-                                mmin(R(s.ql) + R(s.qr), R(tice_d - s.tz) / tcpk))),
+        pg = cap<R>(mmin(mmin(double(c.fac_v2g * pgsub), 0.2 * dq), mmin(s.ql + s.qr, (tice_d - s.tz) / tcpk)),
                     s.qv);
     } else {
-      pg = -cap<R>(-double(mmax(c.fac_g2v * pgsub, dq) * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.1))), s.qg);
+      pg = -cap<R>(-(mmax(double(c.fac_g2v * pgsub), dq) * double(mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.1)))),
+                   s.qg);
     }
     s.qg = s.qg + pg;
     s.qv = s.qv - pg;
-    q_sol = q_sol + R(pg);
+    q_sol = q_sol + pg;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + double(R(pg) * (lhl + lhi) / cvm);
+    s.tz = s.tz + pg * (lhl + lhi) / cvm;
   }
 
   // minimum evaporation of rain in dry air
-  lhl = c.lv00 + c.d0_vap * R(s.tz);
+  lhl = c.lv00_d + c.d0_vap_d * s.tz;
   lcpk = lhl / cvm;
   if (s.qr > QCMIN) {
     double dqsdt;
     const double qsw2 = wqs2<double>(s.tz, den_d, dqsdt);
-    double sink = mmin(s.qr, double(R(dim(rh_rain * qsw2, s.qv)) / (R(1.0) + lcpk * R(dqsdt))));
+    double sink = mmin(s.qr, dim(rh_rain * qsw2, s.qv) / (1.0 + lcpk * dqsdt));
     s.qv = s.qv + sink;
     s.qr = s.qr - sink;
-    q_liq = q_liq - R(sink);
+    q_liq = q_liq - sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz - double(R(sink) * lhl / cvm);
+    s.tz = s.tz - sink * lhl / cvm;
   }
-  lhl = c.lv00 + c.d0_vap * R(s.tz);
-  cvm = c.c_air + (R(s.qv) + q_liq + q_sol) * c.c_vap;
+  lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  cvm = c.c_air_d + (s.qv + q_liq + q_sol) * c.c_vap_d;
   lcpk = lhl / cvm;
 
   // cloud fraction
@@ -463,7 +473,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     const double q_liq2 = c.rad_rain ? s.ql + s.qr : s.ql;
     const double q_cond = q_liq2 + q_sol2;
     const double qpz2 = s.qv + q_cond;
-    const double tin2 = s.tz - double(lcpk * R(q_cond) + icpk * R(q_sol2));
+    const double tin2 = s.tz - (lcpk * q_cond + icpk * q_sol2);
     double qstar;
     if (tin2 <= c.t_wfr_d) {
       qstar = iqs1<double>(tin2, den_d);
@@ -497,35 +507,35 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 // icloud continues from.
 template <class R>
 __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice_d, double t_wfr_d, double ql_mlt,
-                                                   Cell& s, double den, R& q_liq, R& q_sol, R& cvm) {
-  const R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
-  q_liq = R(s.ql) + R(s.qr);
-  q_sol = R(s.qi) + R(s.qs) + R(s.qg);
+                                                   Cell& s, double den, double& q_liq, double& q_sol, double& cvm) {
+  const double lhi = LI00 + DC_ICE * s.tz;
+  q_liq = s.ql + s.qr;
+  q_sol = s.qi + s.qs + s.qg;
   cvm = SH_CVM(c, s, q_liq, q_sol);
-  const R icpk = lhi / cvm;
+  const double icpk = lhi / cvm;
   if ((s.tz > tice_d) && (s.qi > QCMIN)) {
-    const double melt = mmin(s.qi, double(c.fac_imlt * R(s.tz - tice_d) / icpk));
+    const double melt = mmin(s.qi, double(c.fac_imlt) * (s.tz - tice_d) / icpk);
     const double tmp = mmin(melt, dim(ql_mlt, s.ql));
     s.ql = s.ql + tmp;
     s.qr = s.qr + melt - tmp;
     s.qi = s.qi - melt;
-    q_liq = q_liq + R(melt);
-    q_sol = q_sol - R(melt);
+    q_liq = q_liq + melt;
+    q_sol = q_sol - melt;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz - double(R(melt) * lhi / cvm);
+    s.tz = s.tz - melt * lhi / cvm;
   } else if ((s.tz < t_wfr_d) && (s.ql > QCMIN)) {
-    const R dtmp = R(t_wfr_d - s.tz);
-    const R factor = mmin(R(1.0), dtmp / R(DT_FR));
-    const double sink = mmin(s.ql * double(factor), double(dtmp / icpk));
+    const double dtmp = t_wfr_d - s.tz;
+    const R factor = mmin(R(1.0), R(dtmp) / R(DT_FR));
+    const double sink = mmin(s.ql * double(factor), dtmp / icpk);
     const R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * R(tice_d - s.tz)) / R(den);
     const double tmp = mmin(sink, dim(double(qi_crt), s.qi));
     s.ql = s.ql - sink;
     s.qs = s.qs + sink - tmp;
     s.qi = s.qi + tmp;
-    q_liq = q_liq - R(sink);
-    q_sol = q_sol + R(sink);
+    q_liq = q_liq - sink;
+    q_sol = q_sol + sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + double(R(sink) * lhi / cvm);
+    s.tz = s.tz + sink * lhi / cvm;
   }
 }
 
@@ -533,11 +543,11 @@ __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice
 template <class R>
 __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, Cell& s, R h_var,
                                             double rh_adj, double rh_rain, double den_d, R denfac, R vtgz, R vtrz,
-                                            R vtsz, double p1d, R di, R q_liq, R q_sol, R cvm) {
+                                            R vtsz, double p1d, R di, double q_liq, double q_sol, double cvm) {
   const R dts = c.dts, rdts = c.rdts;
   const R den = R(den_d);
-  R lhi = R(LI00) + R(DC_ICE) * R(s.tz);
-  R icpk = lhi / cvm;
+  double lhi = LI00 + DC_ICE * s.tz;
+  double icpk = lhi / cvm;
   if (p1d >= P_MIN) {
     R pgacr = R(0.0), pgacw = R(0.0);
     R tc = R(s.tz - tice_d);
@@ -560,18 +570,18 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           pracs = R(0.0);
         }
         R psmlt = mmax(R(0.0), smlt(tc, dqs0, R(s.qs) * den, psacw, psacr, c.csmlt, den, denfac));
-        double sink = mmin(s.qs, double(mmin(dts * (psmlt + pracs), tc / icpk)));
+        double sink = mmin(s.qs, mmin(double(dts * (psmlt + pracs)), (s.tz - tice_d) / icpk));
         s.qs = s.qs - sink;
         double tmp = mmin(sink, double(dim(c.qs_mlt, R(s.ql))));
         s.ql = s.ql + tmp;
         s.qr = s.qr + sink - tmp;
-        q_liq = q_liq + R(sink);
-        q_sol = q_sol - R(sink);
+        q_liq = q_liq + sink;
+        q_sol = q_sol - sink;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz - double(R(sink) * lhi / cvm);
+        s.tz = s.tz - sink * lhi / cvm;
         tc = R(s.tz - tice_d);
       }
-      lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+      lhi = LI00 + DC_ICE * s.tz;
       icpk = lhi / cvm;
       if ((s.qg > QCMIN) && (tc > R(0.0))) {
         if (s.qr > QRMIN)
@@ -583,13 +593,13 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           pgacw = factor / (R(1.0) + dts * factor) * R(s.ql);
         }
         R pgmlt = dts * gmlt(tc, dqs0, qden, pgacw, pgacr, c.cgmlt, den);
-        double pm = mmin(double(mmax(R(0.0), pgmlt)), mmin(s.qg, double(tc / icpk)));
+        double pm = mmin(double(mmax(R(0.0), pgmlt)), mmin(s.qg, (s.tz - tice_d) / icpk));
         s.qg = s.qg - pm;
         s.qr = s.qr + pm;
-        q_liq = q_liq + R(pm);
-        q_sol = q_sol - R(pm);
+        q_liq = q_liq + pm;
+        q_sol = q_sol - pm;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz - double(R(pm) * lhi / cvm);
+        s.tz = s.tz - pm * lhi / cvm;
       }
     } else {
       if (s.qi > 3.0e-7) {
@@ -642,7 +652,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         // the limiter that can drain all rain is applied in FP64, so a fully
         // frozen layer keeps no FP32 residue
         const double sink = double(psacr) + double(pgfr);
-        const double factor = mmin(sink, mmin(s.qr, double(-tc / icpk))) / mmax(sink, QRMIN);
+        const double factor = mmin(sink, mmin(s.qr, -(s.tz - tice_d) / icpk)) / mmax(sink, QRMIN);
         double ps = factor * double(psacr), pg = factor * double(pgfr);
         if constexpr (is_f32<R>::value) {
           double tot = ps + pg;
@@ -656,12 +666,12 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         s.qr = s.qr - sk;
         s.qs = s.qs + ps;
         s.qg = s.qg + pg;
-        q_liq = q_liq - R(sk);
-        q_sol = q_sol + R(sk);
+        q_liq = q_liq - sk;
+        q_sol = q_sol + sk;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz + double(R(sk) * lhi / cvm);
+        s.tz = s.tz + sk * lhi / cvm;
       }
-      lhi = R(LI00) + R(DC_ICE) * R(s.tz);
+      lhi = LI00 + DC_ICE * s.tz;
       icpk = lhi / cvm;
       if (s.qs > 1.0e-7) {
         R sink;
@@ -692,16 +702,16 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         else
           pgacr = R(0.0);
         const double sink = double(pgacr) + double(pgacw);
-        const double factor = mmin(sink, double(R(dim(tice_d, s.tz)) / icpk)) / mmax(sink, QRMIN);
+        const double factor = mmin(sink, dim(tice_d, s.tz) / icpk) / mmax(sink, QRMIN);
         const double pr = cap<R>(factor * double(pgacr), s.qr), pl = cap<R>(factor * double(pgacw), s.ql);
         double sk = pr + pl;
         s.qg = s.qg + sk;
         s.qr = s.qr - pr;
         s.ql = s.ql - pl;
-        q_liq = q_liq - R(sk);
-        q_sol = q_sol + R(sk);
+        q_liq = q_liq - sk;
+        q_sol = q_sol + sk;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz + double(R(sk) * lhi / cvm);
+        s.tz = s.tz + sk * lhi / cvm;
       }
     }
   }

@@ -11,7 +11,7 @@ hipError_t launch_forward(const shield_mp_shape& sh, const shield_mp_params& p, 
 struct LayoutChoice {
   int S, P;
 };
-LayoutChoice choose_layout(int64_t nz);
+LayoutChoice choose_layout(int64_t nz, int mode, int ntimes);
 }  // namespace shield
 
 using namespace shield;
@@ -148,8 +148,9 @@ int shield_mp_update_timestep(shield_mp_params* p, double timestep) {
   return SHIELD_OK;
 }
 
-int shield_mp_layout(int64_t nk, int* cols_per_wave, int* slots) {
-  const LayoutChoice lc = choose_layout(nk);
+int shield_mp_layout(int64_t nk, int mode, int ntimes, int* cols_per_wave, int* slots) {
+  if (mode != SHIELD_MP_FP64 && mode != SHIELD_MP_MIXED) return SHIELD_ERR_MODE;
+  const LayoutChoice lc = choose_layout(nk, mode, ntimes);
   if (cols_per_wave) *cols_per_wave = lc.P;
   if (slots) *slots = lc.S;
   return lc.S == 0 ? SHIELD_ERR_SHAPE : SHIELD_OK;

@@ -92,8 +92,11 @@ def test_strerror_and_layout():
     L = _lib.lib()
     for code in range(-8, 1):
         assert L.shield_strerror(code)
+    # without a device the layout with the most busy lanes wins
     assert _lib.layout(79) == (4, 5)       # 4 columns in 5 slots: 316 of 320 lanes busy
     assert _lib.layout(127) == (1, 2)
     assert _lib.layout(8) == (8, 1)
     with pytest.raises(_lib.ShieldError):
         _lib.layout(1000)
+    with pytest.raises(_lib.ShieldError):
+        _lib.layout(79, 7)                 # unknown precision mode
