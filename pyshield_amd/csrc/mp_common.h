@@ -121,14 +121,17 @@ __device__ __forceinline__ T dim(T x, T y) {
 constexpr int pmax_for(int S) { return S >= 3 ? 16 : 64; }
 
 // --------------------------------------------------- coefficient block
+// Offsets of the coefficient arrays in Coef::tab: acco[3][4] (row-major),
+// cssub[5], crevp[5], csmlt[5], cgmlt[5], cgfr[2].
+enum { T_ACCO = 0, T_CSSUB = 12, T_CREVP = 17, T_CSMLT = 22, T_CGMLT = 27, T_CGFR = 32, T_N = 34 };
+
 // The scalars the kernels read, in the rate type R.  Filled on the host from
 // shield_mp_params (which is built by shield_mp_setup).
 template <class R>
 struct Coef {
   R c_air, c_vap, d0_vap, lv00, so3, log_10, tice, tice0, t_wfr, t_sub;
   R fac_rc, cracs, csacr, cgacr, cgacs, csacw, csaci, cgacw, cgaci, cracw, ces0;
-  R acco[3][4];
-  R cssub[5], crevp[5], cgfr[2], csmlt[5], cgmlt[5];
+  R tab[T_N];
   R dts, rdts, dt_rain, fac_i2s, fac_g2v, fac_v2g, fac_imlt, fac_l2v;
   R ql_mlt, qs_mlt, qi_gen, qi_lim, qi0_crt, qs0_crt, qc_crt;
   R vs_fac, vg_fac, vi_fac, vr_fac, vs_max, vg_max, vi_max, vr_max;
@@ -139,6 +142,22 @@ struct Coef {
   double t_sub_d, t_wfr_d, qc_crt_d;
   double c_air_d, c_vap_d, d0_vap_d, lv00_d;
 };
+
+// The mixed-precision kernel keeps its copy of Coef::tab in LDS (filled at
+// its start): with the whole block in scalar registers the compiler spills
+// them to VGPR lanes and reloads them with v_readlane inside the process
+// code.  Each translation unit gets its own (per-kernel) copy.
+static __shared__ float coef_tab_lds[T_N];
+
+// Coefficient-array entry i (a constant after inlining).  In mixed precision
+// it is read from the LDS copy, in FP64 from the kernel argument.
+template <class R>
+__device__ __forceinline__ R ck(const Coef<R>& c, int i) {
+  if constexpr (sizeof(R) == 4)
+    return coef_tab_lds[i];
+  else
+    return c.tab[i];
+}
 
 template <class R>
 inline Coef<R> make_coef(const shield_mp_params& p) {
@@ -151,12 +170,12 @@ inline Coef<R> make_coef(const shield_mp_params& p) {
   c.cgacs = R(p.cgacs); c.csacw = R(p.csacw); c.csaci = R(p.csaci); c.cgacw = R(p.cgacw);
   c.cgaci = R(p.cgaci); c.cracw = R(p.cracw); c.ces0 = R(p.ces0);
   for (int i = 0; i < 3; ++i)
-    for (int k = 0; k < 4; ++k) c.acco[i][k] = R(p.acco[i][k]);
+    for (int k = 0; k < 4; ++k) c.tab[T_ACCO + 4 * i + k] = R(p.acco[i][k]);
   for (int i = 0; i < 5; ++i) {
-    c.cssub[i] = R(p.cssub[i]); c.crevp[i] = R(p.crevp[i]);
-    c.csmlt[i] = R(p.csmlt[i]); c.cgmlt[i] = R(p.cgmlt[i]);
+    c.tab[T_CSSUB + i] = R(p.cssub[i]); c.tab[T_CREVP + i] = R(p.crevp[i]);
+    c.tab[T_CSMLT + i] = R(p.csmlt[i]); c.tab[T_CGMLT + i] = R(p.cgmlt[i]);
   }
-  c.cgfr[0] = R(p.cgfr[0]); c.cgfr[1] = R(p.cgfr[1]);
+  c.tab[T_CGFR] = R(p.cgfr[0]); c.tab[T_CGFR + 1] = R(p.cgfr[1]);
   c.dts = R(p.dts); c.rdts = R(p.rdts); c.dt_rain = R(p.dt_rain);
   c.fac_i2s = R(p.fac_i2s); c.fac_g2v = R(p.fac_g2v); c.fac_v2g = R(p.fac_v2g);
   c.fac_imlt = R(p.fac_imlt); c.fac_l2v = R(p.fac_l2v);

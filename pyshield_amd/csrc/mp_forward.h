@@ -477,11 +477,20 @@ __device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>&
 }
 
 // ------------------------------------------------------------------ kernel
+// Waves per SIMD the kernel is compiled for.  The single-sub-step
+// mixed-precision kernel with at most 3 slots fits three waves (168 VGPRs)
+// with a few bytes of spill; every other instance is left to the register
+// allocator.
+template <int S, class R, bool MULTI>
+constexpr int min_waves() {
+  return (S <= 3 && sizeof(R) == 4 && !MULTI) ? 3 : 1;
+}
+
 // MULTI: more than one sub-step.  With a single sub-step the body is not
 // wrapped in a loop: loop-invariant code motion would otherwise keep every
 // hoisted constant of the process code in a register across the whole body.
 template <int S, class R, bool MULTI>
-__global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<R> c,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S, R, MULTI>()))) void mp_forward_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<R> c,
                                                         shield_mp_fields f, int cols_per_wave) {
   __shared__ Shared<S> sh;
   const int lane = lane_id();
@@ -530,6 +539,9 @@ __global__ __launch_bounds__(64) void mp_forward_kernel(shield_mp_shape sh3, shi
     sh.colv[CV_SNOW][lane] = 0.0;
     sh.colv[CV_ICE][lane] = 0.0;
     sh.colv[CV_GRAUPEL][lane] = 0.0;
+  }
+  if constexpr (sizeof(R) == 4) {
+    if (lane < T_N) coef_tab_lds[lane] = c.tab[lane];
   }
   __syncthreads();
 

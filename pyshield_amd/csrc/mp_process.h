@@ -111,14 +111,18 @@ __device__ __forceinline__ R acr3d(R v1, R v2, R q1, R q2, R c, R cac_ik, R cac_
   return c * mabs(v1 - v2) * q1 * s2 * (cac_ik * t1 + cac_i1k * msqrt(t1) * s2 + cac_i2k * s1);
 }
 template <class R>
-__device__ __forceinline__ R smlt(R tc, R dqs, R qsrho, R psacw, R psacr, const R* k, R rho, R rhofac) {
-  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qsrho) + k[3] * pw<R>(qsrho, 0.65625) * msqrt(rhofac)) +
-         k[4] * tc * (psacw + psacr);
+__device__ __forceinline__ R smlt(const Coef<R>& c, R tc, R dqs, R qsrho, R psacw, R psacr, R rho, R rhofac) {
+  const int k = T_CSMLT;
+  return (ck(c, k) * tc / rho - ck(c, k + 1) * dqs) *
+             (ck(c, k + 2) * msqrt(qsrho) + ck(c, k + 3) * pw<R>(qsrho, 0.65625) * msqrt(rhofac)) +
+         ck(c, k + 4) * tc * (psacw + psacr);
 }
 template <class R>
-__device__ __forceinline__ R gmlt(R tc, R dqs, R qgrho, R pgacw, R pgacr, const R* k, R rho) {
-  return (k[0] * tc / rho - k[1] * dqs) * (k[2] * msqrt(qgrho) + k[3] * pw<R>(qgrho, 0.6875) / pw<R>(rho, 0.25)) +
-         k[4] * tc * (pgacw + pgacr);
+__device__ __forceinline__ R gmlt(const Coef<R>& c, R tc, R dqs, R qgrho, R pgacw, R pgacr, R rho) {
+  const int k = T_CGMLT;
+  return (ck(c, k) * tc / rho - ck(c, k + 1) * dqs) *
+             (ck(c, k + 2) * msqrt(qgrho) + ck(c, k + 3) * pw<R>(qgrho, 0.6875) / pw<R>(rho, 0.25)) +
+         ck(c, k + 4) * tc * (pgacw + pgacr);
 }
 
 // ------------------------------------------------ f:183-265
@@ -149,8 +153,8 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
         dq = R(0.25) * R((q_minus - qsat) * (q_minus - qsat)) / dqh;
       R qden = R(s.qr) * den;
       R t2 = R(tin) * R(tin);
-      R evap = c.crevp[0] * t2 * dq * (c.crevp[1] * msqrt(qden) + c.crevp[2] * pw_el<R>(qden, 0.725)) /
-               (c.crevp[3] * t2 + c.crevp[4] * R(qsat) * den);
+      R evap = ck(c, T_CREVP + 0) * t2 * dq * (ck(c, T_CREVP + 1) * msqrt(qden) + ck(c, T_CREVP + 2) * pw_el<R>(qden, 0.725)) /
+               (ck(c, T_CREVP + 3) * t2 + ck(c, T_CREVP + 4) * R(qsat) * den);
       double ev = mmin(s.qr, mmin(double(dt * evap), dqv / (1.0 + lcpk * dqsdt)));
       s.qr = s.qr - ev;
       s.qv = s.qv + ev;
@@ -402,8 +406,8 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     const double dsat_d = qsi - s.qv;
     const R dsat = R(dsat_d);
     const double dq = dsat_d / (1.0 + tcpk * dqsdt);
-    R pssub = c.cssub[0] * tsq * (c.cssub[1] * msqrt(qden) + c.cssub[2] * tmp * msqrt(denfac)) /
-              (c.cssub[3] * tsq + c.cssub[4] * R(qsi) * den);
+    R pssub = ck(c, T_CSSUB + 0) * tsq * (ck(c, T_CSSUB + 1) * msqrt(qden) + ck(c, T_CSSUB + 2) * tmp * msqrt(denfac)) /
+              (ck(c, T_CSSUB + 3) * tsq + ck(c, T_CSSUB + 4) * R(qsi) * den);
     pssub = dsat * dts * pssub;
     double ps;
     if (pssub > R(0.0)) {
@@ -562,14 +566,14 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           psacw = R(0.0);
         }
         if (s.qr > QRMIN) {
-          psacr = mmin(acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, c.acco[0][1], c.acco[1][1], c.acco[2][1], den),
+          psacr = mmin(acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, ck(c, T_ACCO + 1), ck(c, T_ACCO + 5), ck(c, T_ACCO + 9), den),
                        R(s.qr) * rdts);
-          pracs = acr3d(vtrz, vtsz, R(s.qs), R(s.qr), c.cracs, c.acco[0][0], c.acco[1][0], c.acco[2][0], den);
+          pracs = acr3d(vtrz, vtsz, R(s.qs), R(s.qr), c.cracs, ck(c, T_ACCO + 0), ck(c, T_ACCO + 4), ck(c, T_ACCO + 8), den);
         } else {
           psacr = R(0.0);
           pracs = R(0.0);
         }
-        R psmlt = mmax(R(0.0), smlt(tc, dqs0, R(s.qs) * den, psacw, psacr, c.csmlt, den, denfac));
+        R psmlt = mmax(R(0.0), smlt(c, tc, dqs0, R(s.qs) * den, psacw, psacr, den, denfac));
         double sink = mmin(s.qs, mmin(double(dts * (psmlt + pracs)), (s.tz - tice_d) / icpk));
         s.qs = s.qs - sink;
         double tmp = mmin(sink, double(dim(c.qs_mlt, R(s.ql))));
@@ -585,14 +589,14 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
       icpk = lhi / cvm;
       if ((s.qg > QCMIN) && (tc > R(0.0))) {
         if (s.qr > QRMIN)
-          pgacr = mmin(acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, c.acco[0][2], c.acco[1][2], c.acco[2][2], den),
+          pgacr = mmin(acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, ck(c, T_ACCO + 2), ck(c, T_ACCO + 6), ck(c, T_ACCO + 10), den),
                        rdts * R(s.qr));
         R qden = R(s.qg) * den;
         if (s.ql > QRMIN) {
           R factor = c.cgacw * qden / msqrt(den * msqrt(msqrt(qden)));
           pgacw = factor / (R(1.0) + dts * factor) * R(s.ql);
         }
-        R pgmlt = dts * gmlt(tc, dqs0, qden, pgacw, pgacr, c.cgmlt, den);
+        R pgmlt = dts * gmlt(c, tc, dqs0, qden, pgacw, pgacr, den);
         double pm = mmin(double(mmax(R(0.0), pgmlt)), mmin(s.qg, (s.tz - tice_d) / icpk));
         s.qg = s.qg - pm;
         s.qr = s.qr + pm;
@@ -645,10 +649,10 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
       if ((s.qr > 1e-7) && (s.tz - tice_d < 0.0)) {
         R psacr;
         if (s.qs > 1.0e-7)
-          psacr = dts * acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, c.acco[0][1], c.acco[1][1], c.acco[2][1], den);
+          psacr = dts * acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, ck(c, T_ACCO + 1), ck(c, T_ACCO + 5), ck(c, T_ACCO + 9), den);
         else
           psacr = R(0.0);
-        R pgfr = dts * c.cgfr[0] / den * (mexp(-c.cgfr[1] * tc) - R(1.0)) * pw_el<R>(R(s.qr) * den, 1.75);
+        R pgfr = dts * ck(c, T_CGFR + 0) / den * (mexp(-ck(c, T_CGFR + 1) * tc) - R(1.0)) * pw_el<R>(R(s.qr) * den, 1.75);
         // the limiter that can drain all rain is applied in FP64, so a fully
         // frozen layer keeps no FP32 residue
         const double sink = double(psacr) + double(pgfr);
@@ -676,7 +680,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
       if (s.qs > 1.0e-7) {
         R sink;
         if (s.qg > QRMIN)
-          sink = dts * acr3d(vtgz, vtsz, R(s.qs), R(s.qg), c.cgacs, c.acco[0][3], c.acco[1][3], c.acco[2][3], den);
+          sink = dts * acr3d(vtgz, vtsz, R(s.qs), R(s.qg), c.cgacs, ck(c, T_ACCO + 3), ck(c, T_ACCO + 7), ck(c, T_ACCO + 11), den);
         else
           sink = R(0.0);
         R qsm = c.qs0_crt / den;
@@ -697,7 +701,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           pgacw = R(0.0);
         }
         if (s.qr > 1.0e-6)
-          pgacr = mmin(dts * acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, c.acco[0][2], c.acco[1][2], c.acco[2][2], den),
+          pgacr = mmin(dts * acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, ck(c, T_ACCO + 2), ck(c, T_ACCO + 6), ck(c, T_ACCO + 10), den),
                        R(s.qr));
         else
           pgacr = R(0.0);
