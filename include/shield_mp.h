@@ -136,6 +136,42 @@ int shield_mp_update_timestep(shield_mp_params* p, double timestep);
 int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p,
                       const shield_mp_fields* f, int mode, void* stream);
 
+/* The state-update part of shield_mp_forward_update: the sequence
+ * Physics.__call__ runs after the microphysics (physics.py:331-366) and
+ * UpdateAtmosphereState.__call__ runs next (update_atmos_state.py:301-349,
+ * without ApplyPhysicsToDycore).  3D fields use the microphysics shape's
+ * si/sj; prsi has nk+1 levels with strides si_i/sj_i and k stride 1.        */
+typedef struct shield_update_fields {
+  /* physics_updated_* = x + x_dt * dt (physics.py:146-194), written, in the
+   * order qvapor, qliquid, qrain, qice, qsnow, qgraupel, qcld, pt, ua, va  */
+  double* updated[10];
+  /* dycore tendencies, accumulated: += (x_updated - x) * rdt for ua, va, pt
+   * (update_atmos_state.py:76-78)                                          */
+  double *u_dt, *v_dt, *pt_dt;
+  /* dycore (FV3) tracers qvapor, qliquid, qrain, qsnow, qice, qgraupel and
+   * delp, rewritten in place by the GFS -> FV3 conversion (:79-92)         */
+  double* tracers[6];
+  double* delp;
+  const double* prsi;
+  int64_t si_i, sj_i;
+  double dt;     /* forward Euler step (the physics timestep)               */
+  double rdt;    /* 1 / dt_atmos of the dycore tendencies                   */
+  double q_min;  /* fill_gfs_delp threshold on the updated vapour (1e-9)    */
+} shield_update_fields;
+
+/* shield_mp_forward, then in the same kernel
+ * update_physics_state_with_tendencies (physics.py:146-194), fill_gfs_delp
+ * on physics_updated_specific_humidity with the dycore delp
+ * (update_atmos_state.py:21-37, :318-320) and
+ * prepare_tendencies_and_update_tracers (:40-92, :321-345): the state the
+ * dycore receives, without writing and re-reading the tendencies.  The
+ * reference also fills the halo columns of the updated vapour; only the
+ * columns in `shape` are filled here, the only ones the conversion reads.  No
+ * single reference function: it replaces that sequence of four calls.     */
+int shield_mp_forward_update(const shield_mp_shape* shape, const shield_mp_params* p,
+                             const shield_mp_fields* f, const shield_update_fields* u, int mode,
+                             void* stream);
+
 /* ------------------------------------------------- state-update path */
 /* For these entry points shape->si2 and shape->sj2 are the i/j strides of
  * the nk+1-level interface field (phii or prsi), which has k stride 1 like

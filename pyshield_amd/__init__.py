@@ -7,10 +7,10 @@ kernels for gfx950 behind the C ABI in include/shield_mp.h.
 
 from ._config import PHYSICS_PACKAGES, PhysicsConfig
 from .microphysics import (GridData, Microphysics, MicrophysicsState, QuantityFactory,
-                           StencilFactory, forward)
+                           StencilFactory, forward, forward_update)
 
 __all__ = [
     "PHYSICS_PACKAGES", "PhysicsConfig", "Microphysics", "MicrophysicsState",
-    "StencilFactory", "QuantityFactory", "GridData", "forward",
+    "StencilFactory", "QuantityFactory", "GridData", "forward", "forward_update",
 ]
 __version__ = "0.2.0"

@@ -19,7 +19,7 @@ MODES = {"fp64": SHIELD_MP_FP64, "mixed": SHIELD_MP_MIXED}
 
 # every symbol the header declares
 EXPORTS = (
-    "shield_mp_setup", "shield_mp_update_timestep", "shield_mp_forward",
+    "shield_mp_setup", "shield_mp_update_timestep", "shield_mp_forward", "shield_mp_forward_update",
     "shield_prepare_microphysics", "shield_update_physics_state_with_tendencies",
     "shield_fill_gfs_delp", "shield_prepare_tendencies_and_update_tracers",
     "shield_strerror", "shield_device_count", "shield_mp_layout",
@@ -86,6 +86,18 @@ class ShieldMpFields(ctypes.Structure):
     _fields_ = [(n, _VP) for n in FIELD_NAMES]
 
 
+# physics_updated_* order of shield_update_fields.updated and the FV3 tracer
+# order of shield_update_fields.tracers (include/shield_mp.h)
+UPDATED_NAMES = ("qvapor", "qliquid", "qrain", "qice", "qsnow", "qgraupel", "qcld", "pt", "ua", "va")
+TRACER_NAMES = ("qvapor", "qliquid", "qrain", "qsnow", "qice", "qgraupel")
+
+
+class ShieldUpdateFields(ctypes.Structure):
+    _fields_ = [("updated", _VP * 10), ("u_dt", _VP), ("v_dt", _VP), ("pt_dt", _VP), ("tracers", _VP * 6),
+                ("delp", _VP), ("prsi", _VP), ("si_i", _I64), ("sj_i", _I64), ("dt", _D), ("rdt", _D),
+                ("q_min", _D)]
+
+
 _lib = None
 
 
@@ -105,6 +117,9 @@ def lib():
         L.shield_mp_update_timestep.argtypes = [ctypes.POINTER(ShieldMpParams), _D]
         L.shield_mp_forward.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldMpParams),
                                         ctypes.POINTER(ShieldMpFields), ctypes.c_int, _VP]
+        L.shield_mp_forward_update.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldMpParams),
+                                               ctypes.POINTER(ShieldMpFields), ctypes.POINTER(ShieldUpdateFields),
+                                               ctypes.c_int, _VP]
         L.shield_prepare_microphysics.argtypes = [ctypes.POINTER(ShieldMpShape), _VP, _VP, _VP, _VP, _VP,
                                                   _VP, _VP, _VP * 10, _VP]
         L.shield_update_physics_state_with_tendencies.argtypes = [ctypes.POINTER(ShieldMpShape), _VP * 10,

@@ -466,6 +466,87 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
   }
 }
 
+// The state update that follows the microphysics for this wavefront's
+// columns: update_physics_state_with_tendencies (physics.py:146-194),
+// fill_gfs_delp on the updated vapour with the dycore delp
+// (update_atmos_state.py:21-37) and prepare_tendencies_and_update_tracers
+// (:40-92).  The tendencies were just written by this wavefront and are read
+// back from the cache, so every expression is the one the separate kernels
+// evaluate.
+template <int S>
+__device__ __forceinline__ void state_update(const Lay<S>& L, const Glob<S>& G, Shared<S>& sh, const shield_mp_shape& sh3,
+                                             int64_t c0, const shield_mp_fields& f, const shield_update_fields& u) {
+  const int lane = lane_id();
+  const int nz = L.nz;
+  const double* in[10] = {f.qv, f.ql, f.qr, f.qi, f.qs, f.qg, f.qa, f.pt, f.ua, f.va};
+  const double* td[10] = {f.qv_dt, f.ql_dt, f.qr_dt, f.qi_dt, f.qs_dt, f.qg_dt, f.qa_dt, f.pt_dt, f.udt, f.vdt};
+  // forward Euler; the updated vapour and the dycore delp go to LDS
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double qv1 = 0.0, dp = 1.0;
+    if (L.v[s]) {
+      const int32_t o = G.off[s];
+#pragma unroll
+      for (int n = 0; n < 10; ++n) {
+        const double x = in[n][o] + td[n][o] * u.dt;
+        u.updated[n][o] = x;
+        if (n == 0) qv1 = x;
+      }
+      dp = u.delp[o];
+    }
+    sh.buf[64 * s + lane] = qv1;
+    sh.cell[CL_TMP][64 * s + lane] = dp;
+  }
+  __syncthreads();
+  // fill_gfs_delp: a non-linear vertical recurrence, one lane per column
+  if (lane < L.ncols) {
+    double* q = sh.buf + lane * nz;
+    const double* dp = sh.cell[CL_TMP] + lane * nz;
+    const double q_min = u.q_min;
+    for (int k = nz - 2; k >= 0; --k)
+      if (q[k + 1] < q_min) q[k] = q[k] + (q[k + 1] - q_min) * dp[k + 1] / dp[k];
+    for (int k = 1; k < nz; ++k)
+      if (q[k] < q_min) q[k] = q_min;
+    for (int k = 1; k < nz; ++k)
+      if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * (dp[k - 1]) / (dp[k]);
+    for (int k = 0; k < nz; ++k)
+      if (q[k] < 0.0) q[k] = 0.0;
+  }
+  __syncthreads();
+  // prepare_tendencies_and_update_tracers
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s]) continue;
+    const int32_t o = G.off[s];
+    const int64_t col = c0 + L.cl[s];
+    const int64_t i = col / sh3.nj, j = col - i * sh3.nj;
+    const int64_t oi = i * u.si_i + j * u.sj_i + L.k[s];
+    const double qv1 = sh.buf[64 * s + lane];
+    u.updated[0][o] = qv1;
+    u.u_dt[o] += (u.updated[8][o] - f.ua[o]) * u.rdt;
+    u.v_dt[o] += (u.updated[9][o] - f.va[o]) * u.rdt;
+    u.pt_dt[o] += (u.updated[7][o] - f.pt[o]) * u.rdt;
+    const double dp = u.prsi[oi + 1] - u.prsi[oi];
+    const double qwat_qv = dp * qv1;
+    const double qwat_ql = dp * u.updated[1][o];
+    const double qwat_qr = dp * u.updated[2][o];
+    const double qwat_qs = dp * u.updated[4][o];
+    const double qwat_qi = dp * u.updated[3][o];
+    const double qwat_qg = dp * u.updated[5][o];
+    const double qt = qwat_qv + qwat_ql + qwat_qr + qwat_qs + qwat_qi + qwat_qg;
+    double* const* tr = u.tracers;  // qv, ql, qr, qs, qi, qg
+    const double q_sum = tr[0][o] + tr[1][o] + tr[2][o] + tr[3][o] + tr[4][o] + tr[5][o];
+    const double q0 = u.delp[o] * (1.0 - q_sum) + qt;
+    u.delp[o] = q0;
+    tr[0][o] = qwat_qv / q0;
+    tr[1][o] = qwat_ql / q0;
+    tr[2][o] = qwat_qr / q0;
+    tr[3][o] = qwat_qs / q0;
+    tr[4][o] = qwat_qi / q0;
+    tr[5][o] = qwat_qg / q0;
+  }
+}
+
 // One sub-step (microphysics.py:2045-2220).
 template <int S, class R>
 __device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
@@ -481,17 +562,20 @@ __device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>&
 // mixed-precision kernel with at most 3 slots fits three waves (168 VGPRs)
 // with a few bytes of spill; every other instance is left to the register
 // allocator.
-template <int S, class R, bool MULTI>
+template <int S, class R, bool MULTI, bool FUSED>
 constexpr int min_waves() {
-  return (S <= 3 && sizeof(R) == 4 && !MULTI) ? 3 : 1;
+  return (S <= 3 && sizeof(R) == 4 && !MULTI && !FUSED) ? 3 : 1;
 }
 
 // MULTI: more than one sub-step.  With a single sub-step the body is not
 // wrapped in a loop: loop-invariant code motion would otherwise keep every
 // hoisted constant of the process code in a register across the whole body.
-template <int S, class R, bool MULTI>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S, R, MULTI>()))) void mp_forward_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<R> c,
-                                                        shield_mp_fields f, int cols_per_wave) {
+// FUSED: the state update of shield_mp_forward_update follows (a separate
+// instance, so the plain kernel keeps its registers).
+template <int S, class R, bool MULTI, bool FUSED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S, R, MULTI, FUSED>()))) void mp_forward_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<R> c,
+                                                        shield_mp_fields f, shield_update_fields u,
+                                                        int cols_per_wave) {
   __shared__ Shared<S> sh;
   const int lane = lane_id();
   const int nz = (int)sh3.nk;
@@ -764,31 +848,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
     if (f.ice) f.ice[o2] = sh.colv[CV_ICE][lane] * convt;
     if (f.graupel) f.graupel[o2] = sh.colv[CV_GRAUPEL][lane] * convt;
   }
+  if constexpr (FUSED) state_update(L, G, sh, sh3, c0, f, u);
 }
 
 // Launch the kernel for one slot count: a grid of ceil(ncol / P) one-wave
-// workgroups.
+// workgroups.  u.delp != NULL selects the fused state update.
 template <int S, class R>
-hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f, int P,
-                        hipStream_t stream) {
+hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
+                        const shield_update_fields& u, int P, hipStream_t stream) {
   const int64_t ncol = sh.ni * sh.nj;
-  const int64_t grid = (ncol + P - 1) / P;
+  const dim3 grid((unsigned)((ncol + P - 1) / P)), block(64);
   const Coef<R> c = make_coef<R>(p);
-  if (p.ntimes > 1)
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, true>), dim3((unsigned)grid), dim3(64), 0, stream, sh, p, c, f, P);
+  const bool multi = p.ntimes > 1, fused = u.delp != nullptr;
+  if (multi && fused)
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, true>), grid, block, 0, stream, sh, p, c, f, u, P);
+  else if (multi)
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, false>), grid, block, 0, stream, sh, p, c, f, u, P);
+  else if (fused)
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, true>), grid, block, 0, stream, sh, p, c, f, u, P);
   else
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, false>), dim3((unsigned)grid), dim3(64), 0, stream, sh, p, c, f, P);
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, false>), grid, block, 0, stream, sh, p, c, f, u, P);
   return hipGetLastError();
 }
 
 // Resident wavefronts per SIMD of one kernel instance on the current device
 // (0 when the query fails, e.g. without a device).
 template <int S, class R>
-int waves_per_simd(bool multi) {
+int waves_per_simd(bool multi, bool fused) {
   int blocks = 0;
-  const hipError_t e =
-      multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, true>, 64, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, false>, 64, 0);
+  hipError_t e;
+  if (multi && fused)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, true, true>, 64, 0);
+  else if (multi)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, true, false>, 64, 0);
+  else if (fused)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, false, true>, 64, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, false, false>, 64, 0);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return 0;

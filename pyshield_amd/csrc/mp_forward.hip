@@ -10,11 +10,11 @@
 namespace shield {
 
 template <int S, class R>
-hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f, int P,
-                        hipStream_t stream);
+hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
+                        const shield_update_fields& u, int P, hipStream_t stream);
 
 template <int S, class R>
-int waves_per_simd(bool multi);
+int waves_per_simd(bool multi, bool fused);
 
 struct LayoutChoice {
   int S, P;
@@ -32,32 +32,32 @@ static double occupancy_weight(int waves) {
   return 1.0;
 }
 
-static int waves_of(int S, int mode, bool multi) {
+static int waves_of(int S, int mode, bool multi, bool fused) {
   // queried once per instance and cached as value + 1 (0: not yet queried);
   // concurrent first calls compute the same value
-  static std::atomic<int> cache[6][2][2];
+  static std::atomic<int> cache[6][2][2][2];
   const int m = mode == SHIELD_MP_FP64 ? 1 : 0;
-  int v = cache[S][m][multi].load(std::memory_order_relaxed);
+  int v = cache[S][m][multi][fused].load(std::memory_order_relaxed);
   if (v == 0) {
     int w = 0;
     switch (S) {
-      case 1: w = m ? waves_per_simd<1, double>(multi) : waves_per_simd<1, float>(multi); break;
-      case 2: w = m ? waves_per_simd<2, double>(multi) : waves_per_simd<2, float>(multi); break;
-      case 3: w = m ? waves_per_simd<3, double>(multi) : waves_per_simd<3, float>(multi); break;
-      case 4: w = m ? waves_per_simd<4, double>(multi) : waves_per_simd<4, float>(multi); break;
-      case 5: w = m ? waves_per_simd<5, double>(multi) : waves_per_simd<5, float>(multi); break;
+      case 1: w = m ? waves_per_simd<1, double>(multi, fused) : waves_per_simd<1, float>(multi, fused); break;
+      case 2: w = m ? waves_per_simd<2, double>(multi, fused) : waves_per_simd<2, float>(multi, fused); break;
+      case 3: w = m ? waves_per_simd<3, double>(multi, fused) : waves_per_simd<3, float>(multi, fused); break;
+      case 4: w = m ? waves_per_simd<4, double>(multi, fused) : waves_per_simd<4, float>(multi, fused); break;
+      case 5: w = m ? waves_per_simd<5, double>(multi, fused) : waves_per_simd<5, float>(multi, fused); break;
     }
     v = w + 1;
-    cache[S][m][multi].store(v, std::memory_order_relaxed);
+    cache[S][m][multi][fused].store(v, std::memory_order_relaxed);
   }
   return v - 1;
 }
 
 // Columns per wavefront P and slots S for nz levels: P = floor(64 S / nz)
 // whole columns in 64 S lanes, scored by the busy-lane fraction times the
-// occupancy weight of that instance.  SHIELD_MP_SLOTS=<S> forces a slot
-// count (tuning).
-LayoutChoice choose_layout(int64_t nz, int mode, int ntimes) {
+// occupancy weight of that instance (plain or fused).  SHIELD_MP_SLOTS=<S>
+// forces a slot count (tuning).
+LayoutChoice choose_layout(int64_t nz, int mode, int ntimes, bool fused) {
   LayoutChoice best{0, 0};
   if (nz < 1) return best;
   static const int forced = [] {
@@ -72,7 +72,7 @@ LayoutChoice choose_layout(int64_t nz, int mode, int ntimes) {
     if (P < 1) continue;
     const int64_t Pc = P > pmax_for(S) ? pmax_for(S) : P;
     const double util = double(Pc * nz) / double(64 * S);
-    const double score = util * occupancy_weight(waves_of(S, mode, ntimes > 1));
+    const double score = util * occupancy_weight(waves_of(S, mode, ntimes > 1, fused));
     if (score > best_score + 1e-9) {
       best_score = score;
       best = {S, (int)Pc};
@@ -83,23 +83,24 @@ LayoutChoice choose_layout(int64_t nz, int mode, int ntimes) {
 
 template <class R>
 static hipError_t launch_r(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
-                           LayoutChoice lc, hipStream_t stream) {
+                           const shield_update_fields& u, LayoutChoice lc, hipStream_t stream) {
   switch (lc.S) {
-    case 1: return launch_slots<1, R>(sh, p, f, lc.P, stream);
-    case 2: return launch_slots<2, R>(sh, p, f, lc.P, stream);
-    case 3: return launch_slots<3, R>(sh, p, f, lc.P, stream);
-    case 4: return launch_slots<4, R>(sh, p, f, lc.P, stream);
-    case 5: return launch_slots<5, R>(sh, p, f, lc.P, stream);
+    case 1: return launch_slots<1, R>(sh, p, f, u, lc.P, stream);
+    case 2: return launch_slots<2, R>(sh, p, f, u, lc.P, stream);
+    case 3: return launch_slots<3, R>(sh, p, f, u, lc.P, stream);
+    case 4: return launch_slots<4, R>(sh, p, f, u, lc.P, stream);
+    case 5: return launch_slots<5, R>(sh, p, f, u, lc.P, stream);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_forward(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f, int mode,
-                          hipStream_t stream) {
-  const LayoutChoice lc = choose_layout(sh.nk, mode, p.ntimes);
+// u.delp == NULL: the microphysics alone (the state update is skipped).
+hipError_t launch_forward(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
+                          const shield_update_fields& u, int mode, hipStream_t stream) {
+  const LayoutChoice lc = choose_layout(sh.nk, mode, p.ntimes, u.delp != nullptr);
   if (lc.S == 0) return hipErrorInvalidValue;
-  if (mode == SHIELD_MP_FP64) return launch_r<double>(sh, p, f, lc, stream);
-  return launch_r<float>(sh, p, f, lc, stream);
+  if (mode == SHIELD_MP_FP64) return launch_r<double>(sh, p, f, u, lc, stream);
+  return launch_r<float>(sh, p, f, u, lc, stream);
 }
 
 }  // namespace shield

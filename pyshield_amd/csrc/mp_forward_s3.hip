@@ -5,9 +5,11 @@
 
 namespace shield {
 template hipError_t launch_slots<3, float>(const shield_mp_shape&, const shield_mp_params&,
-                                            const shield_mp_fields&, int, hipStream_t);
+                                            const shield_mp_fields&, const shield_update_fields&, int,
+                                            hipStream_t);
 template hipError_t launch_slots<3, double>(const shield_mp_shape&, const shield_mp_params&,
-                                             const shield_mp_fields&, int, hipStream_t);
-template int waves_per_simd<3, float>(bool);
-template int waves_per_simd<3, double>(bool);
+                                             const shield_mp_fields&, const shield_update_fields&, int,
+                                            hipStream_t);
+template int waves_per_simd<3, float>(bool, bool);
+template int waves_per_simd<3, double>(bool, bool);
 }  // namespace shield

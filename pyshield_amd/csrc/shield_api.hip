@@ -6,12 +6,12 @@
 #include "mp_common.h"
 
 namespace shield {
-hipError_t launch_forward(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f, int mode,
-                          hipStream_t stream);
+hipError_t launch_forward(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
+                          const shield_update_fields& u, int mode, hipStream_t stream);
 struct LayoutChoice {
   int S, P;
 };
-LayoutChoice choose_layout(int64_t nz, int mode, int ntimes);
+LayoutChoice choose_layout(int64_t nz, int mode, int ntimes, bool fused);
 }  // namespace shield
 
 using namespace shield;
@@ -150,7 +150,7 @@ int shield_mp_update_timestep(shield_mp_params* p, double timestep) {
 
 int shield_mp_layout(int64_t nk, int mode, int ntimes, int* cols_per_wave, int* slots) {
   if (mode != SHIELD_MP_FP64 && mode != SHIELD_MP_MIXED) return SHIELD_ERR_MODE;
-  const LayoutChoice lc = choose_layout(nk, mode, ntimes);
+  const LayoutChoice lc = choose_layout(nk, mode, ntimes, false);
   if (cols_per_wave) *cols_per_wave = lc.P;
   if (slots) *slots = lc.S;
   return lc.S == 0 ? SHIELD_ERR_SHAPE : SHIELD_OK;
@@ -177,14 +177,20 @@ const char* shield_strerror(int code) {
   }
 }
 
-int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f, int mode,
-                      void* stream) {
+// Checks shared by shield_mp_forward and shield_mp_forward_update; 1 when
+// the domain is empty (nothing to launch).
+static int check_forward(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                         int mode, int* empty) {
+  *empty = 0;
   if (!shape || !p || !f) return SHIELD_ERR_NULL;
   if (mode != SHIELD_MP_FP64 && mode != SHIELD_MP_MIXED) return SHIELD_ERR_MODE;
   if (p->nml.use_ppm || p->nml.prog_ccn) return SHIELD_ERR_UNSUPPORTED;
   if (!(p->timestep > 0.0) || p->ntimes < 1) return SHIELD_ERR_TIMESTEP;
   if (!shape_ok(shape, 2, 320)) return SHIELD_ERR_SHAPE;
-  if (shape->ni * shape->nj == 0) return SHIELD_OK;
+  if (shape->ni * shape->nj == 0) {
+    *empty = 1;
+    return SHIELD_OK;
+  }
   // the kernel addresses cells with 32-bit element offsets
   const int64_t last3 = (shape->ni - 1) * shape->si + (shape->nj - 1) * shape->sj + (shape->nk - 1);
   const int64_t last2 = (shape->ni - 1) * shape->si2 + (shape->nj - 1) * shape->sj2;
@@ -212,7 +218,57 @@ int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p, c
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SHIELD_ERR_NOGPU;
-  const hipError_t e = launch_forward(*shape, *p, *f, mode, (hipStream_t)stream);
+  return SHIELD_OK;
+}
+
+int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f, int mode,
+                      void* stream) {
+  int empty = 0;
+  const int rc = check_forward(shape, p, f, mode, &empty);
+  if (rc != SHIELD_OK || empty) return rc;
+  shield_update_fields none;
+  memset(&none, 0, sizeof(none));
+  const hipError_t e = launch_forward(*shape, *p, *f, none, mode, (hipStream_t)stream);
+  return e == hipSuccess ? SHIELD_OK : SHIELD_ERR_LAUNCH;
+}
+
+int shield_mp_forward_update(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                             const shield_update_fields* u, int mode, void* stream) {
+  if (!u) return SHIELD_ERR_NULL;
+  for (int n = 0; n < 10; ++n)
+    if (!u->updated[n]) return SHIELD_ERR_NULL;
+  for (int n = 0; n < 6; ++n)
+    if (!u->tracers[n]) return SHIELD_ERR_NULL;
+  if (!u->u_dt || !u->v_dt || !u->pt_dt || !u->delp || !u->prsi) return SHIELD_ERR_NULL;
+  if (!shape) return SHIELD_ERR_NULL;
+  if (u->si_i < 0 || u->sj_i < 0) return SHIELD_ERR_SHAPE;
+  if (shape->ni * shape->nj > 0 &&
+      (shape->ni - 1) * u->si_i + (shape->nj - 1) * u->sj_i + shape->nk > INT32_MAX)
+    return SHIELD_ERR_SHAPE;
+  if (!(u->dt > 0.0) || !isfinite(u->dt) || !isfinite(u->rdt)) return SHIELD_ERR_TIMESTEP;
+  // the update's outputs are distinct from each other and from every
+  // microphysics field
+  const void* uo[] = {u->updated[0], u->updated[1], u->updated[2], u->updated[3], u->updated[4],
+                      u->updated[5], u->updated[6], u->updated[7], u->updated[8], u->updated[9],
+                      u->u_dt,       u->v_dt,       u->pt_dt,      u->tracers[0], u->tracers[1],
+                      u->tracers[2], u->tracers[3], u->tracers[4], u->tracers[5], u->delp};
+  const int nuo = (int)(sizeof(uo) / sizeof(uo[0]));
+  if (f) {
+    const void* mp[] = {f->pt,    f->qv,    f->ql,    f->qr,    f->qi,    f->qs,    f->qg,    f->qa,
+                        f->ua,    f->va,    f->delp,  f->delz,  f->w,     f->pt_dt, f->qv_dt, f->ql_dt,
+                        f->qr_dt, f->qi_dt, f->qs_dt, f->qg_dt, f->qa_dt, f->udt,   f->vdt,   f->qaz};
+    for (int a = 0; a < nuo; ++a) {
+      for (int b = a + 1; b < nuo; ++b)
+        if (uo[a] == uo[b]) return SHIELD_ERR_ALIAS;
+      for (const void* q : mp)
+        if (q && uo[a] == q) return SHIELD_ERR_ALIAS;
+      if (uo[a] == u->prsi) return SHIELD_ERR_ALIAS;
+    }
+  }
+  int empty = 0;
+  const int rc = check_forward(shape, p, f, mode, &empty);
+  if (rc != SHIELD_OK || empty) return rc;
+  const hipError_t e = launch_forward(*shape, *p, *f, *u, mode, (hipStream_t)stream);
   return e == hipSuccess ? SHIELD_OK : SHIELD_ERR_LAUNCH;
 }
 

@@ -200,12 +200,8 @@ FIELDS_3D = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp",
 FIELDS_2D = ("land", "area", "rain", "snow", "ice", "graupel")
 
 
-def forward(fields: dict, params, mode=_lib.SHIELD_MP_FP64, stream=None):
-    """One microphysics call on tensors: ``fields`` maps the names of
-    shield_mp_fields to float64 GPU tensors ([ni, nj, nk], k contiguous; 2D
-    [ni, nj]).  Launches on ``stream`` (default: torch's current stream)."""
-    if isinstance(mode, str):
-        mode = _lib.MODES[mode]
+def _mp_args(fields: dict):
+    """(shape, shield_mp_fields) of a field dict; checks dtypes, shapes, strides."""
     pt = fields["pt"]
     ni, nj, nk = pt.shape
     t3 = [fields[n] for n in FIELDS_3D if fields.get(n) is not None]
@@ -223,8 +219,62 @@ def forward(fields: dict, params, mode=_lib.SHIELD_MP_FP64, stream=None):
     for n in _lib.FIELD_NAMES:
         t = fields.get(n)
         setattr(f, n, t.data_ptr() if t is not None else None)
+    return shape, f
+
+
+def forward(fields: dict, params, mode=_lib.SHIELD_MP_FP64, stream=None):
+    """One microphysics call on tensors: ``fields`` maps the names of
+    shield_mp_fields to float64 GPU tensors ([ni, nj, nk], k contiguous; 2D
+    [ni, nj]).  Launches on ``stream`` (default: torch's current stream)."""
+    if isinstance(mode, str):
+        mode = _lib.MODES[mode]
+    shape, f = _mp_args(fields)
     if stream is None:
-        stream = torch.cuda.current_stream(pt.device).cuda_stream
+        stream = torch.cuda.current_stream(fields["pt"].device).cuda_stream
     rc = _lib.lib().shield_mp_forward(ctypes.byref(shape), ctypes.byref(params), ctypes.byref(f),
                                       int(mode), ctypes.c_void_p(stream))
     _lib.check(rc, "shield_mp_forward")
+
+
+def forward_update(fields: dict, update: dict, params, dt: float, rdt: float, q_min: float = 1.0e-9,
+                   mode=_lib.SHIELD_MP_FP64, stream=None):
+    """The microphysics call and, in the same kernel, the state update that
+    Physics.__call__ and UpdateAtmosphereState.__call__ run after it
+    (physics.py:331-366, update_atmos_state.py:301-349 without
+    ApplyPhysicsToDycore): ``physics_updated_*`` by forward Euler over ``dt``,
+    fill_gfs_delp of the updated vapour (threshold ``q_min``) with the dycore
+    delp, and the GFS -> FV3 tracer and delp conversion with the dycore
+    tendencies accumulated over ``rdt``.
+
+    ``update`` holds: "updated" (dict of the ten physics_updated tensors keyed
+    by pyshield_amd._lib.UPDATED_NAMES), "u_dt", "v_dt", "pt_dt", "tracers"
+    (dict keyed by TRACER_NAMES), "delp" (all [ni, nj, nk] with the strides of
+    the microphysics fields) and "prsi" ([ni, nj, nk + 1], k contiguous)."""
+    if isinstance(mode, str):
+        mode = _lib.MODES[mode]
+    shape, f = _mp_args(fields)
+    ni, nj, nk = fields["pt"].shape
+    ref = fields["pt"]
+    layer = [update["updated"][n] for n in _lib.UPDATED_NAMES]
+    layer += [update["u_dt"], update["v_dt"], update["pt_dt"], update["delp"]]
+    layer += [update["tracers"][n] for n in _lib.TRACER_NAMES]
+    for t in layer:
+        _check3("update field", t, (ni, nj, nk))
+        if t.stride()[:2] != ref.stride()[:2]:
+            raise ValueError("state-update fields must have the strides of the microphysics fields")
+    prsi = update["prsi"]
+    _check3("prsi", prsi, (ni, nj, nk + 1))
+    u = _lib.ShieldUpdateFields()
+    for i, n in enumerate(_lib.UPDATED_NAMES):
+        u.updated[i] = update["updated"][n].data_ptr()
+    for i, n in enumerate(_lib.TRACER_NAMES):
+        u.tracers[i] = update["tracers"][n].data_ptr()
+    u.u_dt, u.v_dt, u.pt_dt = (update[k].data_ptr() for k in ("u_dt", "v_dt", "pt_dt"))
+    u.delp, u.prsi = update["delp"].data_ptr(), prsi.data_ptr()
+    u.si_i, u.sj_i = prsi.stride(0), prsi.stride(1)
+    u.dt, u.rdt, u.q_min = float(dt), float(rdt), float(q_min)
+    if stream is None:
+        stream = torch.cuda.current_stream(ref.device).cuda_stream
+    rc = _lib.lib().shield_mp_forward_update(ctypes.byref(shape), ctypes.byref(params), ctypes.byref(f),
+                                             ctypes.byref(u), int(mode), ctypes.c_void_p(stream))
+    _lib.check(rc, "shield_mp_forward_update")

@@ -104,3 +104,48 @@ def prepare_tendencies_and_update_tracers(u_dt, v_dt, pt_dt, u_t1, v_t1, physics
     rc = _lib.lib().shield_prepare_tendencies_and_update_tracers(ctypes.byref(_shape(delp, prsi)), *args, float(rdt),
                                                                  _stream(delp))
     _lib.check(rc, "shield_prepare_tendencies_and_update_tracers")
+
+
+def _tensor(x):
+    """A torch tensor from a tensor or a Quantity-like object with .data."""
+    return x.data if hasattr(x, "data") and not isinstance(x, torch.Tensor) else x
+
+
+class UpdateAtmosphereState:
+    """UpdateAtmosphereState (pySHiELD/update/update_atmos_state.py:235-349)
+    without its ApplyPhysicsToDycore step: fill_gfs_delp of the updated vapour
+    (or, dycore-only, of the dycore vapour) and
+    prepare_tendencies_and_update_tracers, each one kernel on the GPU.
+
+    Fields are the nk physical layers ([X, Y, Z] tensors or objects holding
+    one in ``.data``); ``phy_state.prsi`` has nk + 1 interfaces.  The reference
+    runs fill_gfs_delp over the halo columns too; it fills whatever columns
+    the passed tensors span.  ``apply_tendencies=True`` needs the cubed-sphere
+    A-to-D wind interpolation and a halo exchange, which are not part of this
+    library, and raises NotImplementedError.
+    """
+
+    def __init__(self, stencil_factory=None, grid_data=None, namelist=None, comm=None, grid_info=None,
+                 state=None, quantity_factory=None, dycore_only: bool = False, apply_tendencies: bool = False,
+                 tendency_state=None):
+        if apply_tendencies:
+            raise NotImplementedError("ApplyPhysicsToDycore (fv_update_phys.py) is not part of pyshield_amd")
+        if namelist is None:
+            raise ValueError("namelist (with dt_atmos) is required")
+        self._rdt = 1.0 / float(namelist.dt_atmos)
+        self._dycore_only = dycore_only
+
+    def __call__(self, dycore_state, phy_state, u_dt, v_dt, pt_dt, dt: float):
+        T = _tensor
+        if self._dycore_only:
+            fill_gfs_delp(T(dycore_state.delp), T(dycore_state.qvapor), 1.0e-9)
+            return
+        fill_gfs_delp(T(dycore_state.delp), T(phy_state.physics_updated_specific_humidity), 1.0e-9)
+        prepare_tendencies_and_update_tracers(
+            T(u_dt), T(v_dt), T(pt_dt), T(phy_state.physics_updated_ua), T(phy_state.physics_updated_va),
+            T(phy_state.physics_updated_pt), T(phy_state.physics_updated_specific_humidity),
+            T(phy_state.physics_updated_qliquid), T(phy_state.physics_updated_qrain),
+            T(phy_state.physics_updated_qsnow), T(phy_state.physics_updated_qice),
+            T(phy_state.physics_updated_qgraupel), T(phy_state.ua), T(phy_state.va), T(phy_state.pt),
+            T(dycore_state.qvapor), T(dycore_state.qliquid), T(dycore_state.qrain), T(dycore_state.qsnow),
+            T(dycore_state.qice), T(dycore_state.qgraupel), T(phy_state.prsi), T(dycore_state.delp), self._rdt)

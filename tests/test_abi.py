@@ -37,6 +37,60 @@ def test_struct_sizes_match_layout():
     assert ctypes.sizeof(_lib.ShieldMpNamelist) == 37 * 8 + 20 * 4
     assert ctypes.sizeof(_lib.ShieldMpShape) == 7 * 8
     assert ctypes.sizeof(_lib.ShieldMpFields) == len(_lib.FIELD_NAMES) * 8
+    # 21 pointers, two int64 strides, three doubles
+    assert ctypes.sizeof(_lib.ShieldUpdateFields) == 21 * 8 + 2 * 8 + 3 * 8
+
+
+def _fake_update(base=0x100000, alias=None):
+    u = _lib.ShieldUpdateFields()
+    ptrs = iter(range(base, base + 0x1000 * 30, 0x1000))
+    for i in range(10):
+        u.updated[i] = next(ptrs)
+    for i in range(6):
+        u.tracers[i] = next(ptrs)
+    u.u_dt, u.v_dt, u.pt_dt, u.delp, u.prsi = (next(ptrs) for _ in range(5))
+    u.si_i, u.sj_i, u.dt, u.rdt, u.q_min = 80, 80 * 4, 150.0, 1.0 / 150.0, 1.0e-9
+    if alias:
+        alias(u)
+    return u
+
+
+def test_forward_update_rejects_bad_arguments():
+    L = _lib.lib()
+    p = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
+    shape = _lib.ShieldMpShape(3, 4, 79, 79 * 4, 79, 4, 1)
+    fwd = L.shield_mp_forward_update
+    f = _fake_fields()
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), None, 0, None) == -1
+    missing = _fake_update()
+    missing.tracers[3] = None
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(missing), 0, None) == -1
+    # an updated field written over a microphysics input
+    def over_input(u):
+        u.updated[2] = f.qr
+    bad = _fake_update(alias=over_input)
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(bad), 0, None) == -5
+    # two update outputs sharing memory
+    def twice(u):
+        u.tracers[0] = u.delp
+    bad = _fake_update(alias=twice)
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(bad), 0, None) == -5
+    bad = _fake_update()
+    bad.dt = 0.0
+    assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(bad), 0, None) == -7
+    # every check passes: without a GPU the call reports it
+    if _lib.device_count() == 0:
+        ok = _fake_update()
+        assert fwd(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(ok), 0, None) == -8
+
+
+def test_update_atmosphere_state_scope():
+    from types import SimpleNamespace
+    from pyshield_amd.update import UpdateAtmosphereState
+    nml = SimpleNamespace(dt_atmos=225)
+    with pytest.raises(NotImplementedError):
+        UpdateAtmosphereState(namelist=nml, apply_tendencies=True)
+    assert UpdateAtmosphereState(namelist=nml)._rdt == pytest.approx(1.0 / 225.0)
 
 
 def test_setup_errors():
