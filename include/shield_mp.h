@@ -216,6 +216,41 @@ int shield_prepare_tendencies_and_update_tracers(
     double* qv_t0, double* ql_t0, double* qr_t0, double* qs_t0, double* qi_t0,
     double* qg_t0, const double* prsi, double* delp, double rdt, void* stream);
 
+/* ------------------------------------------- Physics driver prologue */
+/* Fields of shield_physics_prologue.  Layer fields have nk levels and use
+ * shape->si/sj; phii, prsi and prsik have nk+1 interfaces with strides
+ * si_i/sj_i.  The reference stores every field with nk+1 levels; the extra
+ * level of its layer fields is padding that no result reads.               */
+typedef struct shield_prologue_fields {
+  const double* delz;
+  /* in: total layer mass (Pa); out: the layer-mean pressure, which the
+   * reference writes into the same field (physics.py:89-93)                */
+  double* delp;
+  /* qvapor, qliquid, qrain, qice, qsnow, qgraupel, qo3mr: in per total mass,
+   * out per dry mass (physics.py:54-82)                                     */
+  double* q[7];
+  double* qsgs_tke;  /* divided by the dry layer mass (physics.py:82)       */
+  const double* pt;
+  double *phii, *prsi, *prsik;     /* out, nk+1 interfaces                  */
+  double *dm, *delprsi, *del_gz;   /* out: dry layer mass, prsi[k+1]-prsi[k],
+                                    * and del_gz as get_phi_fv3 leaves it   */
+  double* phil;                    /* out: geopotential at layer middles     */
+  /* prepare_microphysics (physics.py:105-143) when omga != NULL: dz, wmp and
+   * the ten tendencies {udt, vdt, pt_dt, qv_dt, ql_dt, qr_dt, qi_dt, qs_dt,
+   * qg_dt, qa_dt} zeroed                                                    */
+  const double* omga;
+  double *dz, *wmp;
+  double* tend[10];
+  int64_t si_i, sj_i;
+  double ptop;  /* model-top pressure (grid_data.ptop)                      */
+} shield_prologue_fields;
+
+/* The column-local part of Physics.__call__ before the microphysics
+ * (physics.py:278-330): atmos_phys_driver_statein (physics.py:26-102),
+ * get_prs_fv3 (get_prs_fv3.py:7-20), get_phi_fv3 (get_phi_fv3.py:7-25) and,
+ * when f->omga is set, prepare_microphysics, in one kernel.  nwat = 6.      */
+int shield_physics_prologue(const shield_mp_shape* shape, const shield_prologue_fields* f, void* stream);
+
 /* ------------------------------------------------------------ utilities */
 const char* shield_strerror(int code);
 /* Number of HIP devices visible (0 without a GPU; never fails).            */

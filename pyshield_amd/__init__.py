@@ -8,9 +8,11 @@ kernels for gfx950 behind the C ABI in include/shield_mp.h.
 from ._config import PHYSICS_PACKAGES, PhysicsConfig
 from .microphysics import (GridData, Microphysics, MicrophysicsState, QuantityFactory,
                            StencilFactory, forward, forward_update)
+from .physics import Physics, physics_prologue
 
 __all__ = [
     "PHYSICS_PACKAGES", "PhysicsConfig", "Microphysics", "MicrophysicsState",
-    "StencilFactory", "QuantityFactory", "GridData", "forward", "forward_update",
+    "StencilFactory", "QuantityFactory", "GridData", "forward", "forward_update", "Physics",
+    "physics_prologue",
 ]
 __version__ = "0.2.0"

@@ -22,7 +22,7 @@ EXPORTS = (
     "shield_mp_setup", "shield_mp_update_timestep", "shield_mp_forward", "shield_mp_forward_update",
     "shield_prepare_microphysics", "shield_update_physics_state_with_tendencies",
     "shield_fill_gfs_delp", "shield_prepare_tendencies_and_update_tracers",
-    "shield_strerror", "shield_device_count", "shield_mp_layout",
+    "shield_strerror", "shield_device_count", "shield_mp_layout", "shield_physics_prologue",
 )
 
 _D = ctypes.c_double
@@ -92,6 +92,13 @@ UPDATED_NAMES = ("qvapor", "qliquid", "qrain", "qice", "qsnow", "qgraupel", "qcl
 TRACER_NAMES = ("qvapor", "qliquid", "qrain", "qsnow", "qice", "qgraupel")
 
 
+class ShieldPrologueFields(ctypes.Structure):
+    _fields_ = [("delz", _VP), ("delp", _VP), ("q", _VP * 7), ("qsgs_tke", _VP), ("pt", _VP), ("phii", _VP),
+                ("prsi", _VP), ("prsik", _VP), ("dm", _VP), ("delprsi", _VP), ("del_gz", _VP), ("phil", _VP),
+                ("omga", _VP), ("dz", _VP), ("wmp", _VP), ("tend", _VP * 10), ("si_i", _I64), ("sj_i", _I64),
+                ("ptop", _D)]
+
+
 class ShieldUpdateFields(ctypes.Structure):
     _fields_ = [("updated", _VP * 10), ("u_dt", _VP), ("v_dt", _VP), ("pt_dt", _VP), ("tracers", _VP * 6),
                 ("delp", _VP), ("prsi", _VP), ("si_i", _I64), ("sj_i", _I64), ("dt", _D), ("rdt", _D),
@@ -120,6 +127,8 @@ def lib():
         L.shield_mp_forward_update.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldMpParams),
                                                ctypes.POINTER(ShieldMpFields), ctypes.POINTER(ShieldUpdateFields),
                                                ctypes.c_int, _VP]
+        L.shield_physics_prologue.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldPrologueFields),
+                                              _VP]
         L.shield_prepare_microphysics.argtypes = [ctypes.POINTER(ShieldMpShape), _VP, _VP, _VP, _VP, _VP,
                                                   _VP, _VP, _VP * 10, _VP]
         L.shield_update_physics_state_with_tendencies.argtypes = [ctypes.POINTER(ShieldMpShape), _VP * 10,

@@ -39,6 +39,32 @@ def test_struct_sizes_match_layout():
     assert ctypes.sizeof(_lib.ShieldMpFields) == len(_lib.FIELD_NAMES) * 8
     # 21 pointers, two int64 strides, three doubles
     assert ctypes.sizeof(_lib.ShieldUpdateFields) == 21 * 8 + 2 * 8 + 3 * 8
+    # 31 pointers, two int64 strides, one double
+    assert ctypes.sizeof(_lib.ShieldPrologueFields) == 31 * 8 + 2 * 8 + 8
+
+
+def test_prologue_rejects_bad_arguments():
+    L = _lib.lib()
+    shape = _lib.ShieldMpShape(3, 4, 79, 79 * 4, 79, 0, 0)
+    f = _lib.ShieldPrologueFields()
+    names = [n for n, _ in f._fields_ if n not in ("q", "tend", "omga", "dz", "wmp", "si_i", "sj_i", "ptop")]
+    for i, n in enumerate(names):
+        setattr(f, n, 0x1000 * (i + 1))
+    for i in range(7):
+        f.q[i] = 0x100000 + 0x1000 * i
+    f.si_i, f.sj_i, f.ptop = 80 * 4, 80, 300.0
+    assert L.shield_physics_prologue(ctypes.byref(shape), None, None) == -1
+    f.q[4] = None
+    assert L.shield_physics_prologue(ctypes.byref(shape), ctypes.byref(f), None) == -1
+    f.q[4] = 0x200000
+    f.omga = 0x300000          # prepare_microphysics requested without its outputs
+    assert L.shield_physics_prologue(ctypes.byref(shape), ctypes.byref(f), None) == -1
+    f.omga = None
+    f.ptop = 0.0
+    assert L.shield_physics_prologue(ctypes.byref(shape), ctypes.byref(f), None) == -2
+    f.ptop = 300.0
+    if _lib.device_count() == 0:
+        assert L.shield_physics_prologue(ctypes.byref(shape), ctypes.byref(f), None) == -8
 
 
 def _fake_update(base=0x100000, alias=None):

@@ -1,0 +1,105 @@
+"""The Physics-prologue kernel (shield_physics_prologue) against the NumPy
+restatements (oracle/prologue_numpy.py, oracle/update_numpy.py), and the
+whole Physics.__call__ (prologue, microphysics, forward Euler) against the
+oracle chain."""
+
+import numpy as np
+import pytest
+
+from oracle import prologue_numpy as PR
+from oracle import update_numpy as U
+from pyshield_amd._config import PhysicsConfig
+from pyshield_amd.synthetic import make_state
+
+from helpers import TEND, norm_err, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+PTOP = 300.0
+
+
+def _state(ni=6, nj=5, nz=79, seed=41):
+    st = make_state(ni, nj, nz, seed)
+    rng = np.random.default_rng(seed)
+    sh = (ni, nj, nz)
+    # the tracers as the dycore hands them over: per total mass
+    q = {n: st[src] for n, src in zip(PR.SPECIES[:6], ("qv", "ql", "qr", "qi", "qs", "qg"))}
+    q["qo3mr"] = rng.uniform(1e-8, 1e-5, sh)
+    return st, q, rng.uniform(0, 1, sh), rng.normal(0, 1.0, sh)
+
+
+def _t(x):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda:0").clone()
+
+
+def test_prologue_matches_restatement():
+    import torch
+    from pyshield_amd.physics import physics_prologue
+    st, q, tke, omga = _state()
+    ni, nj, nz = st["pt"].shape
+    z = lambda k: torch.zeros((ni, nj, k), dtype=torch.float64, device="cuda:0")  # noqa: E731
+    g = {"delz": _t(st["delz"]), "delp": _t(st["delp"]), "qsgs_tke": _t(tke), "pt": _t(st["pt"]),
+         "phii": z(nz + 1), "prsi": z(nz + 1), "prsik": z(nz + 1), "dm": z(nz), "delprsi": z(nz),
+         "del_gz": z(nz), "phil": z(nz), "omga": _t(omga), "dz": z(nz), "wmp": z(nz)}
+    gq = {n: _t(q[n]) for n in PR.SPECIES}
+    tend = [_t(np.full((ni, nj, nz), 5.0)) for _ in range(10)]
+    physics_prologue(g["delz"], g["delp"], gq, g["qsgs_tke"], g["pt"], g["phii"], g["prsi"], g["prsik"], g["dm"],
+                     g["delprsi"], g["del_gz"], g["phil"], PTOP, omga=g["omga"], dz=g["dz"], wmp=g["wmp"],
+                     tend=tend)
+    torch.cuda.synchronize()
+    ref = PR.prologue(st["delz"], st["delp"], q, tke, st["pt"], PTOP)
+    rdz, rw = U.prepare_microphysics(ref["phii"], omga, ref["qvapor"], st["pt"], ref["delp"])
+    ref.update(dz=rdz, wmp=rw)
+    got = {n: g[n].cpu().numpy() for n in ("phii", "prsi", "prsik", "delp", "qsgs_tke", "dm", "delprsi",
+                                           "del_gz", "phil", "dz", "wmp")}
+    got.update({n: gq[n].cpu().numpy() for n in PR.SPECIES})
+    errs = {n: norm_err(got[n], ref[n], floor=1e-30) for n in got}
+    print({k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert all(v <= 1e-13 for v in errs.values()), errs
+    for t in tend:
+        assert torch.all(t == 0.0)
+
+
+def test_physics_call_matches_oracle_chain():
+    import torch
+    from types import SimpleNamespace
+    from pyshield_amd.microphysics import GridData, MicrophysicsState, QuantityFactory, StencilFactory
+    from pyshield_amd.physics import Physics
+    st, q, tke, omga = _state(seed=43)
+    ni, nj, nz = st["pt"].shape
+    dt = 150.0
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=nz)
+    z = lambda k=nz: torch.zeros((ni, nj, k), dtype=torch.float64, device="cuda:0")  # noqa: E731
+    s = SimpleNamespace(delz=_t(st["delz"]), delp=_t(st["delp"]), pt=_t(st["pt"]), ua=_t(st["ua"]),
+                        va=_t(st["va"]), qcld=_t(st["qa"]), qsgs_tke=_t(tke), omga=_t(omga), phii=z(nz + 1),
+                        prsi=z(nz + 1), delprsi=z(), dz=z(), wmp=z(), phil=z(), land=_t(st["land"]),
+                        **{n: _t(q[n]) for n in PR.SPECIES})
+    for n in ("specific_humidity", "qliquid", "qrain", "qice", "qsnow", "qgraupel", "cloud_fraction", "pt", "ua",
+              "va"):
+        setattr(s, "physics_updated_" + n, z())
+    s.microphysics = MicrophysicsState(s.pt, s.qvapor, s.qliquid, s.qrain, s.qice, s.qsnow, s.qgraupel, s.qcld,
+                                       s.ua, s.va, s.delp, s.delz, s.omga, s.delprsi, s.wmp, s.dz, z(), s.land)
+    phys = Physics(StencilFactory((ni, nj, nz)), QuantityFactory((ni, nj, nz)),
+                   SimpleNamespace(ptop=PTOP, area=_t(st["area"])), cfg)
+    phys(s, dt)
+    torch.cuda.synchronize()
+    # the oracle chain: prologue restatement, C oracle on the converted state
+    # (delprsi as the microphysics' delp), forward Euler
+    pro = PR.prologue(st["delz"], st["delp"], q, tke, st["pt"], PTOP)
+    ms = dict(st, delp=pro["delprsi"], qv=pro["qvapor"], ql=pro["qliquid"], qr=pro["qrain"], qi=pro["qice"],
+              qs=pro["qsnow"], qg=pro["qgraupel"])
+    ora = run_oracle(ms, cfg, dt)
+    ins = [pro["qvapor"], pro["qliquid"], pro["qrain"], pro["qice"], pro["qsnow"], pro["qgraupel"], st["qa"],
+           st["pt"], st["ua"], st["va"]]
+    tds = [ora[n] for n in ("qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa_dt", "pt_dt", "udt", "vdt")]
+    ref = U.update_physics_state_with_tendencies(ins, tds, dt)
+    names = ("specific_humidity", "qliquid", "qrain", "qice", "qsnow", "qgraupel", "cloud_fraction", "pt", "ua",
+             "va")
+    errs = {n: norm_err(getattr(s, "physics_updated_" + n).cpu().numpy(), r, floor=1e-30)
+            for n, r in zip(names, ref)}
+    m = s.microphysics
+    for n in TEND:
+        errs[n] = norm_err(getattr(m, n).cpu().numpy(), ora[n], floor=1e-30)
+    print({k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert all(v <= 1e-9 for v in errs.values()), errs
