@@ -105,6 +105,25 @@ __device__ __forceinline__ double pw<double>(double x, double p) { return pow(x,
 template <>
 __device__ __forceinline__ float pw<float>(float x, double p) { return pw_split(x, p); }
 
+// a / b for the FP64 parts of the process code.  FP64 mode keeps the IEEE
+// division of the reference's expression; mixed mode multiplies by a
+// reciprocal from v_rcp_f64 refined by two Newton steps (within an ulp, half
+// the instructions), which the compiler shares between divisions by the same
+// value (lhl / cvm and lhi / cvm, ...).
+__device__ __forceinline__ double rcp64(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return r;
+}
+template <class R>
+__device__ __forceinline__ double fdiv(double a, double b) {
+  if constexpr (sizeof(R) == 4)
+    return a * rcp64(b);
+  else
+    return a / b;
+}
+
 template <class T>
 __device__ __forceinline__ T mmin(T a, T b) { return a < b ? a : b; }
 template <class T>

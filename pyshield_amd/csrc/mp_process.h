@@ -17,9 +17,12 @@
 //    accurate to FP64;
 //  * temperature differences (tz - tice, ...) and the limiters that scale a
 //    set of sinks to what is available;
-//  * limiters that bound a sink by the species it drains are evaluated
-//    against the FP64 state (`cap`), so no species is driven below zero by
-//    FP32 rounding.
+//  * where the reference's arithmetic bounds a sink by the species it drains
+//    (x = f / (1 + f) q, min(q - qc, ...), x <= qv - qsat, ...), an FP32
+//    sink that rounding put just above the FP64 state is bounded to it
+//    (`cap`), so FP32 rounding drives no species below zero; sinks the
+//    reference leaves unbounded (the subgrid autoconversion) and states the
+//    reference itself drove negative are left as they are.
 #pragma once
 
 #include "mp_common.h"
@@ -39,11 +42,15 @@ struct is_f32<float> {
   static constexpr bool value = true;
 };
 
-// Bound a transfer x out of species q (mixed precision only; identity in FP64
-// where the reference's arithmetic already guarantees x <= q).
+// Bound a transfer x out of species q at a site where the reference's
+// arithmetic guarantees x <= q (mixed precision only; identity in FP64).  It
+// binds only where FP32 rounding put x at most 1e-5 q above a positive q: a
+// reference state can hold a negative species (an autoconversion larger than
+// the cloud water leaves ql < 0, and qv follows), and there the reference's
+// transfer (0, or f q / (1 + f)) stays as it is.
 template <class R>
 __device__ __forceinline__ double cap(double x, double q) {
-  if constexpr (is_f32<R>::value) return x < q ? x : q;
+  if constexpr (is_f32<R>::value) return (x > q && x <= q * (1.0 + 1.0e-5)) ? q : x;
   return x;
 }
 
@@ -102,6 +109,49 @@ __device__ __forceinline__ R iqs2(R ta, R den, R& dqdt) {
   return tmp;
 }
 
+// Saturation mixing ratio and gradient in the mixed-precision kernel: the
+// closed forms above, evaluated in FP64 with the divisions by constants
+// folded into products and the others replaced by reciprocals
+// ((ta - TICE) / (ta TICE) = 1/TICE - 1/ta).  Below T_SAT_MIN and above
+// TICE + 102 the reference's branches are the same expression at the clamped
+// temperature.
+__device__ __forceinline__ double qs_fast(double t, double d, double l, double den, double& dqdt) {
+  constexpr double rtice = 1.0 / TICE, rrv = 1.0 / RVGAS;
+  const double rt = rcp64(t);
+  const double e = E00 * exp(d * rrv * log(t * rtice) + l * rrv * (rtice - rt));
+  const double q = e * rrv * rt * rcp64(den);
+  dqdt = q * (d + l * rt) * rrv * rt;
+  return q;
+}
+__device__ __forceinline__ double iqs_fast(double ta, double den, double& dqdt) {
+  const bool ice = ta < TICE;
+  const double t = ice ? (ta < T_SAT_MIN ? T_SAT_MIN : ta) : (ta > TICE + 102.0 ? TICE + 102.0 : ta);
+  return qs_fast(t, ice ? D2ICE : DC_VAP, ice ? LI2 : LV0, den, dqdt);
+}
+// the reference's forms in FP64 mode, the folded ones in mixed mode
+template <class R>
+__device__ __forceinline__ double qs_w(double ta, double den, double& dqdt) {
+  if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, dqdt);
+  else return wqs2<double>(ta, den, dqdt);
+}
+template <class R>
+__device__ __forceinline__ double qs_w1(double ta, double den) {
+  double g;
+  if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, g);
+  else return wqs1<double>(ta, den);
+}
+template <class R>
+__device__ __forceinline__ double qs_i(double ta, double den, double& dqdt) {
+  if constexpr (is_f32<R>::value) return iqs_fast(ta, den, dqdt);
+  else return iqs2<double>(ta, den, dqdt);
+}
+template <class R>
+__device__ __forceinline__ double qs_i1(double ta, double den) {
+  double g;
+  if constexpr (is_f32<R>::value) return iqs_fast(ta, den, g);
+  else return iqs1<double>(ta, den);
+}
+
 // ------------------------------------------------ f:149-179
 template <class R>
 __device__ __forceinline__ R acr3d(R v1, R v2, R q1, R q2, R c, R cac_ik, R cac_i1k, R cac_i2k, R rho) {
@@ -135,11 +185,11 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
     double q_liq = s.ql + s.qr;
     double q_sol = s.qi + s.qs + s.qg;
     double cvm = SH_CVM(c, s, q_liq, q_sol);
-    double lcpk = lhl / cvm;
+    double lcpk = fdiv<R>(lhl, cvm);
     const double tin = s.tz - double(lcpk * R(s.ql));
     const double qpz = s.qv + s.ql;
     double dqsdt;
-    const double qsat = wqs2<double>(tin, den_d, dqsdt);
+    const double qsat = qs_w<R>(tin, den_d, dqsdt);
     R dqh = mmax(R(s.ql), h_var * mmax(R(qpz), R(QCMIN)));
     dqh = mmin(dqh, R(0.2) * R(qpz));
     const double dqv = qsat - s.qv;
@@ -155,12 +205,12 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
       R t2 = R(tin) * R(tin);
       R evap = ck(c, T_CREVP + 0) * t2 * dq * (ck(c, T_CREVP + 1) * msqrt(qden) + ck(c, T_CREVP + 2) * pw_el<R>(qden, 0.725)) /
                (ck(c, T_CREVP + 3) * t2 + ck(c, T_CREVP + 4) * R(qsat) * den);
-      double ev = mmin(s.qr, mmin(double(dt * evap), dqv / (1.0 + lcpk * dqsdt)));
+      double ev = mmin(s.qr, mmin(double(dt * evap), fdiv<R>(dqv, 1.0 + lcpk * dqsdt)));
       s.qr = s.qr - ev;
       s.qv = s.qv + ev;
       q_liq = q_liq - ev;
       cvm = SH_CVM(c, s, q_liq, q_sol);
-      s.tz = s.tz - ev * lhl / cvm;
+      s.tz = s.tz - fdiv<R>(ev * lhl, cvm);
     }
     if ((s.qr > QRMIN) && (s.ql > 1.0e-6) && (qsat < q_minus)) {
       R sink = dt * denfac * c.cracw * pw_el<R>(R(s.qr) * den, 0.95);
@@ -242,8 +292,10 @@ __device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_
     R qc = c.use_ccn ? qc0 : qc0 / den;
     R dq = R(0.5 * (s.ql + double(dl) - double(qc)));
     if (dq > R(0.0)) {
+      // unbounded by ql, as in the reference: a cell with ql ~ 1e-2 and a
+      // long rain step converts more than it holds and leaves ql < 0
       R sink = mmin(R(1.0), dq / dl) * c.dt_rain * c_praut * den * pw_el<R>(R(s.ql), 7.0 / 3.0);
-      double sk = cap<R>(double(sink), s.ql);
+      const double sk = double(sink);
       s.ql = s.ql - sk;
       s.qr = s.qr + sk;
     }
@@ -261,8 +313,8 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   double q_liq = s.ql + s.qr;
   double q_sol = s.qi + s.qs + s.qg;
   double cvm = SH_CVM(c, s, q_liq, q_sol);
-  double lcpk = lhl / cvm;
-  double icpk = lhi / cvm;
+  double lcpk = fdiv<R>(lhl, cvm);
+  double icpk = fdiv<R>(lhi, cvm);
   double tcpk = lcpk + icpk;
   double tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
   if (!(p1 >= P_MIN)) return;
@@ -272,14 +324,14 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     s.qi = s.qi + sink;
     q_sol = q_sol + sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + sink * (lhl + lhi) / cvm;
+    s.tz = s.tz + fdiv<R>(sink * (lhl + lhi), cvm);
     if (!c.do_qa) s.qa = s.qa + 1.0;
     return;
   }
   lhl = c.lv00_d + c.d0_vap_d * s.tz;
   lhi = LI00 + DC_ICE * s.tz;
-  lcpk = lhl / cvm;
-  icpk = lhi / cvm;
+  lcpk = fdiv<R>(lhl, cvm);
+  icpk = fdiv<R>(lhi, cvm);
   tcpk = lcpk + icpk;
   tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
 
@@ -291,7 +343,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   const double t_sub6 = c.t_sub_d + 6.0;
   double rh = 0.0;
   if (tin > t_sub6) {
-    rh = qpz / iqs1<double>(tin, den_d);
+    rh = fdiv<R>(qpz, qs_i1<R>(tin, den_d));
     if (rh < rh_adj) {
       // all vapour, liquid and ice of the cell collapse into vapour
       s.tz = tin;
@@ -304,36 +356,36 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // cloud water <-> vapour adjustment
   double dwsdt;
-  const double qsw = wqs2<double>(s.tz, den_d, dwsdt);
+  const double qsw = qs_w<R>(s.tz, den_d, dwsdt);
   const double dq0 = qsw - s.qv;
   double evap;
   if (dq0 > 0.0) {
     R factor = mmin(R(1.0), c.fac_l2v * R(10.0 * dq0 / qsw));
-    evap = mmin(s.ql, double(factor) * dq0 / (1.0 + tcp3 * dwsdt));
+    evap = mmin(s.ql, fdiv<R>(double(factor) * dq0, 1.0 + tcp3 * dwsdt));
   } else {
-    evap = dq0 / (1.0 + tcp3 * dwsdt);
+    evap = fdiv<R>(dq0, 1.0 + tcp3 * dwsdt);
   }
   s.qv = s.qv + evap;
   s.ql = s.ql - evap;
   q_liq = q_liq - evap;
   cvm = SH_CVM(c, s, q_liq, q_sol);
-  s.tz = s.tz - evap * lhl / cvm;
+  s.tz = s.tz - fdiv<R>(evap * lhl, cvm);
   lhi = LI00 + DC_ICE * s.tz;
-  icpk = lhi / cvm;
+  icpk = fdiv<R>(lhi, cvm);
 
   // complete freezing below t_wfr
   const double dtmp = c.t_wfr_d - s.tz;
   if ((dtmp > 0.0) && (s.ql > QCMIN)) {
-    double sink = mmin(s.ql, mmin(double(R(s.ql) * R(dtmp) * R(0.125)), dtmp / icpk));
+    double sink = mmin(s.ql, mmin(double(R(s.ql) * R(dtmp) * R(0.125)), fdiv<R>(dtmp, icpk)));
     s.ql = s.ql - sink;
     s.qi = s.qi + sink;
     q_liq = q_liq - sink;
     q_sol = q_sol + sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + sink * lhi / cvm;
+    s.tz = s.tz + fdiv<R>(sink * lhi, cvm);
   }
   lhi = LI00 + DC_ICE * s.tz;
-  icpk = lhi / cvm;
+  icpk = fdiv<R>(lhi, cvm);
 
   // Bigg mechanism
   R dt_pisub;
@@ -345,27 +397,27 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     const R tc = R(tcd);
     if ((s.ql > QRMIN) && (tcd > 0.0)) {
       R sink = R(3.3333e-10) * dts * (mexp(R(0.66) * tc) - R(1.0)) * den * R(s.ql) * R(s.ql);
-      double sk = mmin(s.ql, mmin(tcd / icpk, double(sink)));
+      double sk = mmin(s.ql, mmin(fdiv<R>(tcd, icpk), double(sink)));
       s.ql = s.ql - sk;
       s.qi = s.qi + sk;
       q_liq = q_liq - sk;
       q_sol = q_sol + sk;
       cvm = SH_CVM(c, s, q_liq, q_sol);
-      s.tz = s.tz + sk * lhi / cvm;
+      s.tz = s.tz + fdiv<R>(sk * lhi, cvm);
     }
   }
   lhl = c.lv00_d + c.d0_vap_d * s.tz;
   lhi = LI00 + DC_ICE * s.tz;
-  lcpk = lhl / cvm;
-  icpk = lhi / cvm;
+  lcpk = fdiv<R>(lhl, cvm);
+  icpk = fdiv<R>(lhi, cvm);
   tcpk = lcpk + icpk;
 
   // sublimation / deposition of cloud ice
   if (s.tz < tice_d) {
     double dqsdt;
-    const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
+    const double qsi = qs_i<R>(s.tz, den_d, dqsdt);
     const double dq = s.qv - qsi;
-    const double sink = dq / (1.0 + tcpk * dqsdt);
+    const double sink = fdiv<R>(dq, 1.0 + tcpk * dqsdt);
     R pidep;
     if (s.qi > QRMIN) {
       pidep = dt_pisub * R(dq) * R(349138.78) * pw_el<R>(R(s.qi) * den, 0.875) /
@@ -378,7 +430,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
       const double tmpd = tice_d - s.tz;
       const R tmp = R(tmpd);
       R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * tmp) / den;
-      sk = mmin(sink, mmin(double(mmax(R(double(qi_crt) - s.qi), pidep)), tmpd / tcpk));
+      sk = mmin(sink, mmin(double(mmax(R(double(qi_crt) - s.qi), pidep)), fdiv<R>(tmpd, tcpk)));
       sk = cap<R>(sk, s.qv);
     } else {
       pidep = pidep * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.2));
@@ -388,24 +440,24 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     s.qi = s.qi + sk;
     q_sol = q_sol + sk;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + sk * (lhl + lhi) / cvm;
+    s.tz = s.tz + fdiv<R>(sk * (lhl + lhi), cvm);
   }
   lhl = c.lv00_d + c.d0_vap_d * s.tz;
   lhi = LI00 + DC_ICE * s.tz;
-  lcpk = lhl / cvm;
-  icpk = lhi / cvm;
+  lcpk = fdiv<R>(lhl, cvm);
+  icpk = fdiv<R>(lhi, cvm);
   tcpk = lcpk + icpk;
 
   // sublimation / deposition of snow
   if (s.qs > QRMIN) {
     double dqsdt;
-    const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
+    const double qsi = qs_i<R>(s.tz, den_d, dqsdt);
     R qden = R(s.qs) * den;
     R tmp = pw_el<R>(qden, 0.65625);
     R tsq = R(s.tz) * R(s.tz);
     const double dsat_d = qsi - s.qv;
     const R dsat = R(dsat_d);
-    const double dq = dsat_d / (1.0 + tcpk * dqsdt);
+    const double dq = fdiv<R>(dsat_d, 1.0 + tcpk * dqsdt);
     R pssub = ck(c, T_CSSUB + 0) * tsq * (ck(c, T_CSSUB + 1) * msqrt(qden) + ck(c, T_CSSUB + 2) * tmp * msqrt(denfac)) /
               (ck(c, T_CSSUB + 3) * tsq + ck(c, T_CSSUB + 4) * R(qsi) * den);
     pssub = dsat * dts * pssub;
@@ -416,32 +468,32 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
       if (s.tz > tice_d)
         ps = 0.0;
       else
-        ps = cap<R>(-mmax(double(pssub), mmax(dq, (s.tz - tice_d) / tcpk)), s.qv) * -1.0;
+        ps = cap<R>(-mmax(double(pssub), mmax(dq, fdiv<R>(s.tz - tice_d, tcpk))), s.qv) * -1.0;
     }
     s.qs = s.qs - ps;
     s.qv = s.qv + ps;
     q_sol = q_sol - ps;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz - ps * (lhl + lhi) / cvm;
+    s.tz = s.tz - fdiv<R>(ps * (lhl + lhi), cvm);
   }
   lhl = c.lv00_d + c.d0_vap_d * s.tz;
   lhi = LI00 + DC_ICE * s.tz;
-  lcpk = lhl / cvm;
-  icpk = lhi / cvm;
+  lcpk = fdiv<R>(lhl, cvm);
+  icpk = fdiv<R>(lhi, cvm);
   tcpk = lcpk + icpk;
 
   // two-way graupel sublimation / deposition
   if (s.qg > QRMIN) {
     double dqsdt;
-    const double qsi = iqs2<double>(s.tz, den_d, dqsdt);
-    const double dq = (s.qv - qsi) / (1.0 + tcpk * dqsdt);
+    const double qsi = qs_i<R>(s.tz, den_d, dqsdt);
+    const double dq = fdiv<R>(s.qv - qsi, 1.0 + tcpk * dqsdt);
     R pgsub = R(s.qv / qsi - 1.0) * R(s.qg);
     double pg;
     if (pgsub > R(0.0)) {
       if (s.tz > tice_d)
         pg = 0.0;
       else
-        pg = cap<R>(mmin(mmin(double(c.fac_v2g * pgsub), 0.2 * dq), mmin(s.ql + s.qr, (tice_d - s.tz) / tcpk)),
+        pg = cap<R>(mmin(mmin(double(c.fac_v2g * pgsub), 0.2 * dq), mmin(s.ql + s.qr, fdiv<R>(tice_d - s.tz, tcpk))),
                     s.qv);
     } else {
       pg = -cap<R>(-(mmax(double(c.fac_g2v * pgsub), dq) * double(mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.1)))),
@@ -451,25 +503,25 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     s.qv = s.qv - pg;
     q_sol = q_sol + pg;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + pg * (lhl + lhi) / cvm;
+    s.tz = s.tz + fdiv<R>(pg * (lhl + lhi), cvm);
   }
 
   // minimum evaporation of rain in dry air
   lhl = c.lv00_d + c.d0_vap_d * s.tz;
-  lcpk = lhl / cvm;
+  lcpk = fdiv<R>(lhl, cvm);
   if (s.qr > QCMIN) {
     double dqsdt;
-    const double qsw2 = wqs2<double>(s.tz, den_d, dqsdt);
-    double sink = mmin(s.qr, dim(rh_rain * qsw2, s.qv) / (1.0 + lcpk * dqsdt));
+    const double qsw2 = qs_w<R>(s.tz, den_d, dqsdt);
+    double sink = mmin(s.qr, fdiv<R>(dim(rh_rain * qsw2, s.qv), 1.0 + lcpk * dqsdt));
     s.qv = s.qv + sink;
     s.qr = s.qr - sink;
     q_liq = q_liq - sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz - sink * lhl / cvm;
+    s.tz = s.tz - fdiv<R>(sink * lhl, cvm);
   }
   lhl = c.lv00_d + c.d0_vap_d * s.tz;
   cvm = c.c_air_d + (s.qv + q_liq + q_sol) * c.c_vap_d;
-  lcpk = lhl / cvm;
+  lcpk = fdiv<R>(lhl, cvm);
 
   // cloud fraction
   if (!c.do_qa) {
@@ -480,12 +532,12 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     const double tin2 = s.tz - (lcpk * q_cond + icpk * q_sol2);
     double qstar;
     if (tin2 <= c.t_wfr_d) {
-      qstar = iqs1<double>(tin2, den_d);
+      qstar = qs_i1<R>(tin2, den_d);
     } else if (tin2 >= tice_d) {
-      qstar = wqs1<double>(tin2, den_d);
+      qstar = qs_w1<R>(tin2, den_d);
     } else {
-      const double qsi = iqs1<double>(tin2, den_d);
-      const double qsw3 = wqs1<double>(tin2, den_d);
+      const double qsi = qs_i1<R>(tin2, den_d);
+      const double qsw3 = qs_w1<R>(tin2, den_d);
       double rqi;
       if (q_cond > 3.0e-6)
         rqi = q_sol2 / q_cond;
@@ -516,9 +568,9 @@ __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice
   q_liq = s.ql + s.qr;
   q_sol = s.qi + s.qs + s.qg;
   cvm = SH_CVM(c, s, q_liq, q_sol);
-  const double icpk = lhi / cvm;
+  const double icpk = fdiv<R>(lhi, cvm);
   if ((s.tz > tice_d) && (s.qi > QCMIN)) {
-    const double melt = mmin(s.qi, double(c.fac_imlt) * (s.tz - tice_d) / icpk);
+    const double melt = mmin(s.qi, fdiv<R>(double(c.fac_imlt) * (s.tz - tice_d), icpk));
     const double tmp = mmin(melt, dim(ql_mlt, s.ql));
     s.ql = s.ql + tmp;
     s.qr = s.qr + melt - tmp;
@@ -526,11 +578,11 @@ __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice
     q_liq = q_liq + melt;
     q_sol = q_sol - melt;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz - melt * lhi / cvm;
+    s.tz = s.tz - fdiv<R>(melt * lhi, cvm);
   } else if ((s.tz < t_wfr_d) && (s.ql > QCMIN)) {
     const double dtmp = t_wfr_d - s.tz;
     const R factor = mmin(R(1.0), R(dtmp) / R(DT_FR));
-    const double sink = mmin(s.ql * double(factor), dtmp / icpk);
+    const double sink = mmin(s.ql * double(factor), fdiv<R>(dtmp, icpk));
     const R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * R(tice_d - s.tz)) / R(den);
     const double tmp = mmin(sink, dim(double(qi_crt), s.qi));
     s.ql = s.ql - sink;
@@ -539,7 +591,7 @@ __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice
     q_liq = q_liq - sink;
     q_sol = q_sol + sink;
     cvm = SH_CVM(c, s, q_liq, q_sol);
-    s.tz = s.tz + sink * lhi / cvm;
+    s.tz = s.tz + fdiv<R>(sink * lhi, cvm);
   }
 }
 
@@ -551,7 +603,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
   const R dts = c.dts, rdts = c.rdts;
   const R den = R(den_d);
   double lhi = LI00 + DC_ICE * s.tz;
-  double icpk = lhi / cvm;
+  double icpk = fdiv<R>(lhi, cvm);
   if (p1d >= P_MIN) {
     R pgacr = R(0.0), pgacw = R(0.0);
     R tc = R(s.tz - tice_d);
@@ -574,7 +626,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           pracs = R(0.0);
         }
         R psmlt = mmax(R(0.0), smlt(c, tc, dqs0, R(s.qs) * den, psacw, psacr, den, denfac));
-        double sink = mmin(s.qs, mmin(double(dts * (psmlt + pracs)), (s.tz - tice_d) / icpk));
+        double sink = mmin(s.qs, mmin(double(dts * (psmlt + pracs)), fdiv<R>(s.tz - tice_d, icpk)));
         s.qs = s.qs - sink;
         double tmp = mmin(sink, double(dim(c.qs_mlt, R(s.ql))));
         s.ql = s.ql + tmp;
@@ -582,11 +634,11 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         q_liq = q_liq + sink;
         q_sol = q_sol - sink;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz - sink * lhi / cvm;
+        s.tz = s.tz - fdiv<R>(sink * lhi, cvm);
         tc = R(s.tz - tice_d);
       }
       lhi = LI00 + DC_ICE * s.tz;
-      icpk = lhi / cvm;
+      icpk = fdiv<R>(lhi, cvm);
       if ((s.qg > QCMIN) && (tc > R(0.0))) {
         if (s.qr > QRMIN)
           pgacr = mmin(acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, ck(c, T_ACCO + 2), ck(c, T_ACCO + 6), ck(c, T_ACCO + 10), den),
@@ -597,13 +649,13 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           pgacw = factor / (R(1.0) + dts * factor) * R(s.ql);
         }
         R pgmlt = dts * gmlt(c, tc, dqs0, qden, pgacw, pgacr, den);
-        double pm = mmin(double(mmax(R(0.0), pgmlt)), mmin(s.qg, (s.tz - tice_d) / icpk));
+        double pm = mmin(double(mmax(R(0.0), pgmlt)), mmin(s.qg, fdiv<R>(s.tz - tice_d, icpk)));
         s.qg = s.qg - pm;
         s.qr = s.qr + pm;
         q_liq = q_liq + pm;
         q_sol = q_sol - pm;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz - pm * lhi / cvm;
+        s.tz = s.tz - fdiv<R>(pm * lhi, cvm);
       }
     } else {
       if (s.qi > 3.0e-7) {
@@ -656,7 +708,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         // the limiter that can drain all rain is applied in FP64, so a fully
         // frozen layer keeps no FP32 residue
         const double sink = double(psacr) + double(pgfr);
-        const double factor = mmin(sink, mmin(s.qr, -(s.tz - tice_d) / icpk)) / mmax(sink, QRMIN);
+        const double factor = mmin(sink, mmin(s.qr, fdiv<R>(-(s.tz - tice_d), icpk))) / mmax(sink, QRMIN);
         double ps = factor * double(psacr), pg = factor * double(pgfr);
         if constexpr (is_f32<R>::value) {
           double tot = ps + pg;
@@ -673,10 +725,10 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         q_liq = q_liq - sk;
         q_sol = q_sol + sk;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz + sk * lhi / cvm;
+        s.tz = s.tz + fdiv<R>(sk * lhi, cvm);
       }
       lhi = LI00 + DC_ICE * s.tz;
-      icpk = lhi / cvm;
+      icpk = fdiv<R>(lhi, cvm);
       if (s.qs > 1.0e-7) {
         R sink;
         if (s.qg > QRMIN)
@@ -706,7 +758,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         else
           pgacr = R(0.0);
         const double sink = double(pgacr) + double(pgacw);
-        const double factor = mmin(sink, dim(tice_d, s.tz) / icpk) / mmax(sink, QRMIN);
+        const double factor = mmin(sink, fdiv<R>(dim(tice_d, s.tz), icpk)) / mmax(sink, QRMIN);
         const double pr = cap<R>(factor * double(pgacr), s.qr), pl = cap<R>(factor * double(pgacw), s.ql);
         double sk = pr + pl;
         s.qg = s.qg + sk;
@@ -715,7 +767,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         q_liq = q_liq - sk;
         q_sol = q_sol + sk;
         cvm = SH_CVM(c, s, q_liq, q_sol);
-        s.tz = s.tz + sk * lhi / cvm;
+        s.tz = s.tz + fdiv<R>(sk * lhi, cvm);
       }
     }
   }

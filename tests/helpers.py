@@ -93,6 +93,19 @@ def cold_top_state(qv_top=1.0e-8, t_top=170.0):
     return st
 
 
+def overdraw_state(ni=4, nj=6, nz=79, seed=515):
+    """Warm layers (>= 280 K) holding 1e-2 .. 3e-2 kg/kg of cloud water in half
+    of the columns.  There the subgrid autoconversion of one rain step
+    (dt c_praut den ql^(7/3)) exceeds ql, which the reference does not bound:
+    ql and, after the saturation adjustment, qv turn negative and the layers
+    warm by up to ~180 K."""
+    st = make_state(ni, nj, nz, seed)
+    hot = (slice(0, ni // 2), slice(None), slice(52, 60))
+    st["ql"][hot] = np.linspace(1.0e-2, 3.0e-2, 8)
+    st["pt"][hot] = np.maximum(st["pt"][hot], 280.0)
+    return st
+
+
 def low_pressure_top_state(thin=True):
     """A dry column whose top layer is supersaturated over ice at 200 K; with
     thin=True its dry pressure p1 is below P_MIN (100 Pa)."""

@@ -1,0 +1,113 @@
+"""The benchmark configurations at their full sizes: the full cubed-sphere C96
+(6 x 96 x 96 columns x 79 levels, tile-dependent temperature offsets, dt 225 s,
+two sub-steps) and the deep tile (768 x 768 columns x 127 levels, dt 90 s).
+
+Columns are independent, so the whole-domain GPU run is checked against the
+oracle on a random sample of its columns (FP64 to round-off, mixed within the
+field-wise 1e-5), and through size-independent properties over every column:
+total-water conservation in both precisions and agreement of the mixed run
+with the FP64 run (1e-3 K at every cell, the vapour change within 1e-5 of its
+scale)."""
+
+import numpy as np
+import pytest
+
+from pyshield_amd._config import PhysicsConfig
+from pyshield_amd.synthetic import WORKLOAD_IDS, WORKLOADS, make_state
+
+from helpers import PRECIP, TEND, WATER, norm_err, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+NSAMPLE = 384
+MIX_T_TOL = 1e-3        # K, every cell
+
+
+def _load(name):
+    """The workload's state on cuda:0, tile by tile, and the host copy of the
+    sampled columns (flat column indices `cols`)."""
+    import torch
+    w = WORKLOADS[name]
+    dev = torch.device("cuda:0")
+    ni, nj, nz = w.ni * w.ntiles, w.nj, w.nz
+    names = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz", "w", "land", "area")
+    f = {n: torch.empty((ni, nj, nz) if n not in ("land", "area") else (ni, nj), dtype=torch.float64,
+                        device=dev) for n in names}
+    rng = np.random.default_rng(7)
+    cols = np.sort(rng.choice(ni * nj, NSAMPLE, replace=False))
+    sample = {n: np.empty((NSAMPLE, 1, nz) if n not in ("land", "area") else (NSAMPLE, 1)) for n in names}
+    for t in range(w.ntiles):
+        off = (-10.0 + 4.0 * t) if w.tile_offsets else 0.0
+        st = make_state(w.ni, w.nj, nz, 1000 * WORKLOAD_IDS[name] + t, ptop=w.ptop, t_offset=off,
+                        cloud_top=w.cloud_top, median=w.median, qg_median=w.qg_median)
+        sl = slice(t * w.ni, (t + 1) * w.ni)
+        inside = (cols >= t * w.ni * nj) & (cols < (t + 1) * w.ni * nj)
+        loc = cols[inside] - t * w.ni * nj
+        for n in names:
+            f[n][sl].copy_(torch.from_numpy(st[n]))
+            sample[n][inside, 0] = st[n].reshape((w.ni * nj,) + st[n].shape[2:])[loc]
+        del st
+    for n in TEND + ("qaz",):
+        f[n] = torch.zeros((ni, nj, nz), dtype=torch.float64, device=dev)
+    for n in PRECIP:
+        f[n] = torch.zeros((ni, nj), dtype=torch.float64, device=dev)
+    return w, f, cols, sample
+
+
+def _column_water_imbalance(f, dt):
+    """(sum_k delp sum_species q_dt dt + surface precipitation in Pa) / column
+    water, per column, on the GPU."""
+    wat = sum(f[f"{q}_dt"] for q in WATER)
+    col = (f["delp"] * wat * dt).sum(-1)
+    pr = sum(f[p] for p in PRECIP) * (dt * 9.80665 / 86400.0)
+    tot = (f["delp"] * sum(f[q] for q in WATER)).sum(-1)
+    return ((col + pr) / tot).abs()
+
+
+@pytest.mark.parametrize("name", ["c96", "deep768"])
+def test_full_configuration(name):
+    import torch
+    from pyshield_amd import _lib
+    from pyshield_amd.microphysics import forward
+    print("config", name, flush=True)
+    w, f, cols, sample = _load(name)
+    cfg = PhysicsConfig(dt_atmos=int(w.timestep), npz=w.nz)
+    params = _lib.setup(cfg.mp_namelist(), w.timestep)
+    params.accumulate = 0
+    print(name, "ntimes", params.ntimes, "layout fp64", _lib.layout(w.nz, "fp64", params.ntimes),
+          "mixed", _lib.layout(w.nz, "mixed", params.ntimes), flush=True)
+    outs = {}
+    for mode in ("fp64", "mixed"):
+        f["qaz"].zero_()  # the accumulator persists between calls
+        forward(f, params, mode)
+        torch.cuda.synchronize()
+        imb = _column_water_imbalance(f, w.timestep).max().item()
+        print(name, mode, "max column water imbalance", imb, flush=True)
+        assert imb <= 1e-12
+        outs[mode] = {n: f[n].reshape((-1,) + f[n].shape[2:])[torch.as_tensor(cols, device=f[n].device)].cpu()
+                      .numpy()[:, None] for n in TEND + PRECIP}
+        outs[mode]["all_pt_dt"] = f["pt_dt"].clone()
+        outs[mode]["all_qv_dt"] = f["qv_dt"].clone()
+    # mixed against FP64 over every cell: the temperature and vapour changes of
+    # the call
+    dT = ((outs["mixed"]["all_pt_dt"] - outs["fp64"]["all_pt_dt"]) * w.timestep).abs().flatten()
+    n = dT.numel()
+    big = int((dT > MIX_T_TOL).sum().item())
+    q = torch.quantile(dT[torch.randperm(n, device=dT.device)[:2 ** 24]], torch.tensor(
+        [0.5, 0.999, 0.99999], dtype=dT.dtype, device=dT.device)).tolist()
+    print(name, f"mixed vs fp64 |dT| over {n} cells: max {dT.max().item():.3e} K, median {q[0]:.1e}, "
+          f"99.9% {q[1]:.1e}, 99.999% {q[2]:.1e}; {big} cells above {MIX_T_TOL} K", flush=True)
+    dq = (outs["mixed"]["all_qv_dt"] - outs["fp64"]["all_qv_dt"]).abs().max() * w.timestep
+    qscale = outs["fp64"]["all_qv_dt"].abs().max() * w.timestep
+    print(name, f"qv change: max difference {(dq / qscale).item():.1e} of the field scale", flush=True)
+    assert big == 0
+    assert dq <= 1e-5 * qscale
+    ora = run_oracle(sample, cfg, w.timestep)
+    errs = {n: norm_err(outs["fp64"][n], ora[n], floor=1e-30) for n in TEND}
+    print(name, "fp64 vs oracle on", NSAMPLE, "columns", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert all(v <= 1e-9 for v in errs.values()), errs
+    # the cloud-fraction tendency steps by 1/ntimes where a cell crosses the
+    # cloud threshold, so mixed precision is held to it only through T and water
+    errs = {n: norm_err(outs["mixed"][n], ora[n], floor=1e-30) for n in TEND if n != "qa_dt"}
+    print(name, "mixed vs oracle", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert all(v <= 1e-5 for v in errs.values()), errs

@@ -151,7 +151,7 @@ def test_deterministic_and_column_independent():
 
 
 from test_oracle import NAMELISTS  # noqa: E402
-from helpers import cold_top_state, dry_state, low_pressure_top_state  # noqa: E402
+from helpers import cold_top_state, dry_state, low_pressure_top_state, overdraw_state  # noqa: E402
 
 
 @pytest.mark.parametrize("overrides", NAMELISTS)
@@ -187,6 +187,26 @@ def test_special_columns(mode):
     assert thin["qv_dt"][..., 0] == 0.0
     thick = run_gpu(low_pressure_top_state(False), cfg, 150.0, mode)
     assert thick["qv_dt"][..., 0] < 0.0
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("dt", [150.0, 225.0])
+def test_overdrawn_autoconversion(mode, dt):
+    """Cells where one rain step's autoconversion exceeds the cloud water: the
+    reference leaves ql, then qv, negative and both modes must follow it."""
+    st = overdraw_state()
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79)
+    gpu, ora = run_gpu(st, cfg, dt, mode), run_oracle(st, cfg, dt)
+    no, ng = updated_state(st, ora, dt), updated_state(st, gpu, dt)
+    assert (no["qv"] < 0.0).sum() > 10 and (no["ql"] < 0.0).sum() > 10
+    if mode == "fp64":
+        _assert_fp64(gpu, ora)
+        return
+    terr = float(np.max(np.abs(ng["pt"] - no["pt"])))
+    qerr = {q: float(np.max(np.abs(ng[q] - no[q])) / np.abs(no[q]).max()) for q in WATER}
+    print("T err", terr, {k: f"{v:.1e}" for k, v in qerr.items()}, flush=True)
+    assert terr <= MIX_T_TOL
+    assert all(v <= MIX_Q_RTOL for v in qerr.values()), qerr
 
 
 def test_halo_strided_views():
