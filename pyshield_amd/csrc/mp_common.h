@@ -74,7 +74,8 @@ __device__ __forceinline__ double mpow(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ double mabs(double x) { return fabs(x); }
 __device__ __forceinline__ float mexp(float x) { return __expf(x); }
 __device__ __forceinline__ float mlog(float x) { return __logf(x); }
-__device__ __forceinline__ float msqrt(float x) { return __fsqrt_rn(x); }
+// v_sqrt_f32 (1 ulp); the FP32 rate laws need no correctly rounded root
+__device__ __forceinline__ float msqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float mpow(float x, float y) { return __expf(y * __logf(x)); }
 __device__ __forceinline__ float mabs(float x) { return fabsf(x); }
 // Powers x^p (x > 0, p a literal).  pw_el is the reference's exp(p*log(x)),
@@ -122,6 +123,59 @@ __device__ __forceinline__ double fdiv(double a, double b) {
     return a * rcp64(b);
   else
     return a / b;
+}
+
+// exp and log for the FP64 saturation functions of the mixed-precision kernel,
+// whose arguments are moderate (the exponent of e_sat(T) has |x| < 50, the log
+// argument T / tice is positive and normal) and which need ~1e-14, not the
+// last ulp.
+//  * exp: Cody-Waite reduction x = n ln2 + r, |r| <= ln2 / 2, with ln2 split
+//    into a 32-bit head and its remainder so n ln2 is exact, then the Taylor
+//    series of exp(r) to r^11 (truncation < 7e-15 relative); no overflow or
+//    denormal handling (|x| < 700).
+//  * log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+//    s = (m - 1) / (m + 1), |s| <= 0.172, summed to s^21 (truncation < 3e-17);
+//    x must be positive and normal.
+__device__ __forceinline__ double exp_sat(double x) {
+  constexpr double LOG2E = 1.4426950408889634, LN2_HI = 6.93147180369123816490e-01,
+                   LN2_LO = 1.90821492927058770002e-10;
+  const double n = __builtin_rint(x * LOG2E);
+  const double r = fma(-n, LN2_LO, fma(-n, LN2_HI, x));
+  double p = 1.0 / 39916800.0;  // 1/11!
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return __builtin_ldexp(p, int(n));
+}
+__device__ __forceinline__ double log_sat(double x) {
+  constexpr double LN2 = 6.93147180559945286227e-01, RSQRT2 = 0.70710678118654752440;
+  int e;
+  double m = __builtin_frexp(x, &e);  // [0.5, 1)
+  const bool lo = m < RSQRT2;
+  m = lo ? m + m : m;
+  e = lo ? e - 1 : e;
+  const double s = (m - 1.0) * rcp64(m + 1.0);
+  const double z = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, z, 1.0 / 19.0);
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  const double s2 = s + s;
+  return fma(double(e), LN2, fma(s2 * z, p, s2));
 }
 
 template <class T>

@@ -111,14 +111,15 @@ __device__ __forceinline__ R iqs2(R ta, R den, R& dqdt) {
 
 // Saturation mixing ratio and gradient in the mixed-precision kernel: the
 // closed forms above, evaluated in FP64 with the divisions by constants
-// folded into products and the others replaced by reciprocals
-// ((ta - TICE) / (ta TICE) = 1/TICE - 1/ta).  Below T_SAT_MIN and above
+// folded into products, the others replaced by reciprocals
+// ((ta - TICE) / (ta TICE) = 1/TICE - 1/ta), and the series exp and log of
+// mp_common.h (~1e-14 relative).  Below T_SAT_MIN and above
 // TICE + 102 the reference's branches are the same expression at the clamped
 // temperature.
 __device__ __forceinline__ double qs_fast(double t, double d, double l, double den, double& dqdt) {
   constexpr double rtice = 1.0 / TICE, rrv = 1.0 / RVGAS;
   const double rt = rcp64(t);
-  const double e = E00 * exp(d * rrv * log(t * rtice) + l * rrv * (rtice - rt));
+  const double e = E00 * exp_sat(d * rrv * log_sat(t * rtice) + l * rrv * (rtice - rt));
   const double q = e * rrv * rt * rcp64(den);
   dqdt = q * (d + l * rt) * rrv * rt;
   return q;
