@@ -688,8 +688,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
         ccn = ccn * RDGAS * X.tz[s] / p1;
       }
       sh.colv[CV_CCN][cl] = ccn;
-      sh.colv[CV_CPRAUT][cl] = P.cpaut * pow(ccn * RHOR, -1.0 / 3.0);
     }
+  }
+  __syncthreads();
+  if (lane < ncols) {
+    const double x = sh.colv[CV_CCN][lane] * RHOR;
+    if constexpr (sizeof(R) == 4)
+      sh.colv[CV_CPRAUT][lane] = P.cpaut * exp_sat(log_sat(x) * (-1.0 / 3.0));
+    else
+      sh.colv[CV_CPRAUT][lane] = P.cpaut * pow(x, -1.0 / 3.0);
   }
   __syncthreads();
   if (n.fix_negative) {
