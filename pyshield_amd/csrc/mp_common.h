@@ -123,6 +123,13 @@ __device__ __forceinline__ double fdiv(double a, double b) {
   else
     return a / b;
 }
+template <class R>
+__device__ __forceinline__ double recip(double b) {
+  if constexpr (sizeof(R) == 4)
+    return rcp64(b);
+  else
+    return 1.0 / b;
+}
 
 // exp and log for the FP64 saturation functions of the mixed-precision kernel,
 // whose arguments are moderate (the exponent of e_sat(T) has |x| < 50, the log

@@ -77,7 +77,8 @@ struct Glob {
 };
 
 // den0 = -dp1 / (g dz0) (microphysics.py:149), recomputed where needed
-__device__ __forceinline__ double dry_density(double dp1, double dz0) { return -dp1 / (GRAV * dz0); }
+template <class R>
+__device__ __forceinline__ double dry_density(double dp1, double dz0) { return fdiv<R>(-dp1, GRAV * dz0); }
 
 // Layer-bottom heights ze (BACKWARD, microphysics.py:373-380 / :756-765) and
 // layer thicknesses dz = ze[k] - ze[k+1], dz[nz-1] = ze - zs (:445-452).
@@ -116,7 +117,7 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X
   col_prev(sh.buf, dd, ddp);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const double r = 1.0 / (dz[s] + dd[s]);
+    const double r = recip<R>(dz[s] + dd[s]);
     a[s] = ddp[s] * r;
     b[s] = qq[s] * r;
   }
@@ -145,15 +146,15 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X
       const int32_t o = G.off[s];
       const double w = f.w[o];
       if (L.k[s] == 0)
-        f.w[o] = (dm * w + flux[s] * vd[s]) / (dm - flux[s]);
+        f.w[o] = fdiv<R>(dm * w + flux[s] * vd[s], dm - flux[s]);
       else
-        f.w[o] = (dm * w - fp[s] * vp[s] + flux[s] * vd[s]) / (dm + fp[s] - flux[s]);
+        f.w[o] = fdiv<R>(dm * w - fp[s] * vp[s] + flux[s] * vd[s], dm + fp[s] - flux[s]);
     }
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (fall[s]) {
-      q[s] = qm[s] / X.dp1[s];
+      q[s] = fdiv<R>(qm[s], X.dp1[s]);
     } else {
       flux[s] = 0.0;
       surf[s] = 0.0;
@@ -176,9 +177,9 @@ __device__ __forceinline__ void sedi_heat(const Lay<S>& L, St<S, R>& X, Shared<S
                                    (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE);
     if (L.k[s] == 0) {
       a[s] = 0.0;
-      b[s] = X.tz[s] + flux[s] * dgz / (cvn + flux[s] * cq);
+      b[s] = X.tz[s] + fdiv<R>(flux[s] * dgz, cvn + flux[s] * cq);
     } else {
-      const double r = 1.0 / (cvn + cq * flux[s]);
+      const double r = recip<R>(cvn + cq * flux[s]);
       a[s] = fp[s] * cq * r;
       b[s] = ((cvn + cq * (flux[s] - fp[s])) * X.tz[s] + dgz * (fp[s] + flux[s])) * r;
     }
@@ -230,10 +231,15 @@ __device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S
     for (int s = 0; s < S; ++s) {
       const double t0 = L.v[s] ? f.pt[G.off[s]] : 250.0;
       const double dz0 = L.v[s] ? f.delz[G.off[s]] : -100.0;
-      const double dz1 = dz0 * X.tz[s] / t0;
+      const double dz1 = fdiv<R>(dz0 * X.tz[s], t0);
       sh.cell[CL_DZ1][64 * s + lane] = dz1;
-      sh.at(CL_DEN, s) = dry_density(X.dp1[s], dz0) * dz0 / dz1;
-      sh.at(CL_DENFAC, s) = sqrt(SFCRHO / sh.at(CL_DEN, s));
+      const double den = fdiv<R>(dry_density<R>(X.dp1[s], dz0) * dz0, dz1);
+      sh.at(CL_DEN, s) = den;
+      // denfac only enters the rate laws, so mixed mode takes it in FP32
+      if constexpr (sizeof(R) == 4)
+        sh.at(CL_DENFAC, s) = double(msqrt(R(SFCRHO) / R(den)));
+      else
+        sh.at(CL_DENFAC, s) = sqrt(SFCRHO / den);
     }
   }
   bool pred[S], fall[S];
@@ -343,8 +349,8 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
       double q_liq = X.ql[s] + X.qr[s];
       double q_sol = X.qi[s] + X.qs[s] + X.qg[s];
       double cvm = c.c_air_d + X.qv[s] * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
-      const double icpk = lhi / cvm;
-      const double sink = mmin(X.qi[s], double(c.fac_imlt) * tcd / icpk);
+      const double icpk = fdiv<R>(lhi, cvm);
+      const double sink = mmin(X.qi[s], fdiv<R>(double(c.fac_imlt) * tcd, icpk));
       const double tmp = mmin(sink, dim(double(P.nml.ql_mlt), X.ql[s]));
       X.ql[s] = X.ql[s] + tmp;
       X.qr[s] = X.qr[s] + sink - tmp;
@@ -352,7 +358,7 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
       q_liq = q_liq + sink;
       q_sol = q_sol - sink;
       cvm = c.c_air_d + X.qv[s] * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
-      X.tz[s] = X.tz[s] - sink * lhi / cvm;
+      X.tz[s] = X.tz[s] - fdiv<R>(sink * lhi, cvm);
     }
   }
   double dz[S], flux[S], surf[S], m1_sol[S];
@@ -458,7 +464,7 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
     const bool v = L.v[s];
     const double t0 = v ? f.pt[o] : 250.0;
     const double dz0 = v ? f.delz[o] : -100.0;
-    const double p1 = dry_density(X.dp1[s], dz0) * RDGAS * t0;
+    const double p1 = dry_density<R>(X.dp1[s], dz0) * RDGAS * t0;
     icloud_main<R>(c, tice_d, P.tice0, cs, R(hv), sh.colv[CV_RHADJ][cl], sh.colv[CV_RHRAIN][cl], den,
                    R(sh.at(CL_DENFAC, s)), X.vtg[s], X.vtr[s], X.vts[s], p1, R(di), q_liq, q_sol,
                    cvm);
@@ -659,7 +665,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
       const double dp0 = f.delp[o];
       const double qv = f.qv[o];
       X.dp1[s] = dp0 * (1.0 - qv);
-      const double omq = dp0 / X.dp1[s];
+      const double omq = fdiv<R>(dp0, X.dp1[s]);
       // the products are materialised exactly (launder) because
       // fields_update forms the same products again and subtracts
       X.qv[s] = launder(qv * omq);
@@ -684,8 +690,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
       double ccn = sh.colv[CV_CCN][cl];
       if (P.use_ccn) {
         const int32_t o = G.off[s];
-        const double p1 = dry_density(X.dp1[s], f.delz[o]) * RDGAS * f.pt[o];
-        ccn = ccn * RDGAS * X.tz[s] / p1;
+        const double p1 = dry_density<R>(X.dp1[s], f.delz[o]) * RDGAS * f.pt[o];
+        ccn = fdiv<R>(ccn * RDGAS * X.tz[s], p1);
       }
       sh.colv[CV_CCN][cl] = ccn;
     }
@@ -702,10 +708,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
   if (n.fix_negative) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+      // lcpk and icpk of the entry state, formed only where a branch uses them
       const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
                          (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
-      const double lcpk = (P.lv00 + P.d0_vap * X.tz[s]) / cvm;
-      const double icpk = (LI00 + DC_ICE * X.tz[s]) / cvm;
+      const double tz0 = X.tz[s];
       if (X.qi[s] < 0.0) {
         X.qs[s] = X.qs[s] + X.qi[s];
         X.qi[s] = 0.0;
@@ -715,6 +721,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
         X.qs[s] = 0.0;
       }
       if (X.qg[s] < 0.0) {
+        const double icpk = fdiv<R>(LI00 + DC_ICE * tz0, cvm);
         X.qr[s] = X.qr[s] + X.qg[s];
         X.tz[s] = X.tz[s] - X.qg[s] * icpk;
         X.qg[s] = 0.0;
@@ -724,37 +731,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
         X.qr[s] = 0.0;
       }
       if (X.ql[s] < 0.0) {
+        const double lcpk = fdiv<R>(P.lv00 + P.d0_vap * tz0, cvm);
         X.qv[s] = X.qv[s] + X.ql[s];
         X.tz[s] = X.tz[s] - X.ql[s] * lcpk;
         X.ql[s] = 0.0;
       }
     }
     // vapour borrowing is a non-linear recurrence: one lane per column walks
-    // its column in LDS (:244-276); dp1 is staged where m1 lives later
+    // its column in LDS (:244-276); dp1 is staged where m1 lives later.  It
+    // changes nothing unless some cell holds negative vapour.
+    bool neg = false;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      sh.buf[64 * s + lane] = X.qv[s];
-      sh.cell[CL_M1][64 * s + lane] = X.dp1[s];
-    }
-    __syncthreads();
-    if (lane < ncols) {
-      double* q = sh.buf + lane * nz;
-      const double* dp = sh.cell[CL_M1] + lane * nz;
-      for (int k = 1; k < nz; ++k)
-        if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * dp[k - 1] / dp[k];
-      for (int k = 0; k < nz - 1; ++k)
-        if (q[k] < 0.0) q[k] = 0.0;
-      const int kb = nz - 1, ka = nz - 2;
-      if ((q[kb] < 0.0) && (q[ka] > 0.0)) {
-        const double dq = mmin(-q[kb] * dp[kb], q[ka] * dp[ka]);
-        q[ka] = q[ka] - dq / dp[ka];
-        q[kb] = q[kb] + dq / dp[kb];
+    for (int s = 0; s < S; ++s) neg = neg || (L.v[s] && X.qv[s] < 0.0);
+    if (__ballot(neg) != 0ull) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        sh.buf[64 * s + lane] = X.qv[s];
+        sh.cell[CL_M1][64 * s + lane] = X.dp1[s];
       }
-    }
-    __syncthreads();
+      __syncthreads();
+      if (lane < ncols) {
+        double* q = sh.buf + lane * nz;
+        const double* dp = sh.cell[CL_M1] + lane * nz;
+        for (int k = 1; k < nz; ++k)
+          if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * dp[k - 1] / dp[k];
+        for (int k = 0; k < nz - 1; ++k)
+          if (q[k] < 0.0) q[k] = 0.0;
+        const int kb = nz - 1, ka = nz - 2;
+        if ((q[kb] < 0.0) && (q[ka] > 0.0)) {
+          const double dq = mmin(-q[kb] * dp[kb], q[ka] * dp[ka]);
+          q[ka] = q[ka] - dq / dp[ka];
+          q[kb] = q[kb] + dq / dp[kb];
+        }
+      }
+      __syncthreads();
 #pragma unroll
-    for (int s = 0; s < S; ++s) X.qv[s] = sh.buf[64 * s + lane];
-    __syncthreads();
+      for (int s = 0; s < S; ++s) X.qv[s] = sh.buf[64 * s + lane];
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -791,7 +805,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
         bv[s] = v0;
       } else {
         const double dp0 = L.v[s] ? f.delp[o] : 100.0;
-        const double r = 1.0 / (dp0 + mp[s]);
+        const double r = recip<R>(dp0 + mp[s]);
         a[s] = mp[s] * r;
         bu[s] = dp0 * u0 * r;
         bv[s] = dp0 * v0 * r;
@@ -825,8 +839,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
     if (!L.v[s]) continue;
     const int32_t o = G.off[s];
     const double dp0 = f.delp[o];
-    const double omq0 = dp0 / X.dp1[s];
-    const double omq = X.dp1[s] / dp0;
+    const double omq0 = fdiv<R>(dp0, X.dp1[s]);
+    const double omq = fdiv<R>(X.dp1[s], dp0);
     // q0 is rounded exactly as fields_init rounded it (launder keeps the
     // product out of a fused multiply-add with the difference), so an
     // untouched species has a zero tendency
@@ -838,11 +852,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
     f.qg_dt[o] = (acc ? f.qg_dt[o] : 0.0) + rdt * (X.qg[s] - launder(f.qg[o] * omq0)) * omq;
     const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
                        (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
-    f.pt_dt[o] = (acc ? f.pt_dt[o] : 0.0) + rdt * (X.tz[s] - f.pt[o]) * cvm / CP_AIR;
+    f.pt_dt[o] = (acc ? f.pt_dt[o] : 0.0) + fdiv<R>(rdt * (X.tz[s] - f.pt[o]) * cvm, CP_AIR);
     if (n.do_qa)
       f.qa_dt[o] = 0.0;
     else
-      f.qa_dt[o] = (acc ? f.qa_dt[o] : 0.0) + rdt * (X.qaz[s] / P.ntimes - f.qa[o]);
+      f.qa_dt[o] = (acc ? f.qa_dt[o] : 0.0) + rdt * (fdiv<R>(X.qaz[s], P.ntimes) - f.qa[o]);
     f.qaz[o] = X.qaz[s];
   }
   if (lane < ncols) {

@@ -338,8 +338,8 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // instant evaporation / sublimation of all clouds when rh < rh_adj
   const double qpz = s.qv + s.ql + s.qi;
-  const double dtin = (lhl * (s.ql + s.qi) + lhi * s.qi) /
-                      (c.c_air_d + qpz * c.c_vap_d + s.qr * C_LIQ + (s.qs + s.qg) * C_ICE);
+  const double dtin = fdiv<R>(lhl * (s.ql + s.qi) + lhi * s.qi,
+                             c.c_air_d + qpz * c.c_vap_d + s.qr * C_LIQ + (s.qs + s.qg) * C_ICE);
   const double tin = s.tz - dtin;
   const double t_sub6 = c.t_sub_d + 6.0;
   double rh = 0.0;
@@ -361,7 +361,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   const double dq0 = qsw - s.qv;
   double evap;
   if (dq0 > 0.0) {
-    R factor = mmin(R(1.0), c.fac_l2v * R(10.0 * dq0 / qsw));
+    R factor = mmin(R(1.0), c.fac_l2v * R(fdiv<R>(10.0 * dq0, qsw)));
     evap = mmin(s.ql, fdiv<R>(double(factor) * dq0, 1.0 + tcp3 * dwsdt));
   } else {
     evap = fdiv<R>(dq0, 1.0 + tcp3 * dwsdt);
@@ -488,7 +488,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     double dqsdt;
     const double qsi = qs_i<R>(s.tz, den_d, dqsdt);
     const double dq = fdiv<R>(s.qv - qsi, 1.0 + tcpk * dqsdt);
-    R pgsub = R(s.qv / qsi - 1.0) * R(s.qg);
+    R pgsub = R(fdiv<R>(s.qv, qsi) - 1.0) * R(s.qg);
     double pg;
     if (pgsub > R(0.0)) {
       if (s.tz > tice_d)
@@ -541,9 +541,9 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
       const double qsw3 = qs_w1<R>(tin2, den_d);
       double rqi;
       if (q_cond > 3.0e-6)
-        rqi = q_sol2 / q_cond;
+        rqi = fdiv<R>(q_sol2, q_cond);
       else
-        rqi = (tice_d - tin2) / (tice_d - c.t_wfr_d);
+        rqi = fdiv<R>(tice_d - tin2, tice_d - c.t_wfr_d);
       qstar = rqi * qsi + (1.0 - rqi) * qsw3;
     }
     if (qpz2 > QRMIN) {
@@ -553,7 +553,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
       if (qstar < q_minus)
         s.qa = s.qa + 1.0;
       else if ((qstar < q_plus) && (q_cond > c.qc_crt_d))
-        s.qa = s.qa + (q_plus - qstar) / (dq + dq);
+        s.qa = s.qa + fdiv<R>(q_plus - qstar, dq + dq);
     }
   }
 }
@@ -609,7 +609,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
     R pgacr = R(0.0), pgacw = R(0.0);
     R tc = R(s.tz - tice_d);
     if (s.tz - tice_d >= 0.0) {
-      R dqs0 = R(double(c.ces0) / p1d - s.qv);
+      R dqs0 = R(fdiv<R>(double(c.ces0), p1d) - s.qv);
       if (s.qs > QCMIN) {
         R psacw, psacr, pracs;
         if (s.ql > QRMIN) {
@@ -709,12 +709,12 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         // the limiter that can drain all rain is applied in FP64, so a fully
         // frozen layer keeps no FP32 residue
         const double sink = double(psacr) + double(pgfr);
-        const double factor = mmin(sink, mmin(s.qr, fdiv<R>(-(s.tz - tice_d), icpk))) / mmax(sink, QRMIN);
+        const double factor = fdiv<R>(mmin(sink, mmin(s.qr, fdiv<R>(-(s.tz - tice_d), icpk))), mmax(sink, QRMIN));
         double ps = factor * double(psacr), pg = factor * double(pgfr);
         if constexpr (is_f32<R>::value) {
           double tot = ps + pg;
           if (tot > s.qr) {
-            double f = s.qr / tot;
+            double f = fdiv<R>(s.qr, tot);
             ps *= f;
             pg = s.qr - ps;
           }
@@ -759,7 +759,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         else
           pgacr = R(0.0);
         const double sink = double(pgacr) + double(pgacw);
-        const double factor = mmin(sink, fdiv<R>(dim(tice_d, s.tz), icpk)) / mmax(sink, QRMIN);
+        const double factor = fdiv<R>(mmin(sink, fdiv<R>(dim(tice_d, s.tz), icpk)), mmax(sink, QRMIN));
         const double pr = cap<R>(factor * double(pgacr), s.qr), pl = cap<R>(factor * double(pgacw), s.ql);
         double sk = pr + pl;
         s.qg = s.qg + sk;
