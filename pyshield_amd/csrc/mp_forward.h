@@ -80,21 +80,17 @@ struct Glob {
 template <class R>
 __device__ __forceinline__ double dry_density(double dp1, double dz0) { return fdiv<R>(-dp1, GRAV * dz0); }
 
-// Layer-bottom heights ze (BACKWARD, microphysics.py:373-380 / :756-765) and
-// layer thicknesses dz = ze[k] - ze[k+1], dz[nz-1] = ze - zs (:445-452).
+// Layer thicknesses dz = ze[k] - ze[k+1], dz[nz-1] = ze[nz-1] - zs
+// (microphysics.py:445-452) of the layer-bottom heights ze, which the
+// reference sums from the surface up, ze[k] = ze[k+1] - dz1[k] with
+// ze[nz-1] = zs - dz1[nz-1] (BACKWARD, :373-380 / :756-765).  The difference
+// of that sum is -dz1 itself; the reference's sum-then-difference differs
+// from it by round-off only (~1e-14 relative at 100 m layers 20 km up).
 template <int S>
-__device__ __forceinline__ void layer_dz(const Lay<S>& L, Shared<S>& sh, double zs, double (&dz)[S]) {
+__device__ __forceinline__ void layer_dz(const Lay<S>& L, Shared<S>& sh, double (&dz)[S]) {
   const int lane = lane_id();
-  double b[S], ze[S], zn[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const double dz1 = sh.cell[CL_DZ1][64 * s + lane];
-    b[s] = (L.kb[s] == 0) ? zs - dz1 : -dz1;
-  }
-  sum_bwd(L, b, ze);
-  col_next(sh.buf, ze, zn);
-#pragma unroll
-  for (int s = 0; s < S; ++s) dz[s] = (L.kb[s] == 0) ? ze[s] - zs : ze[s] - zn[s];
+  for (int s = 0; s < S; ++s) dz[s] = -sh.cell[CL_DZ1][64 * s + lane];
 }
 
 // Implicit upwind fall of one species over dt with speeds vt (the pattern
@@ -122,12 +118,16 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X
     b[s] = qq[s] * r;
   }
   scan_fwd(L, a, b, qm);
+  // The reference sums the flux out of each layer bottom from the top,
+  // m1[k] = m1[k-1] + qq[k] - qm[k] dz[k] (:474-481).  By the recurrence,
+  // qq[k] - qm[k] dz[k] = dd[k] qm[k] - dd[k-1] qm[k-1], so the sum
+  // telescopes to m1[k] = dd[k] qm[k] (nothing enters the top layer); the two
+  // forms differ by round-off only.
 #pragma unroll
   for (int s = 0; s < S; ++s) {
+    flux[s] = dd[s] * qm[s];
     qm[s] = qm[s] * dz[s];
-    b[s] = qq[s] - qm[s];
   }
-  sum_fwd(L, b, flux);
   col_bottom(L, sh.buf, flux, surf);
   if (do_sedi_w) {
     // dm uses the mixing ratios before the fall (:478-486)
@@ -266,7 +266,7 @@ __device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S
       }
     }
     double dz[S];
-    layer_dz(L, sh, P.zs, dz);
+    layer_dz(L, sh, dz);
     implicit_fall(L, X, sh, G, f, dz, X.qr, X.vtr, P.dt_rain, fall, P.do_sedi_w != 0, flux, surf);
     if (P.nml.do_sedi_heat) sedi_heat(L, X, sh, flux, C_LIQ, fall);
 #pragma unroll
@@ -365,7 +365,7 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
   bool fall[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) m1_sol[s] = 0.0;
-  layer_dz(L, sh, P.zs, dz);
+  layer_dz(L, sh, dz);
   const bool sedi_w = P.do_sedi_w != 0;
   // cloud ice (:773-900)
 #pragma unroll
