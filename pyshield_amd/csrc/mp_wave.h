@@ -9,10 +9,9 @@
 // Vertical operations of the reference's FORWARD/BACKWARD blocks become:
 //   * column any / first-hit  -> wave ballots + scalar bit arithmetic,
 //   * k-1 / k+1 / surface / top values -> one LDS round trip,
-//   * linear recurrences x[k] = a[k] x[k-1] + b[k] (and the mirror image
-//     from the surface up) -> segmented Hillis-Steele scans over the 64 lanes
-//     of a slot with a carry between slots; a segment starts at k = 0
-//     (forward) or k = nz-1 (backward), so columns never mix.
+//   * linear recurrences x[k] = a[k] x[k-1] + b[k] -> segmented scans over
+//     the 64 lanes of a slot (DPP row shifts and row broadcasts) with a carry
+//     between slots; a segment starts at k = 0, so columns never mix.
 // The workgroup is exactly one wavefront, so __syncthreads() is the only
 // synchronisation the LDS round trips need.
 #pragma once
@@ -155,62 +154,56 @@ __device__ __forceinline__ void col_bottom(const Lay<S>& L, T* buf, const T (&x)
 }
 
 // ---------------------------------------------------------------- scans
-// x[k] = a[k] x[k-1] + b[k] for k > 0, x[0] = b[0]   (top -> surface)
+// One DPP move of a double (two 32-bit halves).  Lanes whose source lane is
+// outside its row read 0 and lanes of rows outside ROW_MASK are not written;
+// the scan never uses either.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double src) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(src), CTRL, ROW_MASK, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(src), CTRL, ROW_MASK, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+// DPP controls (gfx9): row_shr:d shifts lanes up by d inside each 16-lane
+// row; row_bcast:15 / :31 hand lane 15 of each row (lane 31) to later rows.
+constexpr int DPP_ROW_SHR = 0x110, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+// One step of the segmented affine scan: compose (A, B) with the map of
+// source lane j, fetched by DPP, where `combine` holds: the source lane is
+// valid for the move and the lane's column reaches back to it.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step(double& A, double& B, bool combine) {
+  const double Ap = dpp_move<CTRL, ROW_MASK>(A);
+  const double Bp = dpp_move<CTRL, ROW_MASK>(B);
+  if (combine) {
+    B = A * Bp + B;
+    A = A * Ap;
+  }
+}
+
+// x[k] = a[k] x[k-1] + b[k] for k > 0, x[0] = b[0]   (top -> surface).
+// Within a slot: an inclusive scan of the affine maps x -> a x + b with DPP
+// (row shifts 1, 2, 4, 8, then the row broadcasts), segmented by column: the
+// lane at level k combines with source lane j only if k >= lane - j.  Across
+// slots: the last lane's value of slot s-1 carries into slot s.
 template <int S>
 __device__ __forceinline__ void scan_fwd(const Lay<S>& L, const double (&a)[S], const double (&b)[S],
                                          double (&x)[S]) {
   const int lane = lane_id();
+  const int in_row = lane & 15;
   double carry = 0.0;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     double A = a[s], B = b[s];
     const int k = L.k[s];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
-      if (lane >= d && k >= d) {
-        B = A * Bp + B;
-        A = A * Ap;
-      }
-    }
+    scan_step<DPP_ROW_SHR + 1, 0xF>(A, B, in_row >= 1 && k >= 1);
+    scan_step<DPP_ROW_SHR + 2, 0xF>(A, B, in_row >= 2 && k >= 2);
+    scan_step<DPP_ROW_SHR + 4, 0xF>(A, B, in_row >= 4 && k >= 4);
+    scan_step<DPP_ROW_SHR + 8, 0xF>(A, B, in_row >= 8 && k >= 8);
+    // rows 1 and 3 take lane 15 / 47, then rows 2 and 3 take lane 31
+    scan_step<DPP_ROW_BCAST15, 0xA>(A, B, (lane & 16) && k >= in_row + 1);
+    scan_step<DPP_ROW_BCAST31, 0xC>(A, B, lane >= 32 && k >= lane - 31);
     x[s] = (k <= lane) ? B : A * carry + B;
     carry = readlane(x[s], 63);
-  }
-}
-// x[k] = x[k-1] + b[k] for k > 0, x[0] = b[0]
-template <int S>
-__device__ __forceinline__ void sum_fwd(const Lay<S>& L, const double (&b)[S], double (&x)[S]) {
-  const int lane = lane_id();
-  double carry = 0.0;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    double B = b[s];
-    const int k = L.k[s];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const double Bp = __shfl_up(B, d);
-      if (lane >= d && k >= d) B = Bp + B;
-    }
-    x[s] = (k <= lane) ? B : carry + B;
-    carry = readlane(x[s], 63);
-  }
-}
-// x[k] = x[k+1] + b[k] for k < nz-1, x[nz-1] = b[nz-1]   (surface -> top)
-template <int S>
-__device__ __forceinline__ void sum_bwd(const Lay<S>& L, const double (&b)[S], double (&x)[S]) {
-  const int lane = lane_id();
-  double carry = 0.0;
-#pragma unroll
-  for (int s = S - 1; s >= 0; --s) {
-    double B = b[s];
-    const int kb = L.kb[s];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const double Bp = __shfl_down(B, d);
-      if (lane + d < 64 && kb >= d) B = Bp + B;
-    }
-    x[s] = (kb <= 63 - lane) ? B : carry + B;
-    carry = readlane(x[s], 0);
   }
 }
 
