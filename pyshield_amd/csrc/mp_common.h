@@ -116,15 +116,29 @@ __device__ __forceinline__ double rcp64(double b) {
   const double r = __builtin_amdgcn_rcp(b);
   return fma(fma(-b, r, 1.0), r, r);
 }
+// The wide (state) type of rate type R: double for the forward kernels' float
+// and double, the dual number itself for the adjoint's local Jacobians.
+struct Dual;
 template <class R>
-__device__ __forceinline__ double fdiv(double a, double b) {
+struct Wide {
+  using type = double;
+};
+template <>
+struct Wide<Dual> {
+  using type = Dual;
+};
+template <class R>
+using wide_t = typename Wide<R>::type;
+
+template <class R>
+__device__ __forceinline__ wide_t<R> fdiv(wide_t<R> a, wide_t<R> b) {
   if constexpr (sizeof(R) == 4)
     return a * rcp64(b);
   else
     return a / b;
 }
 template <class R>
-__device__ __forceinline__ double recip(double b) {
+__device__ __forceinline__ wide_t<R> recip(wide_t<R> b) {
   if constexpr (sizeof(R) == 4)
     return rcp64(b);
   else

@@ -342,23 +342,10 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
   col_from_first(L, pred, stop);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const double tcd = X.tz[s] - tice_d;
-    if (stop[s] && (X.qi[s] > QCMIN) && (tcd > 0.0)) {
-      // latent heat and heat capacity in FP64 (both modes)
-      const double lhi = LI00 + DC_ICE * X.tz[s];
-      double q_liq = X.ql[s] + X.qr[s];
-      double q_sol = X.qi[s] + X.qs[s] + X.qg[s];
-      double cvm = c.c_air_d + X.qv[s] * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
-      const double icpk = fdiv<R>(lhi, cvm);
-      const double sink = mmin(X.qi[s], fdiv<R>(double(c.fac_imlt) * tcd, icpk));
-      const double tmp = mmin(sink, dim(double(P.nml.ql_mlt), X.ql[s]));
-      X.ql[s] = X.ql[s] + tmp;
-      X.qr[s] = X.qr[s] + sink - tmp;
-      X.qi[s] = X.qi[s] - sink;
-      q_liq = q_liq + sink;
-      q_sol = q_sol - sink;
-      cvm = c.c_air_d + X.qv[s] * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
-      X.tz[s] = X.tz[s] - fdiv<R>(sink * lhi, cvm);
+    if (stop[s]) {
+      Cell cs = cell_of(X, s);
+      sedi_ice_melt<R>(c, tice_d, P.nml.ql_mlt, cs);
+      set_cell(X, s, cs);
     }
   }
   double dz[S], flux[S], surf[S], m1_sol[S];

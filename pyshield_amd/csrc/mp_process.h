@@ -25,13 +25,17 @@
 //    reference itself drove negative are left as they are.
 #pragma once
 
-#include "mp_common.h"
+#include "mp_dual.h"
 
 namespace shield {
 
-struct Cell {
-  double qv, ql, qr, qi, qs, qg, tz, qa;
+template <class W>
+struct CellT {
+  W qv, ql, qr, qi, qs, qg, tz, qa;
 };
+using Cell = CellT<double>;
+template <class R>
+using CellR = CellT<wide_t<R>>;
 
 template <class R>
 struct is_f32 {
@@ -49,7 +53,7 @@ struct is_f32<float> {
 // the cloud water leaves ql < 0, and qv follows), and there the reference's
 // transfer (0, or f q / (1 + f)) stays as it is.
 template <class R>
-__device__ __forceinline__ double cap(double x, double q) {
+__device__ __forceinline__ wide_t<R> cap(wide_t<R> x, wide_t<R> q) {
   if constexpr (is_f32<R>::value) return (x > q && x <= q * (1.0 + 1.0e-5)) ? q : x;
   return x;
 }
@@ -129,28 +133,29 @@ __device__ __forceinline__ double iqs_fast(double ta, double den, double& dqdt) 
   const double t = ice ? (ta < T_SAT_MIN ? T_SAT_MIN : ta) : (ta > TICE + 102.0 ? TICE + 102.0 : ta);
   return qs_fast(t, ice ? D2ICE : DC_VAP, ice ? LI2 : LV0, den, dqdt);
 }
-// the reference's forms in FP64 mode, the folded ones in mixed mode
+// the reference's forms in FP64 mode (and for the adjoint's dual numbers),
+// the folded ones in mixed mode
 template <class R>
-__device__ __forceinline__ double qs_w(double ta, double den, double& dqdt) {
+__device__ __forceinline__ wide_t<R> qs_w(wide_t<R> ta, wide_t<R> den, wide_t<R>& dqdt) {
   if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, dqdt);
-  else return wqs2<double>(ta, den, dqdt);
+  else return wqs2<wide_t<R>>(ta, den, dqdt);
 }
 template <class R>
-__device__ __forceinline__ double qs_w1(double ta, double den) {
-  double g;
+__device__ __forceinline__ wide_t<R> qs_w1(wide_t<R> ta, wide_t<R> den) {
+  wide_t<R> g;
   if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, g);
-  else return wqs1<double>(ta, den);
+  else return wqs1<wide_t<R>>(ta, den);
 }
 template <class R>
-__device__ __forceinline__ double qs_i(double ta, double den, double& dqdt) {
+__device__ __forceinline__ wide_t<R> qs_i(wide_t<R> ta, wide_t<R> den, wide_t<R>& dqdt) {
   if constexpr (is_f32<R>::value) return iqs_fast(ta, den, dqdt);
-  else return iqs2<double>(ta, den, dqdt);
+  else return iqs2<wide_t<R>>(ta, den, dqdt);
 }
 template <class R>
-__device__ __forceinline__ double qs_i1(double ta, double den) {
-  double g;
+__device__ __forceinline__ wide_t<R> qs_i1(wide_t<R> ta, wide_t<R> den) {
+  wide_t<R> g;
   if constexpr (is_f32<R>::value) return iqs_fast(ta, den, g);
-  else return iqs1<double>(ta, den);
+  else return iqs1<wide_t<R>>(ta, den);
 }
 
 // ------------------------------------------------ f:149-179
@@ -178,24 +183,25 @@ __device__ __forceinline__ R gmlt(const Coef<R>& c, R tc, R dqs, R qgrho, R pgac
 
 // ------------------------------------------------ f:183-265
 template <class R>
-__device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, Cell& s, double den_d,
-                                           R denfac) {
+__device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, CellR<R>& s,
+                                           wide_t<R> den_d, R denfac) {
+  using W = wide_t<R>;
   if ((s.tz > t_wfr) && (s.qr > QRMIN)) {
     const R den = R(den_d);
-    double lhl = c.lv00_d + c.d0_vap_d * s.tz;
-    double q_liq = s.ql + s.qr;
-    double q_sol = s.qi + s.qs + s.qg;
-    double cvm = SH_CVM(c, s, q_liq, q_sol);
-    double lcpk = fdiv<R>(lhl, cvm);
-    const double tin = s.tz - double(lcpk * R(s.ql));
-    const double qpz = s.qv + s.ql;
-    double dqsdt;
-    const double qsat = qs_w<R>(tin, den_d, dqsdt);
+    W lhl = c.lv00_d + c.d0_vap_d * s.tz;
+    W q_liq = s.ql + s.qr;
+    W q_sol = s.qi + s.qs + s.qg;
+    W cvm = SH_CVM(c, s, q_liq, q_sol);
+    W lcpk = fdiv<R>(lhl, cvm);
+    const W tin = s.tz - W(lcpk * R(s.ql));
+    const W qpz = s.qv + s.ql;
+    W dqsdt;
+    const W qsat = qs_w<R>(tin, den_d, dqsdt);
     R dqh = mmax(R(s.ql), h_var * mmax(R(qpz), R(QCMIN)));
     dqh = mmin(dqh, R(0.2) * R(qpz));
-    const double dqv = qsat - s.qv;
-    const double q_minus = qpz - double(dqh);
-    const double q_plus = qpz + double(dqh);
+    const W dqv = qsat - s.qv;
+    const W q_minus = qpz - W(dqh);
+    const W q_plus = qpz + W(dqh);
     if ((dqv > QVMIN) && (qsat > q_minus)) {
       R dq;
       if (qsat > q_plus)
@@ -206,7 +212,7 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
       R t2 = R(tin) * R(tin);
       R evap = ck(c, T_CREVP + 0) * t2 * dq * (ck(c, T_CREVP + 1) * msqrt(qden) + ck(c, T_CREVP + 2) * pw_el<R>(qden, 0.725)) /
                (ck(c, T_CREVP + 3) * t2 + ck(c, T_CREVP + 4) * R(qsat) * den);
-      double ev = mmin(s.qr, mmin(double(dt * evap), fdiv<R>(dqv, 1.0 + lcpk * dqsdt)));
+      W ev = mmin(s.qr, mmin(W(dt * evap), fdiv<R>(dqv, 1.0 + lcpk * dqsdt)));
       s.qr = s.qr - ev;
       s.qv = s.qv + ev;
       q_liq = q_liq - ev;
@@ -215,7 +221,7 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
     }
     if ((s.qr > QRMIN) && (s.ql > 1.0e-6) && (qsat < q_minus)) {
       R sink = dt * denfac * c.cracw * pw_el<R>(R(s.qr) * den, 0.95);
-      double sk = cap<R>(double(sink / (R(1.0) + sink) * R(s.ql)), s.ql);
+      W sk = cap<R>(W(sink / (R(1.0) + sink) * R(s.ql)), s.ql);
       s.ql = s.ql - sk;
       s.qr = s.qr + sk;
     }
@@ -270,14 +276,15 @@ __device__ __forceinline__ R rain_fspeed(const Coef<R>& c, bool no_fall, R qrz, 
 
 // ------------------------------------------------ f:375-428
 template <class R>
-__device__ __forceinline__ void autoconv_no_subgrid_var(const Coef<R>& c, double t_wfr, Cell& s, R den, R ccn,
+__device__ __forceinline__ void autoconv_no_subgrid_var(const Coef<R>& c, double t_wfr, CellR<R>& s, R den, R ccn,
                                                         R c_praut) {
+  using W = wide_t<R>;
   R qc0 = c.fac_rc * ccn;
   if (s.tz > t_wfr) {
     R qc = c.use_ccn ? qc0 : qc0 / den;
-    R dq = R(s.ql - double(qc));
+    R dq = R(s.ql - W(qc));
     if (dq > R(0.0)) {
-      double sink = mmin(double(dq), double(c.dt_rain * c_praut * den * pw_el<R>(R(s.ql), 7.0 / 3.0)));
+      W sink = mmin(W(dq), W(c.dt_rain * c_praut * den * pw_el<R>(R(s.ql), 7.0 / 3.0)));
       sink = cap<R>(sink, s.ql);
       s.ql = s.ql - sink;
       s.qr = s.qr + sink;
@@ -285,18 +292,19 @@ __device__ __forceinline__ void autoconv_no_subgrid_var(const Coef<R>& c, double
   }
 }
 template <class R>
-__device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_wfr, Cell& s, R den, R ccn,
+__device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_wfr, CellR<R>& s, R den, R ccn,
                                                      R c_praut, R dl) {
+  using W = wide_t<R>;
   R qc0 = c.fac_rc * ccn;
   if (s.tz > t_wfr + DT_FR) {
     dl = mmin(mmax(R(1.0e-6), dl), R(0.5) * R(s.ql));
     R qc = c.use_ccn ? qc0 : qc0 / den;
-    R dq = R(0.5 * (s.ql + double(dl) - double(qc)));
+    R dq = R(0.5 * (s.ql + W(dl) - W(qc)));
     if (dq > R(0.0)) {
       // unbounded by ql, as in the reference: a cell with ql ~ 1e-2 and a
       // long rain step converts more than it holds and leaves ql < 0
       R sink = mmin(R(1.0), dq / dl) * c.dt_rain * c_praut * den * pw_el<R>(R(s.ql), 7.0 / 3.0);
-      const double sk = double(sink);
+      const W sk = W(sink);
       s.ql = s.ql - sk;
       s.qr = s.qr + sk;
     }
@@ -305,22 +313,23 @@ __device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_
 
 // ------------------------------------------------ f:431-850
 template <class R>
-__device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, Cell& s, R h_var, double rh_adj,
-                                               double rh_rain, double den_d, R denfac, double p1) {
+__device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var, double rh_adj,
+                                               double rh_rain, wide_t<R> den_d, R denfac, wide_t<R> p1) {
+  using W = wide_t<R>;
   const R tice = c.tice, t_wfr = c.t_wfr, dts = c.dts;
   const R den = R(den_d);
-  double lhl = c.lv00_d + c.d0_vap_d * s.tz;
-  double lhi = LI00 + DC_ICE * s.tz;
-  double q_liq = s.ql + s.qr;
-  double q_sol = s.qi + s.qs + s.qg;
-  double cvm = SH_CVM(c, s, q_liq, q_sol);
-  double lcpk = fdiv<R>(lhl, cvm);
-  double icpk = fdiv<R>(lhi, cvm);
-  double tcpk = lcpk + icpk;
-  double tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
+  W lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  W lhi = LI00 + DC_ICE * s.tz;
+  W q_liq = s.ql + s.qr;
+  W q_sol = s.qi + s.qs + s.qg;
+  W cvm = SH_CVM(c, s, q_liq, q_sol);
+  W lcpk = fdiv<R>(lhl, cvm);
+  W icpk = fdiv<R>(lhi, cvm);
+  W tcpk = lcpk + icpk;
+  W tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
   if (!(p1 >= P_MIN)) return;
   if (s.tz < T_MIN) {
-    double sink = dim(1.0e-7, s.qv);
+    W sink = dim(1.0e-7, s.qv);
     s.qv = s.qv - sink;
     s.qi = s.qi + sink;
     q_sol = q_sol + sink;
@@ -337,12 +346,12 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
 
   // instant evaporation / sublimation of all clouds when rh < rh_adj
-  const double qpz = s.qv + s.ql + s.qi;
-  const double dtin = fdiv<R>(lhl * (s.ql + s.qi) + lhi * s.qi,
+  const W qpz = s.qv + s.ql + s.qi;
+  const W dtin = fdiv<R>(lhl * (s.ql + s.qi) + lhi * s.qi,
                              c.c_air_d + qpz * c.c_vap_d + s.qr * C_LIQ + (s.qs + s.qg) * C_ICE);
-  const double tin = s.tz - dtin;
-  const double t_sub6 = c.t_sub_d + 6.0;
-  double rh = 0.0;
+  const W tin = s.tz - dtin;
+  const W t_sub6 = c.t_sub_d + 6.0;
+  W rh = 0.0;
   if (tin > t_sub6) {
     rh = fdiv<R>(qpz, qs_i1<R>(tin, den_d));
     if (rh < rh_adj) {
@@ -356,13 +365,13 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   if (!(((tin > t_sub6) && (rh >= rh_adj)) || (tin <= t_sub6))) return;
 
   // cloud water <-> vapour adjustment
-  double dwsdt;
-  const double qsw = qs_w<R>(s.tz, den_d, dwsdt);
-  const double dq0 = qsw - s.qv;
-  double evap;
+  W dwsdt;
+  const W qsw = qs_w<R>(s.tz, den_d, dwsdt);
+  const W dq0 = qsw - s.qv;
+  W evap;
   if (dq0 > 0.0) {
     R factor = mmin(R(1.0), c.fac_l2v * R(fdiv<R>(10.0 * dq0, qsw)));
-    evap = mmin(s.ql, fdiv<R>(double(factor) * dq0, 1.0 + tcp3 * dwsdt));
+    evap = mmin(s.ql, fdiv<R>(W(factor) * dq0, 1.0 + tcp3 * dwsdt));
   } else {
     evap = fdiv<R>(dq0, 1.0 + tcp3 * dwsdt);
   }
@@ -375,9 +384,9 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   icpk = fdiv<R>(lhi, cvm);
 
   // complete freezing below t_wfr
-  const double dtmp = c.t_wfr_d - s.tz;
+  const W dtmp = c.t_wfr_d - s.tz;
   if ((dtmp > 0.0) && (s.ql > QCMIN)) {
-    double sink = mmin(s.ql, mmin(double(R(s.ql) * R(dtmp) * R(0.125)), fdiv<R>(dtmp, icpk)));
+    W sink = mmin(s.ql, mmin(W(R(s.ql) * R(dtmp) * R(0.125)), fdiv<R>(dtmp, icpk)));
     s.ql = s.ql - sink;
     s.qi = s.qi + sink;
     q_liq = q_liq - sink;
@@ -394,11 +403,11 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     dt_pisub = R(0.5) * dts;
   } else {
     dt_pisub = dts;
-    const double tcd = tice_d - s.tz;
+    const W tcd = tice_d - s.tz;
     const R tc = R(tcd);
     if ((s.ql > QRMIN) && (tcd > 0.0)) {
       R sink = R(3.3333e-10) * dts * (mexp(R(0.66) * tc) - R(1.0)) * den * R(s.ql) * R(s.ql);
-      double sk = mmin(s.ql, mmin(fdiv<R>(tcd, icpk), double(sink)));
+      W sk = mmin(s.ql, mmin(fdiv<R>(tcd, icpk), W(sink)));
       s.ql = s.ql - sk;
       s.qi = s.qi + sk;
       q_liq = q_liq - sk;
@@ -415,10 +424,10 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // sublimation / deposition of cloud ice
   if (s.tz < tice_d) {
-    double dqsdt;
-    const double qsi = qs_i<R>(s.tz, den_d, dqsdt);
-    const double dq = s.qv - qsi;
-    const double sink = fdiv<R>(dq, 1.0 + tcpk * dqsdt);
+    W dqsdt;
+    const W qsi = qs_i<R>(s.tz, den_d, dqsdt);
+    const W dq = s.qv - qsi;
+    const W sink = fdiv<R>(dq, 1.0 + tcpk * dqsdt);
     R pidep;
     if (s.qi > QRMIN) {
       pidep = dt_pisub * R(dq) * R(349138.78) * pw_el<R>(R(s.qi) * den, 0.875) /
@@ -426,16 +435,16 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
     } else {
       pidep = R(0.0);
     }
-    double sk;
+    W sk;
     if (dq > 0.0) {
-      const double tmpd = tice_d - s.tz;
+      const W tmpd = tice_d - s.tz;
       const R tmp = R(tmpd);
       R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * tmp) / den;
-      sk = mmin(sink, mmin(double(mmax(R(double(qi_crt) - s.qi), pidep)), fdiv<R>(tmpd, tcpk)));
+      sk = mmin(sink, mmin(W(mmax(R(W(qi_crt) - s.qi), pidep)), fdiv<R>(tmpd, tcpk)));
       sk = cap<R>(sk, s.qv);
     } else {
       pidep = pidep * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.2));
-      sk = mmax(mmax(double(pidep), sink), -s.qi);
+      sk = mmax(mmax(W(pidep), sink), -s.qi);
     }
     s.qv = s.qv - sk;
     s.qi = s.qi + sk;
@@ -451,25 +460,25 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // sublimation / deposition of snow
   if (s.qs > QRMIN) {
-    double dqsdt;
-    const double qsi = qs_i<R>(s.tz, den_d, dqsdt);
+    W dqsdt;
+    const W qsi = qs_i<R>(s.tz, den_d, dqsdt);
     R qden = R(s.qs) * den;
     R tmp = pw_el<R>(qden, 0.65625);
     R tsq = R(s.tz) * R(s.tz);
-    const double dsat_d = qsi - s.qv;
+    const W dsat_d = qsi - s.qv;
     const R dsat = R(dsat_d);
-    const double dq = fdiv<R>(dsat_d, 1.0 + tcpk * dqsdt);
+    const W dq = fdiv<R>(dsat_d, 1.0 + tcpk * dqsdt);
     R pssub = ck(c, T_CSSUB + 0) * tsq * (ck(c, T_CSSUB + 1) * msqrt(qden) + ck(c, T_CSSUB + 2) * tmp * msqrt(denfac)) /
               (ck(c, T_CSSUB + 3) * tsq + ck(c, T_CSSUB + 4) * R(qsi) * den);
     pssub = dsat * dts * pssub;
-    double ps;
+    W ps;
     if (pssub > R(0.0)) {
-      ps = mmin(double(pssub * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.2))), s.qs);
+      ps = mmin(W(pssub * mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.2))), s.qs);
     } else {
       if (s.tz > tice_d)
         ps = 0.0;
       else
-        ps = cap<R>(-mmax(double(pssub), mmax(dq, fdiv<R>(s.tz - tice_d, tcpk))), s.qv) * -1.0;
+        ps = cap<R>(-mmax(W(pssub), mmax(dq, fdiv<R>(s.tz - tice_d, tcpk))), s.qv) * -1.0;
     }
     s.qs = s.qs - ps;
     s.qv = s.qv + ps;
@@ -485,19 +494,19 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // two-way graupel sublimation / deposition
   if (s.qg > QRMIN) {
-    double dqsdt;
-    const double qsi = qs_i<R>(s.tz, den_d, dqsdt);
-    const double dq = fdiv<R>(s.qv - qsi, 1.0 + tcpk * dqsdt);
+    W dqsdt;
+    const W qsi = qs_i<R>(s.tz, den_d, dqsdt);
+    const W dq = fdiv<R>(s.qv - qsi, 1.0 + tcpk * dqsdt);
     R pgsub = R(fdiv<R>(s.qv, qsi) - 1.0) * R(s.qg);
-    double pg;
+    W pg;
     if (pgsub > R(0.0)) {
       if (s.tz > tice_d)
         pg = 0.0;
       else
-        pg = cap<R>(mmin(mmin(double(c.fac_v2g * pgsub), 0.2 * dq), mmin(s.ql + s.qr, fdiv<R>(tice_d - s.tz, tcpk))),
+        pg = cap<R>(mmin(mmin(W(c.fac_v2g * pgsub), 0.2 * dq), mmin(s.ql + s.qr, fdiv<R>(tice_d - s.tz, tcpk))),
                     s.qv);
     } else {
-      pg = -cap<R>(-(mmax(double(c.fac_g2v * pgsub), dq) * double(mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.1)))),
+      pg = -cap<R>(-(mmax(W(c.fac_g2v * pgsub), dq) * W(mmin(R(1.0), R(dim(s.tz, c.t_sub_d)) * R(0.1)))),
                    s.qg);
     }
     s.qg = s.qg + pg;
@@ -511,9 +520,9 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
   lhl = c.lv00_d + c.d0_vap_d * s.tz;
   lcpk = fdiv<R>(lhl, cvm);
   if (s.qr > QCMIN) {
-    double dqsdt;
-    const double qsw2 = qs_w<R>(s.tz, den_d, dqsdt);
-    double sink = mmin(s.qr, fdiv<R>(dim(rh_rain * qsw2, s.qv), 1.0 + lcpk * dqsdt));
+    W dqsdt;
+    const W qsw2 = qs_w<R>(s.tz, den_d, dqsdt);
+    W sink = mmin(s.qr, fdiv<R>(dim(rh_rain * qsw2, s.qv), 1.0 + lcpk * dqsdt));
     s.qv = s.qv + sink;
     s.qr = s.qr - sink;
     q_liq = q_liq - sink;
@@ -526,20 +535,20 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 
   // cloud fraction
   if (!c.do_qa) {
-    const double q_sol2 = c.rad_snow ? s.qi + s.qs : s.qi;
-    const double q_liq2 = c.rad_rain ? s.ql + s.qr : s.ql;
-    const double q_cond = q_liq2 + q_sol2;
-    const double qpz2 = s.qv + q_cond;
-    const double tin2 = s.tz - (lcpk * q_cond + icpk * q_sol2);
-    double qstar;
+    const W q_sol2 = c.rad_snow ? s.qi + s.qs : s.qi;
+    const W q_liq2 = c.rad_rain ? s.ql + s.qr : s.ql;
+    const W q_cond = q_liq2 + q_sol2;
+    const W qpz2 = s.qv + q_cond;
+    const W tin2 = s.tz - (lcpk * q_cond + icpk * q_sol2);
+    W qstar;
     if (tin2 <= c.t_wfr_d) {
       qstar = qs_i1<R>(tin2, den_d);
     } else if (tin2 >= tice_d) {
       qstar = qs_w1<R>(tin2, den_d);
     } else {
-      const double qsi = qs_i1<R>(tin2, den_d);
-      const double qsw3 = qs_w1<R>(tin2, den_d);
-      double rqi;
+      const W qsi = qs_i1<R>(tin2, den_d);
+      const W qsw3 = qs_w1<R>(tin2, den_d);
+      W rqi;
       if (q_cond > 3.0e-6)
         rqi = fdiv<R>(q_sol2, q_cond);
       else
@@ -547,14 +556,40 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
       qstar = rqi * qsi + (1.0 - rqi) * qsw3;
     }
     if (qpz2 > QRMIN) {
-      const double dq = mmax(QCMIN, double(h_var) * qpz2);
-      const double q_plus = qpz2 + dq;
-      const double q_minus = qpz2 - dq;
+      const W dq = mmax(QCMIN, W(h_var) * qpz2);
+      const W q_plus = qpz2 + dq;
+      const W q_minus = qpz2 - dq;
       if (qstar < q_minus)
         s.qa = s.qa + 1.0;
       else if ((qstar < q_plus) && (q_cond > c.qc_crt_d))
         s.qa = s.qa + fdiv<R>(q_plus - qstar, dq + dq);
     }
+  }
+}
+
+// ------------------------------------------------ microphysics.py:702-746
+// Melting of cloud ice in a layer at or below the first level warmer than
+// tice (the caller selects the layers).
+template <class R>
+__device__ __forceinline__ void sedi_ice_melt(const Coef<R>& c, double tice_d, double ql_mlt, CellR<R>& s) {
+  using W = wide_t<R>;
+  const W tcd = s.tz - tice_d;
+  if ((s.qi > QCMIN) && (tcd > 0.0)) {
+    // latent heat and heat capacity in FP64 (both modes)
+    const W lhi = LI00 + DC_ICE * s.tz;
+    W q_liq = s.ql + s.qr;
+    W q_sol = s.qi + s.qs + s.qg;
+    W cvm = c.c_air_d + s.qv * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
+    const W icpk = fdiv<R>(lhi, cvm);
+    const W sink = mmin(s.qi, fdiv<R>(W(c.fac_imlt) * tcd, icpk));
+    const W tmp = mmin(sink, dim(W(ql_mlt), s.ql));
+    s.ql = s.ql + tmp;
+    s.qr = s.qr + sink - tmp;
+    s.qi = s.qi - sink;
+    q_liq = q_liq + sink;
+    q_sol = q_sol - sink;
+    cvm = c.c_air_d + s.qv * c.c_vap_d + q_liq * C_LIQ + q_sol * C_ICE;
+    s.tz = s.tz - fdiv<R>(sink * lhi, cvm);
   }
 }
 
@@ -564,15 +599,17 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 // icloud continues from.
 template <class R>
 __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice_d, double t_wfr_d, double ql_mlt,
-                                                   Cell& s, double den, double& q_liq, double& q_sol, double& cvm) {
-  const double lhi = LI00 + DC_ICE * s.tz;
+                                                   CellR<R>& s, wide_t<R> den, wide_t<R>& q_liq, wide_t<R>& q_sol,
+                                                   wide_t<R>& cvm) {
+  using W = wide_t<R>;
+  const W lhi = LI00 + DC_ICE * s.tz;
   q_liq = s.ql + s.qr;
   q_sol = s.qi + s.qs + s.qg;
   cvm = SH_CVM(c, s, q_liq, q_sol);
-  const double icpk = fdiv<R>(lhi, cvm);
+  const W icpk = fdiv<R>(lhi, cvm);
   if ((s.tz > tice_d) && (s.qi > QCMIN)) {
-    const double melt = mmin(s.qi, fdiv<R>(double(c.fac_imlt) * (s.tz - tice_d), icpk));
-    const double tmp = mmin(melt, dim(ql_mlt, s.ql));
+    const W melt = mmin(s.qi, fdiv<R>(W(c.fac_imlt) * (s.tz - tice_d), icpk));
+    const W tmp = mmin(melt, dim(ql_mlt, s.ql));
     s.ql = s.ql + tmp;
     s.qr = s.qr + melt - tmp;
     s.qi = s.qi - melt;
@@ -581,11 +618,11 @@ __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice
     cvm = SH_CVM(c, s, q_liq, q_sol);
     s.tz = s.tz - fdiv<R>(melt * lhi, cvm);
   } else if ((s.tz < t_wfr_d) && (s.ql > QCMIN)) {
-    const double dtmp = t_wfr_d - s.tz;
+    const W dtmp = t_wfr_d - s.tz;
     const R factor = mmin(R(1.0), R(dtmp) / R(DT_FR));
-    const double sink = mmin(s.ql * double(factor), fdiv<R>(dtmp, icpk));
+    const W sink = mmin(s.ql * W(factor), fdiv<R>(dtmp, icpk));
     const R qi_crt = c.qi_gen * mmin(c.qi_lim, R(0.1) * R(tice_d - s.tz)) / R(den);
-    const double tmp = mmin(sink, dim(double(qi_crt), s.qi));
+    const W tmp = mmin(sink, dim(W(qi_crt), s.qi));
     s.ql = s.ql - sink;
     s.qs = s.qs + sink - tmp;
     s.qi = s.qi + tmp;
@@ -598,18 +635,20 @@ __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice
 
 // ------------------------------------------------ f:853-1260
 template <class R>
-__device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, Cell& s, R h_var,
-                                            double rh_adj, double rh_rain, double den_d, R denfac, R vtgz, R vtrz,
-                                            R vtsz, double p1d, R di, double q_liq, double q_sol, double cvm) {
+__device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>& s, R h_var,
+                                            double rh_adj, double rh_rain, wide_t<R> den_d, R denfac, R vtgz, R vtrz,
+                                            R vtsz, wide_t<R> p1d, R di, wide_t<R> q_liq, wide_t<R> q_sol,
+                                            wide_t<R> cvm) {
+  using W = wide_t<R>;
   const R dts = c.dts, rdts = c.rdts;
   const R den = R(den_d);
-  double lhi = LI00 + DC_ICE * s.tz;
-  double icpk = fdiv<R>(lhi, cvm);
+  W lhi = LI00 + DC_ICE * s.tz;
+  W icpk = fdiv<R>(lhi, cvm);
   if (p1d >= P_MIN) {
     R pgacr = R(0.0), pgacw = R(0.0);
     R tc = R(s.tz - tice_d);
     if (s.tz - tice_d >= 0.0) {
-      R dqs0 = R(fdiv<R>(double(c.ces0), p1d) - s.qv);
+      R dqs0 = R(fdiv<R>(W(c.ces0), p1d) - s.qv);
       if (s.qs > QCMIN) {
         R psacw, psacr, pracs;
         if (s.ql > QRMIN) {
@@ -627,9 +666,9 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           pracs = R(0.0);
         }
         R psmlt = mmax(R(0.0), smlt(c, tc, dqs0, R(s.qs) * den, psacw, psacr, den, denfac));
-        double sink = mmin(s.qs, mmin(double(dts * (psmlt + pracs)), fdiv<R>(s.tz - tice_d, icpk)));
+        W sink = mmin(s.qs, mmin(W(dts * (psmlt + pracs)), fdiv<R>(s.tz - tice_d, icpk)));
         s.qs = s.qs - sink;
-        double tmp = mmin(sink, double(dim(c.qs_mlt, R(s.ql))));
+        W tmp = mmin(sink, W(dim(c.qs_mlt, R(s.ql))));
         s.ql = s.ql + tmp;
         s.qr = s.qr + sink - tmp;
         q_liq = q_liq + sink;
@@ -650,7 +689,7 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
           pgacw = factor / (R(1.0) + dts * factor) * R(s.ql);
         }
         R pgmlt = dts * gmlt(c, tc, dqs0, qden, pgacw, pgacr, den);
-        double pm = mmin(double(mmax(R(0.0), pgmlt)), mmin(s.qg, fdiv<R>(s.tz - tice_d, icpk)));
+        W pm = mmin(W(mmax(R(0.0), pgmlt)), mmin(s.qg, fdiv<R>(s.tz - tice_d, icpk)));
         s.qg = s.qg - pm;
         s.qr = s.qr + pm;
         q_liq = q_liq + pm;
@@ -675,12 +714,12 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         R qim = c.qi0_crt / den;
         R tmp = c.const_vi ? c.fac_i2s : c.fac_i2s * mexp(R(0.025) * tc);
         di = mmax(di, R(QRMIN));
-        const double q_plus = s.qi + double(di);
-        const double qimd = double(qim);
+        const W q_plus = s.qi + W(di);
+        const W qimd = W(qim);
         R psaut;
         if (q_plus > (qimd + QRMIN)) {
           R dq;
-          if (qimd > (s.qi - double(di)))
+          if (qimd > (s.qi - W(di)))
             dq = (R(0.25) * R((q_plus - qimd) * (q_plus - qimd))) / di;
           else
             dq = R(s.qi - qimd);
@@ -688,12 +727,12 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         } else {
           psaut = R(0.0);
         }
-        double sink = mmin(0.75 * s.qi, double(psaci + psaut));
+        W sink = mmin(0.75 * s.qi, W(psaci + psaut));
         s.qi = s.qi - sink;
         s.qs = s.qs + sink;
         if (s.qg > 1.0e-6) {
           R factor = dts * c.cgaci * msqrt(den) * R(s.qg);
-          double pgaci = cap<R>(double(factor / (R(1.0) + factor) * R(s.qi)), s.qi);
+          W pgaci = cap<R>(W(factor / (R(1.0) + factor) * R(s.qi)), s.qi);
           s.qi = s.qi - pgaci;
           s.qg = s.qg + pgaci;
         }
@@ -708,18 +747,18 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         R pgfr = dts * ck(c, T_CGFR + 0) / den * (mexp(-ck(c, T_CGFR + 1) * tc) - R(1.0)) * pw_el<R>(R(s.qr) * den, 1.75);
         // the limiter that can drain all rain is applied in FP64, so a fully
         // frozen layer keeps no FP32 residue
-        const double sink = double(psacr) + double(pgfr);
-        const double factor = fdiv<R>(mmin(sink, mmin(s.qr, fdiv<R>(-(s.tz - tice_d), icpk))), mmax(sink, QRMIN));
-        double ps = factor * double(psacr), pg = factor * double(pgfr);
+        const W sink = W(psacr) + W(pgfr);
+        const W factor = fdiv<R>(mmin(sink, mmin(s.qr, fdiv<R>(-(s.tz - tice_d), icpk))), mmax(sink, QRMIN));
+        W ps = factor * W(psacr), pg = factor * W(pgfr);
         if constexpr (is_f32<R>::value) {
-          double tot = ps + pg;
+          W tot = ps + pg;
           if (tot > s.qr) {
-            double f = fdiv<R>(s.qr, tot);
+            W f = fdiv<R>(s.qr, tot);
             ps *= f;
             pg = s.qr - ps;
           }
         }
-        double sk = ps + pg;
+        W sk = ps + pg;
         s.qr = s.qr - sk;
         s.qs = s.qs + ps;
         s.qg = s.qg + pg;
@@ -737,11 +776,11 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
         else
           sink = R(0.0);
         R qsm = c.qs0_crt / den;
-        if (s.qs > double(qsm)) {
+        if (s.qs > W(qsm)) {
           R factor = dts * R(1.0e-3) * mexp(R(0.09) * R(s.tz - tice_d));
-          sink = sink + factor / (R(1.0) + factor) * R(s.qs - double(qsm));
+          sink = sink + factor / (R(1.0) + factor) * R(s.qs - W(qsm));
         }
-        double sk = mmin(s.qs, double(sink));
+        W sk = mmin(s.qs, W(sink));
         s.qs = s.qs - sk;
         s.qg = s.qg + sk;
       }
@@ -758,10 +797,10 @@ __device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, dou
                        R(s.qr));
         else
           pgacr = R(0.0);
-        const double sink = double(pgacr) + double(pgacw);
-        const double factor = fdiv<R>(mmin(sink, fdiv<R>(dim(tice_d, s.tz), icpk)), mmax(sink, QRMIN));
-        const double pr = cap<R>(factor * double(pgacr), s.qr), pl = cap<R>(factor * double(pgacw), s.ql);
-        double sk = pr + pl;
+        const W sink = W(pgacr) + W(pgacw);
+        const W factor = fdiv<R>(mmin(sink, fdiv<R>(dim(tice_d, s.tz), icpk)), mmax(sink, QRMIN));
+        const W pr = cap<R>(factor * W(pgacr), s.qr), pl = cap<R>(factor * W(pgacw), s.ql);
+        W sk = pr + pl;
         s.qg = s.qg + sk;
         s.qr = s.qr - pr;
         s.ql = s.ql - pl;
