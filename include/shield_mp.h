@@ -33,7 +33,9 @@ enum shield_status {
   SHIELD_ERR_ALIAS = -5,       /* an output aliases another field                 */
   SHIELD_ERR_LAUNCH = -6,      /* HIP launch or runtime failure                   */
   SHIELD_ERR_TIMESTEP = -7,    /* timestep <= 0 or not finite                     */
-  SHIELD_ERR_NOGPU = -8        /* no HIP device visible                           */
+  SHIELD_ERR_NOGPU = -8,       /* no HIP device visible                           */
+  SHIELD_ERR_ADJ_SWITCH = -9,  /* a switch the adjoint does not differentiate     */
+  SHIELD_ERR_WORKSPACE = -10   /* workspace smaller than one wavefront's tape     */
 };
 
 /* Precision modes of shield_mp_forward. */
@@ -244,6 +246,41 @@ typedef struct shield_prologue_fields {
   int64_t si_i, sj_i;
   double ptop;  /* model-top pressure (grid_data.ptop)                      */
 } shield_prologue_fields;
+
+/* ---------------------------------------------------------- adjoint */
+/* Cotangents for shield_mp_adjoint.  Output cotangents (read) belong to the
+ * outputs of shield_mp_forward: the ten tendencies (3D, strides of the
+ * shape) and the four surface precipitation rates (2D, any may be NULL for a
+ * zero cotangent).  Input cotangents (written) belong to the twelve input
+ * fields and the incoming cloud-fraction accumulator (qaz may be NULL).
+ * params receives the gradient (device array of SHIELD_MP_NPARAMS doubles,
+ * summed over the domain) of the six process parameters, in the order
+ * fac_rc, vr_fac, vs_fac, vg_fac, vi_fac, qi_gen of shield_mp_params /
+ * shield_mp_namelist.                                                        */
+#define SHIELD_MP_NPARAMS 6
+typedef struct shield_mp_adjoint_fields {
+  const double *pt_dt, *qv_dt, *ql_dt, *qr_dt, *qi_dt, *qs_dt, *qg_dt, *qa_dt, *udt, *vdt;
+  const double *rain, *snow, *ice, *graupel;
+  double *pt, *qv, *ql, *qr, *qi, *qs, *qg, *qa, *ua, *va, *delp, *delz;
+  double* qaz;
+  double* params;
+} shield_mp_adjoint_fields;
+
+/* Bytes of workspace for `waves` concurrent wavefronts (each walks columns
+ * and records its column's per-sub-step state there); 0 on a bad shape.     */
+size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shield_mp_params* p, int64_t waves);
+
+/* Reverse-mode adjoint of one FP64 shield_mp_forward call with the inputs
+ * f (its output pointers are not used): input cotangents and the parameter
+ * gradient from output cotangents, as for overwritten tendencies
+ * (accumulate = 0).  The forward of each column is recomputed, so f's inputs
+ * must be those of the forward call.  The workspace (ws, ws_bytes) bounds
+ * the number of concurrent wavefronts; nk <= 128.  Returns
+ * SHIELD_ERR_ADJ_SWITCH for do_sedi_w, z_slope_ice, fix_negative or de_ice.
+ * No reference counterpart: the reference has no derivative code; this is
+ * the torch.autograd backward of the microphysics call.                     */
+int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                      const shield_mp_adjoint_fields* a, void* ws, size_t ws_bytes, void* stream);
 
 /* The column-local part of Physics.__call__ before the microphysics
  * (physics.py:278-330): atmos_phys_driver_statein (physics.py:26-102),

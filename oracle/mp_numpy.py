@@ -93,7 +93,13 @@ class Ops:
 
 
 class TorchOps:
-    """torch namespace (float64 tensors, autograd-safe where-masking)."""
+    """torch namespace (float64 tensors, autograd-safe where-masking).
+
+    minimum, maximum and abs select an operand, as the reference's
+    ``a < b ? a : b`` does, so a gradient follows the branch the forward
+    takes: at a tie it goes to the second operand (torch.minimum would split
+    it), and abs at 0 passes it unchanged.  Values are those of the numpy
+    namespace."""
 
     def __init__(self):
         import torch
@@ -109,17 +115,27 @@ class TorchOps:
         return self.t.sqrt(x)
 
     def abs(self, x):
-        return self.t.abs(x)
+        return self.t.where(x < 0.0, -x, x)
 
     def minimum(self, a, b):
-        return self.t.minimum(*self._b(a, b))
+        a, b = self._b(a, b)
+        return self.t.where(a < b, a, b)
 
     def maximum(self, a, b):
-        return self.t.maximum(*self._b(a, b))
+        a, b = self._b(a, b)
+        return self.t.where(a > b, a, b)
 
     def where(self, c, a, b):
+        t = self.t
+        if any(isinstance(x, bool) or (isinstance(x, t.Tensor) and x.dtype == t.bool) for x in (a, b)):
+            a, b = self._bb(a, b)
+            if not isinstance(a, t.Tensor):
+                a = t.full_like(c, bool(a))
+            if not isinstance(b, t.Tensor):
+                b = t.full_like(c, bool(b))
+            return t.where(c, a, b)
         a, b = self._b(a, b, like=c)
-        return self.t.where(c, a, b)
+        return t.where(c, a, b)
 
     def full_like(self, x, v):
         return self.t.full_like(x, v)
@@ -782,13 +798,17 @@ def _slope(xp, q):
     return xp.stack_k(dl)
 
 
-def run(nml, timestep, state, qaz=None, tend=None, xp=None):
+def run(nml, timestep, state, qaz=None, tend=None, xp=None, derived_overrides=None):
     """One Microphysics call.  ``state`` maps pt qv ql qr qi qs qg qa ua va delp
     delz w ([ncol, nz]) and land area ([ncol]) to arrays of the namespace's
-    type.  Returns the ten tendencies, qaz, w and the 2D precipitation."""
+    type.  Returns the ten tendencies, qaz, w and the 2D precipitation.
+    ``derived_overrides`` replaces derived scalars (e.g. fac_rc by a tensor
+    that carries a gradient under TorchOps)."""
     xp = xp or Ops
     n = nml
     d = derived(nml, timestep)
+    if derived_overrides:
+        d.update(derived_overrides)
     P = Proc(xp, n, d)
     W, AND, NOT = xp.where, xp.logical_and, xp.logical_not
     if n["use_ppm"] or n["prog_ccn"]:

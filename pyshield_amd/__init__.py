@@ -6,13 +6,13 @@ kernels for gfx950 behind the C ABI in include/shield_mp.h.
 """
 
 from ._config import PHYSICS_PACKAGES, PhysicsConfig
-from .microphysics import (GridData, Microphysics, MicrophysicsState, QuantityFactory,
-                           StencilFactory, forward, forward_update)
+from .microphysics import (GridData, Microphysics, MicrophysicsFunction, MicrophysicsState, QuantityFactory,
+                           StencilFactory, adjoint, forward, forward_update, microphysics_ad)
 from .physics import Physics, physics_prologue
 
 __all__ = [
     "PHYSICS_PACKAGES", "PhysicsConfig", "Microphysics", "MicrophysicsState",
     "StencilFactory", "QuantityFactory", "GridData", "forward", "forward_update", "Physics",
-    "physics_prologue",
+    "physics_prologue", "adjoint", "MicrophysicsFunction", "microphysics_ad",
 ]
 __version__ = "0.2.0"

@@ -23,6 +23,7 @@ EXPORTS = (
     "shield_prepare_microphysics", "shield_update_physics_state_with_tendencies",
     "shield_fill_gfs_delp", "shield_prepare_tendencies_and_update_tracers",
     "shield_strerror", "shield_device_count", "shield_mp_layout", "shield_physics_prologue",
+    "shield_mp_adjoint", "shield_mp_adjoint_workspace_bytes",
 )
 
 _D = ctypes.c_double
@@ -86,6 +87,19 @@ class ShieldMpFields(ctypes.Structure):
     _fields_ = [(n, _VP) for n in FIELD_NAMES]
 
 
+# shield_mp_adjoint_fields: cotangents of the forward's outputs (read), of its
+# inputs (written), and the gradient of the six process parameters
+ADJ_OUT_NAMES = ("pt_dt", "qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa_dt", "udt", "vdt",
+                 "rain", "snow", "ice", "graupel")
+ADJ_IN_NAMES = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz", "qaz")
+AD_PARAM_NAMES = ("fac_rc", "vr_fac", "vs_fac", "vg_fac", "vi_fac", "qi_gen")
+
+
+class ShieldMpAdjointFields(ctypes.Structure):
+    _fields_ = [("d_" + n, _VP) for n in ADJ_OUT_NAMES] + [("g_" + n, _VP) for n in ADJ_IN_NAMES] + [
+        ("params", _VP)]
+
+
 # physics_updated_* order of shield_update_fields.updated and the FV3 tracer
 # order of shield_update_fields.tracers (include/shield_mp.h)
 UPDATED_NAMES = ("qvapor", "qliquid", "qrain", "qice", "qsnow", "qgraupel", "qcld", "pt", "ua", "va")
@@ -140,9 +154,15 @@ def lib():
         L.shield_strerror.restype = ctypes.c_char_p
         L.shield_mp_layout.argtypes = [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int)]
+        L.shield_mp_adjoint.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldMpParams),
+                                        ctypes.POINTER(ShieldMpFields), ctypes.POINTER(ShieldMpAdjointFields),
+                                        _VP, ctypes.c_size_t, _VP]
+        L.shield_mp_adjoint_workspace_bytes.argtypes = [ctypes.POINTER(ShieldMpShape),
+                                                        ctypes.POINTER(ShieldMpParams), _I64]
         for fn in EXPORTS:
             if fn != "shield_strerror":
                 getattr(L, fn).restype = ctypes.c_int
+        L.shield_mp_adjoint_workspace_bytes.restype = ctypes.c_size_t
         _lib = L
     return _lib
 

@@ -1,0 +1,1234 @@
+// mp_adjoint.h -- the reverse-mode adjoint of one FP64 microphysics call.
+//
+// Given the call's inputs and cotangents of its outputs (the ten tendencies
+// and the four surface precipitation rates), the adjoint returns cotangents
+// of the twelve input fields (pt, qv, ql, qr, qi, qs, qg, qa, ua, va, delp,
+// delz), of the incoming cloud-fraction accumulator qaz, and the gradient of
+// six process parameters: fac_rc (autoconversion threshold qc0 = fac_rc ccn),
+// vr_fac, vs_fac, vg_fac, vi_fac (fall-speed coefficients) and qi_gen (ice
+// generation / nucleation).
+//
+// Layout: one column per wavefront (S = ceil(nz / 64) slots), and a
+// persistent grid whose wavefronts walk the columns.  Each wavefront
+//  1. replays the forward of its column (the FP64 forward's building blocks,
+//     in the forward's order) and records the per-cell state at 12 points of
+//     every sub-step in its own region of the workspace;
+//  2. walks the steps backwards.  A point-wise step (density update, fall
+//     speeds, evaporation / accretion, autoconversion, ice melt, icloud)
+//     gets its vector-Jacobian product from the same process code run on
+//     dual numbers, one pass per input, so every derivative follows the
+//     branch the forward took at the point.  The vertical steps have
+//     hand-written transposes: the implicit fall and the sedimentation
+//     heating (backward affine scans), the liquid slope limiter, the
+//     momentum transport, and the precipitation and flux sums.
+// Supported switches: the default ones, plus the const_v*, irain_f,
+// z_slope_liq, do_sedi_heat, sedi_transport, do_qa and reset_qaz variants;
+// do_sedi_w, z_slope_ice, fix_negative and de_ice are rejected by the host.
+#pragma once
+
+#include "mp_forward.h"
+
+namespace shield {
+
+enum { AP_FAC_RC, AP_VR_FAC, AP_VS_FAC, AP_VG_FAC, AP_VI_FAC, AP_QI_GEN, AP_N };
+
+// per-cell record of the state: the prognostic cell and the auxiliary fields
+enum { Z_QV, Z_QL, Z_QR, Z_QI, Z_QS, Z_QG, Z_TZ, Z_QA, Z_DEN, Z_DENFAC, Z_DZ1, Z_VTR, Z_VTG, Z_VTS, Z_N };
+// record points of one sub-step: warm_rain (first) before the evaporation,
+// before the fall, before the second evaporation, before autoconversion;
+// sedimentation at entry, after the ice melt, after the falls; the same four
+// for warm_rain (second); icloud at entry
+enum { RW1_B, RW1_C, RW1_D, RW1_E, RS_F, RS_G, RS_H, RW2_B, RW2_C, RW2_D, RW2_E, RI, REC_N };
+// per-cell cotangents of the auxiliary fields, kept in LDS
+enum { AB_DEN, AB_DENFAC, AB_DZ1, AB_VTR, AB_VTG, AB_VTS, AB_M1, AB_M1SOL, AB_DP1, AB_T0, AB_DZ0, AB_N };
+
+template <int S>
+struct AdjShared {
+  Shared<S> sh;
+  double bar[AB_N][64 * S];
+  double colbar[8];  // rain, snow, ice, graupel (per unit of the accumulated flux), ccn, c_praut
+};
+enum { CB_RAIN, CB_SNOW, CB_ICE, CB_GRAUPEL, CB_CCN, CB_CPRAUT };
+
+template <int S>
+struct TapeView {
+  double* base;  // this wavefront's region: ntimes * REC_N * Z_N * 64 S doubles
+  __device__ __forceinline__ double* at(int it, int rec, int z) const {
+    return base + ((size_t)(it * REC_N + rec) * Z_N + z) * (64 * S);
+  }
+};
+
+template <int S>
+__device__ __forceinline__ void tape_save(const TapeView<S>& T, int it, int rec, const St<S, double>& X,
+                                          Shared<S>& sh) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int i = 64 * s + lane;
+    T.at(it, rec, Z_QV)[i] = X.qv[s];
+    T.at(it, rec, Z_QL)[i] = X.ql[s];
+    T.at(it, rec, Z_QR)[i] = X.qr[s];
+    T.at(it, rec, Z_QI)[i] = X.qi[s];
+    T.at(it, rec, Z_QS)[i] = X.qs[s];
+    T.at(it, rec, Z_QG)[i] = X.qg[s];
+    T.at(it, rec, Z_TZ)[i] = X.tz[s];
+    T.at(it, rec, Z_QA)[i] = X.qaz[s];
+    T.at(it, rec, Z_DEN)[i] = sh.cell[CL_DEN][i];
+    T.at(it, rec, Z_DENFAC)[i] = sh.cell[CL_DENFAC][i];
+    T.at(it, rec, Z_DZ1)[i] = sh.cell[CL_DZ1][i];
+    T.at(it, rec, Z_VTR)[i] = X.vtr[s];
+    T.at(it, rec, Z_VTG)[i] = X.vtg[s];
+    T.at(it, rec, Z_VTS)[i] = X.vts[s];
+  }
+}
+
+// a recorded state: z[field][slot]
+template <int S>
+struct Rec {
+  double z[Z_N][S];
+};
+template <int S>
+__device__ __forceinline__ void tape_load(const TapeView<S>& T, int it, int rec, Rec<S>& r) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int q = 0; q < Z_N; ++q)
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.z[q][s] = T.at(it, rec, q)[64 * s + lane];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// x[k] = a[k] x[k+1] + b[k] for k < nz-1, x[nz-1] = b[nz-1]  (surface -> top),
+// the transpose direction of scan_fwd
+template <int S>
+__device__ __forceinline__ void scan_bwd(const Lay<S>& L, const double (&a)[S], const double (&b)[S],
+                                         double (&x)[S]) {
+  const int lane = lane_id();
+  double carry = 0.0;
+#pragma unroll
+  for (int s = S - 1; s >= 0; --s) {
+    double A = a[s], B = b[s];
+    const int kb = L.kb[s];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double Ap = __shfl_down(A, d), Bp = __shfl_down(B, d);
+      if (lane + d < 64 && kb >= d) {
+        B = A * Bp + B;
+        A = A * Ap;
+      }
+    }
+    x[s] = (kb <= 63 - lane) ? B : A * carry + B;
+    carry = readlane(x[s], 0);
+  }
+}
+
+// ------------------------------------------------------------ dual helpers
+__device__ __forceinline__ Dual seed(double v, int i, int j) { return Dual(v, i == j ? 1.0 : 0.0); }
+
+__device__ __forceinline__ Coef<Dual> dual_coef(const Coef<double>& c) {
+  Coef<Dual> d;
+  d.c_air = c.c_air; d.c_vap = c.c_vap; d.d0_vap = c.d0_vap; d.lv00 = c.lv00; d.so3 = c.so3;
+  d.log_10 = c.log_10; d.tice = c.tice; d.tice0 = c.tice0; d.t_wfr = c.t_wfr; d.t_sub = c.t_sub;
+  d.fac_rc = c.fac_rc; d.cracs = c.cracs; d.csacr = c.csacr; d.cgacr = c.cgacr; d.cgacs = c.cgacs;
+  d.csacw = c.csacw; d.csaci = c.csaci; d.cgacw = c.cgacw; d.cgaci = c.cgaci; d.cracw = c.cracw;
+  d.ces0 = c.ces0;
+  for (int i = 0; i < T_N; ++i) d.tab[i] = c.tab[i];
+  d.dts = c.dts; d.rdts = c.rdts; d.dt_rain = c.dt_rain; d.fac_i2s = c.fac_i2s; d.fac_g2v = c.fac_g2v;
+  d.fac_v2g = c.fac_v2g; d.fac_imlt = c.fac_imlt; d.fac_l2v = c.fac_l2v;
+  d.ql_mlt = c.ql_mlt; d.qs_mlt = c.qs_mlt; d.qi_gen = c.qi_gen; d.qi_lim = c.qi_lim; d.qi0_crt = c.qi0_crt;
+  d.qs0_crt = c.qs0_crt; d.qc_crt = c.qc_crt;
+  d.vs_fac = c.vs_fac; d.vg_fac = c.vg_fac; d.vi_fac = c.vi_fac; d.vr_fac = c.vr_fac; d.vs_max = c.vs_max;
+  d.vg_max = c.vg_max; d.vi_max = c.vi_max; d.vr_max = c.vr_max;
+  d.const_vi = c.const_vi; d.const_vs = c.const_vs; d.const_vg = c.const_vg; d.const_vr = c.const_vr;
+  d.fast_sat_adj = c.fast_sat_adj; d.do_qa = c.do_qa; d.rad_snow = c.rad_snow; d.rad_rain = c.rad_rain;
+  d.use_ccn = c.use_ccn; d.pad_ = c.pad_;
+  d.t_sub_d = c.t_sub_d; d.t_wfr_d = c.t_wfr_d; d.qc_crt_d = c.qc_crt_d;
+  d.c_air_d = c.c_air_d; d.c_vap_d = c.c_vap_d; d.d0_vap_d = c.d0_vap_d; d.lv00_d = c.lv00_d;
+  return d;
+}
+
+// cotangents of the eight cell values, in registers
+template <int S>
+struct CellBar {
+  double v[8][S];
+};
+__device__ __forceinline__ double cell_dot(const double (&yb)[8], const CellT<Dual>& c) {
+  return yb[0] * c.qv.d + yb[1] * c.ql.d + yb[2] * c.qr.d + yb[3] * c.qi.d + yb[4] * c.qs.d + yb[5] * c.qg.d +
+         yb[6] * c.tz.d + yb[7] * c.qa.d;
+}
+template <int S>
+__device__ __forceinline__ CellT<Dual> dual_cell(const Rec<S>& r, int s, int i) {
+  return CellT<Dual>{seed(r.z[Z_QV][s], i, 0), seed(r.z[Z_QL][s], i, 1), seed(r.z[Z_QR][s], i, 2),
+                     seed(r.z[Z_QI][s], i, 3), seed(r.z[Z_QS][s], i, 4), seed(r.z[Z_QG][s], i, 5),
+                     seed(r.z[Z_TZ][s], i, 6), seed(r.z[Z_QA][s], i, 7)};
+}
+
+// the liquid slope limiter at one interior level (the forward's
+// slope_limiter, microphysics.py:595-620), on any scalar type
+template <class W>
+__device__ __forceinline__ W slope_point(W qm, W q0, W qp) {
+  const W dq = 0.5 * (q0 - qm);
+  const W dqn = 0.5 * (qp - q0);
+  W d = 0.5 * mmin(mabs(dq + dqn), 0.5 * q0);
+  if (dq * dqn <= 0.0) {
+    if (dq > 0.0)
+      d = mmin(d, mmin(dq, -dqn));
+    else
+      d = W(0.0);
+  }
+  return d;
+}
+
+// ---------------------------------------------------- vertical transposes
+// Transpose of implicit_fall for one species.  q, vt: the fall's inputs;
+// qbar: in, the cotangent of q after the fall; out, before it.  fluxbar:
+// cotangent of the flux out of each layer bottom (the surface flux included
+// in the surface layer's).  Accumulates vt, dz1 and dp1 cotangents.
+template <int S>
+__device__ __forceinline__ void fall_adj(const Lay<S>& L, Shared<S>& sh, const double (&q)[S], const double (&vt)[S],
+                                         const double (&dz1)[S], const double (&dp1)[S], double dt,
+                                         const bool (&fall)[S], double (&qbar)[S], const double (&fluxbar)[S],
+                                         double (&vtbar)[S], double (&dz1bar)[S], double (&dp1bar)[S]) {
+  double dz[S], dd[S], qq[S], ddp[S], r[S], a[S], b[S], qm[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    dz[s] = -dz1[s];
+    dd[s] = dt * vt[s];
+    qq[s] = q[s] * dp1[s];
+  }
+  col_prev(sh.buf, dd, ddp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    r[s] = 1.0 / (dz[s] + dd[s]);
+    a[s] = ddp[s] * r[s];
+    b[s] = qq[s] * r[s];
+  }
+  scan_fwd(L, a, b, qm);  // mass per unit thickness
+  double g[S], ddbar[S], dzbar[S], an[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (fall[s] && L.v[s]) {
+      const double qn = qm[s] * dz[s] / dp1[s];
+      g[s] = qbar[s] * dz[s] / dp1[s] + fluxbar[s] * dd[s];
+      dzbar[s] = qbar[s] * qm[s] / dp1[s];
+      dp1bar[s] -= qbar[s] * qn / dp1[s];
+      ddbar[s] = fluxbar[s] * qm[s];
+    } else {
+      g[s] = dzbar[s] = ddbar[s] = 0.0;
+    }
+  }
+  col_next(sh.buf, a, an);
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    if (L.kb[s] == 0) an[s] = 0.0;
+  double lam[S], qmp[S], t[S], tn[S];
+  scan_bwd(L, an, g, lam);
+  col_prev(sh.buf, qm, qmp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    t[s] = (fall[s] && L.v[s] && L.k[s] > 0) ? lam[s] * r[s] * qmp[s] : 0.0;
+  }
+  col_next(sh.buf, t, tn);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!(fall[s] && L.v[s])) continue;
+    const double qqbar = lam[s] * r[s];
+    const double sbar = -lam[s] * qm[s] * r[s];  // d qm / d (dz + dd)
+    ddbar[s] += sbar + (L.kb[s] > 0 ? tn[s] : 0.0);
+    dzbar[s] += sbar;
+    qbar[s] = qqbar * dp1[s];
+    dp1bar[s] += qqbar * q[s];
+    vtbar[s] += ddbar[s] * dt;
+    dz1bar[s] -= dzbar[s];
+  }
+}
+
+// Transpose of sedi_heat.  st: the state before the heating (tz and the
+// mixing ratios of cvn); flux: the mass flux it uses; tbar: in, cotangent of
+// tz after; out, before.  Accumulates the flux, mixing-ratio, dp1 and dz1
+// cotangents.
+template <int S>
+__device__ __forceinline__ void heat_adj(const Lay<S>& L, Shared<S>& sh, const Rec<S>& st, const double (&dp1)[S],
+                                         const double (&flux)[S], double cq, const bool (&apply)[S],
+                                         CellBar<S>& xb, double (&fluxbar)[S], double (&dp1bar)[S],
+                                         double (&dz1bar)[S]) {
+  double fp[S], cvn[S], dgz[S], a[S], b[S], tn[S], D[S];
+  col_prev(sh.buf, flux, fp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    dgz[s] = -0.5 * GRAV * st.z[Z_DZ1][s];
+    cvn[s] = dp1[s] * (CV_AIR + st.z[Z_QV][s] * CV_VAP + (st.z[Z_QR][s] + st.z[Z_QL][s]) * C_LIQ +
+                       (st.z[Z_QI][s] + st.z[Z_QS][s] + st.z[Z_QG][s]) * C_ICE);
+    D[s] = cvn[s] + cq * flux[s];
+    const double tz = st.z[Z_TZ][s];
+    if (L.k[s] == 0) {
+      a[s] = 0.0;
+      b[s] = tz + flux[s] * dgz[s] / (cvn[s] + flux[s] * cq);
+    } else {
+      const double r = 1.0 / (cvn[s] + cq * flux[s]);
+      a[s] = fp[s] * cq * r;
+      b[s] = ((cvn[s] + cq * (flux[s] - fp[s])) * tz + dgz[s] * (fp[s] + flux[s])) * r;
+    }
+  }
+  scan_fwd(L, a, b, tn);
+  double g[S], an[S], mu[S], tnp[S], fpb[S], fpbn[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const bool ap = apply[s] && L.v[s];
+    g[s] = ap ? xb.v[6][s] : 0.0;
+    if (ap) xb.v[6][s] = 0.0;
+  }
+  col_next(sh.buf, a, an);
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    if (L.kb[s] == 0) an[s] = 0.0;
+  scan_bwd(L, an, g, mu);
+  col_prev(sh.buf, tn, tnp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    fpb[s] = 0.0;
+    if (!(apply[s] && L.v[s])) continue;
+    const double m = mu[s], tz = st.z[Z_TZ][s];
+    double cvnbar, dgzbar;
+    if (L.k[s] == 0) {
+      xb.v[6][s] += m;
+      fluxbar[s] += m * dgz[s] * cvn[s] / (D[s] * D[s]);
+      dgzbar = m * flux[s] / D[s];
+      cvnbar = -m * flux[s] * dgz[s] / (D[s] * D[s]);
+    } else {
+      xb.v[6][s] += m * (cvn[s] + cq * (flux[s] - fp[s])) / D[s];
+      fluxbar[s] += m * (cq * tz + dgz[s] - cq * tn[s]) / D[s];
+      fpb[s] = m * (-cq * tz + dgz[s] + cq * tnp[s]) / D[s];
+      cvnbar = m * (tz - tn[s]) / D[s];
+      dgzbar = m * (fp[s] + flux[s]) / D[s];
+    }
+    dp1bar[s] += cvnbar * cvn[s] / dp1[s];
+    xb.v[0][s] += cvnbar * dp1[s] * CV_VAP;
+    xb.v[1][s] += cvnbar * dp1[s] * C_LIQ;
+    xb.v[2][s] += cvnbar * dp1[s] * C_LIQ;
+    xb.v[3][s] += cvnbar * dp1[s] * C_ICE;
+    xb.v[4][s] += cvnbar * dp1[s] * C_ICE;
+    xb.v[5][s] += cvnbar * dp1[s] * C_ICE;
+    dz1bar[s] += dgzbar * (-0.5 * GRAV);
+  }
+  col_next(sh.buf, fpb, fpbn);  // flux[k-1] cotangents land on level k-1
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    if (L.v[s] && L.kb[s] > 0) fluxbar[s] += fpbn[s];
+}
+
+// forward implicit fall of a recorded species: returns the flux and the
+// species after the fall
+template <int S>
+__device__ __forceinline__ void fall_fwd(const Lay<S>& L, Shared<S>& sh, const double (&q)[S], const double (&vt)[S],
+                                         const double (&dz1)[S], const double (&dp1)[S], double dt,
+                                         const bool (&fall)[S], double (&qn)[S], double (&flux)[S]) {
+  double dz[S], dd[S], qq[S], ddp[S], a[S], b[S], qm[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    dz[s] = -dz1[s];
+    dd[s] = dt * vt[s];
+    qq[s] = q[s] * dp1[s];
+  }
+  col_prev(sh.buf, dd, ddp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double r = 1.0 / (dz[s] + dd[s]);
+    a[s] = ddp[s] * r;
+    b[s] = qq[s] * r;
+  }
+  scan_fwd(L, a, b, qm);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (fall[s]) {
+      flux[s] = dd[s] * qm[s];
+      qn[s] = qm[s] * dz[s] / dp1[s];
+    } else {
+      flux[s] = 0.0;
+      qn[s] = q[s];
+    }
+  }
+}
+
+
+// ------------------------------------------------------- taped forward
+template <int S>
+__device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X, Shared<S>& sh, const Glob<S>& G,
+                                              const shield_mp_fields& f, const shield_mp_params& P,
+                                              const Coef<double>& c, bool is_first, const TapeView<S>& T, int it,
+                                              int rb) {
+  const int lane = lane_id();
+  const double t_wfr = P.t_wfr;
+  if (is_first) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double t0 = L.v[s] ? f.pt[G.off[s]] : 250.0;
+      const double dz0 = L.v[s] ? f.delz[G.off[s]] : -100.0;
+      const double dz1 = dz0 * X.tz[s] / t0;
+      sh.cell[CL_DZ1][64 * s + lane] = dz1;
+      const double den = dry_density<double>(X.dp1[s], dz0) * dz0 / dz1;
+      sh.at(CL_DEN, s) = den;
+      sh.at(CL_DENFAC, s) = sqrt(SFCRHO / den);
+    }
+  }
+  tape_save(T, it, rb, X, sh);
+  bool pred[S], fall[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qr[s] > QRMIN);
+  col_any(L, pred, fall);
+#pragma unroll
+  for (int s = 0; s < S; ++s) X.vtr[s] = rain_fspeed<double>(c, !fall[s], X.qr[s], sh.at(CL_DEN, s));
+  double flux[S], surf[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) flux[s] = surf[s] = 0.0;
+  const double dt5 = 0.5 * c.dt_rain;
+  if (wave_any(fall)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (fall[s]) {
+        Cell cs = cell_of(X, s);
+        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][0], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
+        set_cell(X, s, cs);
+      }
+    }
+    tape_save(T, it, rb + 1, X, sh);
+    double dz[S];
+    layer_dz(L, sh, dz);
+    implicit_fall(L, X, sh, G, f, dz, X.qr, X.vtr, P.dt_rain, fall, false, flux, surf);
+    if (P.nml.do_sedi_heat) sedi_heat(L, X, sh, flux, C_LIQ, fall);
+    tape_save(T, it, rb + 2, X, sh);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (fall[s]) {
+        Cell cs = cell_of(X, s);
+        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][0], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
+        set_cell(X, s, cs);
+      }
+    }
+  } else {
+    tape_save(T, it, rb + 1, X, sh);
+    tape_save(T, it, rb + 2, X, sh);
+  }
+  tape_save(T, it, rb + 3, X, sh);
+  const double hv = sh.colv[CV_HVAR][0];
+  if (P.nml.irain_f != 0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      Cell cs = cell_of(X, s);
+      autoconv_no_subgrid_var<double>(c, t_wfr, cs, sh.at(CL_DEN, s), sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0]);
+      set_cell(X, s, cs);
+    }
+  } else {
+    double dl[S];
+    if (P.nml.z_slope_liq) slope_limiter(L, sh, X.ql, dl);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      Cell cs = cell_of(X, s);
+      const double d = P.nml.z_slope_liq ? mmax(dl[s], mmax(QVMIN, hv * cs.ql)) : mmax(QVMIN, hv * cs.ql);
+      autoconv_subgrid_var<double>(c, t_wfr, cs, sh.at(CL_DEN, s), sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0], d);
+      set_cell(X, s, cs);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (L.v[s] && L.kb[s] == 0) sh.colv[CV_RAIN][0] += surf[s];
+    double& m1 = sh.cell[CL_M1][64 * s + lane];
+    m1 = is_first ? m1 + flux[s] : m1 + flux[s] + sh.cell[CL_M1SOL][64 * s + lane];
+  }
+  __syncthreads();
+}
+
+template <int S>
+__device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Shared<S>& sh, const Glob<S>& G,
+                                        const shield_mp_fields& f, const shield_mp_params& P, const Coef<double>& c,
+                                        const TapeView<S>& T, int it) {
+  const int lane = lane_id();
+  const double tice_d = P.nml.tice;
+  tape_save(T, it, RS_F, X, sh);
+  double vti[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double vg, vi, vs;
+    fall_speed<double>(c, X.qg[s], X.qi[s], X.qs[s], X.tz[s] - tice_d, sh.at(CL_DEN, s), vg, vi, vs);
+    X.vtg[s] = vg;
+    vti[s] = vi;
+    X.vts[s] = vs;
+  }
+  bool pred[S], stop[S], fall[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.tz[s] > tice_d);
+  col_from_first(L, pred, stop);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (stop[s]) {
+      Cell cs = cell_of(X, s);
+      sedi_ice_melt<double>(c, tice_d, P.nml.ql_mlt, cs);
+      set_cell(X, s, cs);
+    }
+  }
+  tape_save(T, it, RS_G, X, sh);
+  double dz[S], flux[S], surf[S], m1_sol[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) m1_sol[s] = 0.0;
+  layer_dz(L, sh, dz);
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qi[s] > QRMIN);
+  col_any(L, pred, fall);
+  if (!(P.nml.vi_fac < 1.0e-5) && wave_any(fall)) {
+    implicit_fall(L, X, sh, G, f, dz, X.qi, vti, P.dts, fall, false, flux, surf);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      m1_sol[s] = flux[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_ICE][0] += surf[s];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qs[s] > QRMIN);
+  col_any(L, pred, fall);
+  if (wave_any(fall)) {
+    implicit_fall(L, X, sh, G, f, dz, X.qs, X.vts, P.dts, fall, false, flux, surf);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      m1_sol[s] = m1_sol[s] + flux[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_SNOW][0] += surf[s];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qg[s] > QRMIN);
+  col_any(L, pred, fall);
+  if (wave_any(fall)) {
+    implicit_fall(L, X, sh, G, f, dz, X.qg, X.vtg, P.dts, fall, false, flux, surf);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      m1_sol[s] = m1_sol[s] + flux[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_GRAUPEL][0] += surf[s];
+    }
+  }
+  __syncthreads();
+  tape_save(T, it, RS_H, X, sh);
+  if (P.nml.do_sedi_heat) {
+    bool all[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) all[s] = L.v[s];
+    sedi_heat(L, X, sh, m1_sol, C_ICE, all);
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) sh.cell[CL_M1SOL][64 * s + lane] = m1_sol[s];
+}
+
+
+// ----------------------------------------------------------- reverse steps
+// Register-side state of the reverse walk of one column.
+template <int S>
+struct AdjRegs {
+  CellBar<S> xb;     // cotangent of the cell values
+  double gp[AP_N];   // parameter gradient (this lane's share)
+};
+
+template <int S>
+__device__ __forceinline__ double& abar(AdjShared<S>& A, int which, int s) {
+  return A.bar[which][64 * s + lane_id()];
+}
+
+// Evaporation / accretion of rain (revap_racc) at record r, in falling
+// columns: inputs qv..tz (0-6), den (8), denfac (9).
+template <int S>
+__device__ __forceinline__ void revap_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Rec<S>& r,
+                                          const bool (&fall)[S], const Coef<Dual>& cd, double t_wfr, double dt5,
+                                          double hv) {
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!(fall[s] && L.v[s])) continue;
+    double yb[8], g[10];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
+#pragma unroll 1
+    for (int i = 0; i < 10; ++i) {
+      if (i == 7) continue;
+      CellT<Dual> cs = dual_cell(r, s, i);
+      revap_racc<Dual>(cd, t_wfr, Dual(dt5), Dual(hv), cs, seed(r.z[Z_DEN][s], i, 8),
+                       seed(r.z[Z_DENFAC][s], i, 9));
+      g[i] = cell_dot(yb, cs);
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = g[q];
+    abar(A, AB_DEN, s) += g[8];
+    abar(A, AB_DENFAC, s) += g[9];
+  }
+}
+
+// Autoconversion at record r (with the liquid slope limiter's dl): inputs
+// ql (1), qr (2), den (8), ccn (9), c_praut (10), dl (11), fac_rc (12).
+template <int S>
+__device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Rec<S>& r,
+                                             const shield_mp_params& P, const Coef<Dual>& cd0, double hv,
+                                             double ccn, double cpraut) {
+  Shared<S>& sh = A.sh;
+  const bool slope = P.nml.z_slope_liq != 0 && P.nml.irain_f == 0;
+  double ql[S], dl[S], dlbar[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    ql[s] = r.z[Z_QL][s];
+    dlbar[s] = 0.0;
+  }
+  if (slope) slope_limiter(L, sh, ql, dl);
+  double ccnbar = 0.0, cprautbar = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s]) continue;
+    double yb[8], g[13];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
+#pragma unroll 1
+    for (int i = 1; i < 13; ++i) {
+      if (i >= 3 && i <= 7) continue;
+      Coef<Dual> cd = cd0;
+      if (i == 12) cd.fac_rc.d = 1.0;
+      CellT<Dual> cs = dual_cell(r, s, i);
+      const Dual den = seed(r.z[Z_DEN][s], i, 8), cc = seed(ccn, i, 9), cp = seed(cpraut, i, 10);
+      if (P.nml.irain_f != 0) {
+        autoconv_no_subgrid_var<Dual>(cd, P.t_wfr, cs, den, cc, cp);
+      } else {
+        const Dual base = mmax(QVMIN, Dual(hv) * cs.ql);
+        const Dual d = slope ? mmax(seed(dl[s], i, 11), base) : base;
+        autoconv_subgrid_var<Dual>(cd, P.t_wfr, cs, den, cc, cp, d);
+      }
+      g[i] = cell_dot(yb, cs);
+    }
+    G.xb.v[1][s] = g[1];
+    G.xb.v[2][s] = g[2];
+    abar(A, AB_DEN, s) += g[8];
+    ccnbar += g[9];
+    cprautbar += g[10];
+    if (slope) dlbar[s] = g[11];
+    G.gp[AP_FAC_RC] += g[12];
+  }
+  ccnbar = wave_sum(ccnbar);
+  cprautbar = wave_sum(cprautbar);
+  if (lane_id() == 0) {
+    A.colbar[CB_CCN] += ccnbar;
+    A.colbar[CB_CPRAUT] += cprautbar;
+  }
+  __syncthreads();
+  if (slope) {
+    // dl[k] = slope_point(ql[k-1], ql[k], ql[k+1]) at interior levels
+    double qp[S], qn[S], gm[S], gp[S], gmn[S], gpp[S];
+    col_prev(sh.buf, ql, qp);
+    col_next(sh.buf, ql, qn);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      gm[s] = gp[s] = 0.0;
+      if (!L.v[s] || L.k[s] == 0 || L.kb[s] == 0 || dlbar[s] == 0.0) continue;
+      const Dual dm = slope_point<Dual>(Dual(qp[s], 1.0), Dual(ql[s]), Dual(qn[s]));
+      const Dual d0 = slope_point<Dual>(Dual(qp[s]), Dual(ql[s], 1.0), Dual(qn[s]));
+      const Dual dp = slope_point<Dual>(Dual(qp[s]), Dual(ql[s]), Dual(qn[s], 1.0));
+      G.xb.v[1][s] += dlbar[s] * d0.d;
+      gm[s] = dlbar[s] * dm.d;  // lands on level k-1
+      gp[s] = dlbar[s] * dp.d;  // lands on level k+1
+    }
+    col_next(sh.buf, gm, gmn);
+    col_prev(sh.buf, gp, gpp);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!L.v[s]) continue;
+      if (L.kb[s] > 0) G.xb.v[1][s] += gmn[s];
+      if (L.k[s] > 0) G.xb.v[1][s] += gpp[s];
+    }
+  }
+}
+
+// The reverse of one warm_rain call (records rb .. rb+3).
+template <int S>
+__device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
+                                              const shield_mp_fields& f, const shield_mp_params& P,
+                                              const Coef<double>& c, const Coef<Dual>& cd, bool is_first,
+                                              const TapeView<S>& T, int it, int rb, const double (&dp1)[S]) {
+  Shared<S>& sh = A.sh;
+  const double t_wfr = P.t_wfr;
+  const double dt5 = 0.5 * c.dt_rain;
+  const double hv = sh.colv[CV_HVAR][0];
+  Rec<S> rB;
+  tape_load(T, it, rb, rB);
+  bool pred[S], fall[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (rB.z[Z_QR][s] > QRMIN);
+  col_any(L, pred, fall);
+  // m1 += flux (+ m1_sol); rain += surface flux
+  double fluxbar[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double m1b = abar(A, AB_M1, s);
+    fluxbar[s] = m1b;
+    if (!is_first) abar(A, AB_M1SOL, s) += m1b;
+    if (L.v[s] && L.kb[s] == 0) fluxbar[s] += A.colbar[CB_RAIN];
+  }
+  {
+    Rec<S> rE;
+    tape_load(T, it, rb + 3, rE);
+    autoconv_adj(L, A, G, rE, P, cd, hv, sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0]);
+  }
+  if (wave_any(fall)) {
+    {
+      Rec<S> rD;
+      tape_load(T, it, rb + 2, rD);
+      revap_adj(L, A, G, rD, fall, cd, t_wfr, dt5, hv);
+    }
+    Rec<S> rC;
+    tape_load(T, it, rb + 1, rC);
+    double qr[S], vt[S], dz1[S], qn[S], flux[S], vtbar[S], dz1bar[S], dp1bar[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      qr[s] = rC.z[Z_QR][s];
+      vt[s] = rC.z[Z_VTR][s];
+      dz1[s] = rC.z[Z_DZ1][s];
+      vtbar[s] = dz1bar[s] = dp1bar[s] = 0.0;
+    }
+    fall_fwd(L, sh, qr, vt, dz1, dp1, P.dt_rain, fall, qn, flux);
+    if (P.nml.do_sedi_heat) {
+      Rec<S> st = rC;
+#pragma unroll
+      for (int s = 0; s < S; ++s) st.z[Z_QR][s] = qn[s];
+      heat_adj(L, sh, st, dp1, flux, C_LIQ, fall, G.xb, fluxbar, dp1bar, dz1bar);
+    }
+    fall_adj(L, sh, qr, vt, dz1, dp1, P.dt_rain, fall, G.xb.v[2], fluxbar, vtbar, dz1bar, dp1bar);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      abar(A, AB_VTR, s) += vtbar[s];
+      abar(A, AB_DZ1, s) += dz1bar[s];
+      abar(A, AB_DP1, s) += dp1bar[s];
+    }
+    revap_adj(L, A, G, rB, fall, cd, t_wfr, dt5, hv);
+  }
+  // rain fall speed: inputs qr, den, vr_fac
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double vb = abar(A, AB_VTR, s);
+    abar(A, AB_VTR, s) = 0.0;
+    if (!L.v[s] || vb == 0.0) continue;
+    Coef<Dual> cdv = cd;
+    cdv.vr_fac.d = 1.0;
+    const Dual vq = rain_fspeed<Dual>(cd, !fall[s], Dual(rB.z[Z_QR][s], 1.0), Dual(rB.z[Z_DEN][s]));
+    const Dual vd = rain_fspeed<Dual>(cd, !fall[s], Dual(rB.z[Z_QR][s]), Dual(rB.z[Z_DEN][s], 1.0));
+    const Dual vp = rain_fspeed<Dual>(cdv, !fall[s], Dual(rB.z[Z_QR][s]), Dual(rB.z[Z_DEN][s]));
+    G.xb.v[2][s] += vb * vq.d;
+    abar(A, AB_DEN, s) += vb * vd.d;
+    G.gp[AP_VR_FAC] += vb * vp.d;
+  }
+  if (is_first) {
+    // dz1 = dz0 tz / t0, den = den0 dz0 / dz1 = -dp1 / (g dz1), denfac = sqrt(sfcrho / den)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double dzb = abar(A, AB_DZ1, s), db = abar(A, AB_DEN, s), dfb = abar(A, AB_DENFAC, s);
+      abar(A, AB_DZ1, s) = abar(A, AB_DEN, s) = abar(A, AB_DENFAC, s) = 0.0;
+      if (!L.v[s]) continue;
+      const int32_t o = Gl.off[s];
+      const double t0 = f.pt[o], dz0 = f.delz[o], tz = rB.z[Z_TZ][s];
+      const double dz1 = rB.z[Z_DZ1][s], den = rB.z[Z_DEN][s], denfac = rB.z[Z_DENFAC][s];
+      const double denb = db - dfb * 0.5 * denfac / den;
+      const double dz1b = dzb - denb * den / dz1;
+      G.xb.v[6][s] += dz1b * dz0 / t0;
+      abar(A, AB_DZ0, s) += dz1b * tz / t0;
+      abar(A, AB_T0, s) -= dz1b * dz1 / t0;
+      abar(A, AB_DP1, s) += denb * den / dp1[s];
+    }
+  }
+}
+
+// The reverse of sedimentation (records RS_F, RS_G, RS_H).
+template <int S>
+__device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const shield_mp_params& P,
+                                        const Coef<double>& c, const Coef<Dual>& cd, const TapeView<S>& T, int it,
+                                        const double (&dp1)[S]) {
+  Shared<S>& sh = A.sh;
+  const double tice_d = P.nml.tice;
+  Rec<S> rF, rG;
+  tape_load(T, it, RS_F, rF);
+  tape_load(T, it, RS_G, rG);
+  double vti[S], dz1[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double vg, vi, vs;
+    fall_speed<double>(c, rF.z[Z_QG][s], rF.z[Z_QI][s], rF.z[Z_QS][s], rF.z[Z_TZ][s] - tice_d, rF.z[Z_DEN][s], vg,
+                       vi, vs);
+    vti[s] = vi;
+    dz1[s] = rG.z[Z_DZ1][s];
+  }
+  // the three falls from the post-melt state
+  bool pred[S], fi[S], fs[S], fg[S];
+  bool any_i, any_s, any_g;
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (rG.z[Z_QI][s] > QRMIN);
+  col_any(L, pred, fi);
+  any_i = !(P.nml.vi_fac < 1.0e-5) && wave_any(fi);
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (rG.z[Z_QS][s] > QRMIN);
+  col_any(L, pred, fs);
+  any_s = wave_any(fs);
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (rG.z[Z_QG][s] > QRMIN);
+  col_any(L, pred, fg);
+  any_g = wave_any(fg);
+  double qi[S], qs[S], qg[S], vts[S], vtg[S], qn[S], fl_i[S], fl_s[S], fl_g[S], m1s[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    qi[s] = rG.z[Z_QI][s];
+    qs[s] = rG.z[Z_QS][s];
+    qg[s] = rG.z[Z_QG][s];
+    vts[s] = rG.z[Z_VTS][s];
+    vtg[s] = rG.z[Z_VTG][s];
+    fl_i[s] = fl_s[s] = fl_g[s] = 0.0;
+  }
+  if (any_i) fall_fwd(L, sh, qi, vti, dz1, dp1, P.dts, fi, qn, fl_i);
+  if (any_s) fall_fwd(L, sh, qs, vts, dz1, dp1, P.dts, fs, qn, fl_s);
+  if (any_g) fall_fwd(L, sh, qg, vtg, dz1, dp1, P.dts, fg, qn, fl_g);
+  double fluxbar[S], dz1bar[S], dp1bar[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    m1s[s] = fl_i[s] + fl_s[s] + fl_g[s];
+    fluxbar[s] = abar(A, AB_M1SOL, s);
+    abar(A, AB_M1SOL, s) = 0.0;
+    dz1bar[s] = dp1bar[s] = 0.0;
+  }
+  if (P.nml.do_sedi_heat) {
+    Rec<S> rH;
+    tape_load(T, it, RS_H, rH);
+    bool all[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) all[s] = L.v[s];
+    heat_adj(L, sh, rH, dp1, m1s, C_ICE, all, G.xb, fluxbar, dp1bar, dz1bar);
+  }
+  // graupel, snow, ice falls in reverse order; each flux also carries its
+  // surface value into the column's precipitation
+  double vtbar[S];
+  if (any_g) {
+    double fb[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_GRAUPEL] : 0.0);
+      vtbar[s] = 0.0;
+    }
+    fall_adj(L, sh, qg, vtg, dz1, dp1, P.dts, fg, G.xb.v[5], fb, vtbar, dz1bar, dp1bar);
+#pragma unroll
+    for (int s = 0; s < S; ++s) abar(A, AB_VTG, s) += vtbar[s];
+  }
+  if (any_s) {
+    double fb[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_SNOW] : 0.0);
+      vtbar[s] = 0.0;
+    }
+    fall_adj(L, sh, qs, vts, dz1, dp1, P.dts, fs, G.xb.v[4], fb, vtbar, dz1bar, dp1bar);
+#pragma unroll
+    for (int s = 0; s < S; ++s) abar(A, AB_VTS, s) += vtbar[s];
+  }
+  double vtibar[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) vtibar[s] = 0.0;
+  if (any_i) {
+    double fb[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_ICE] : 0.0);
+    fall_adj(L, sh, qi, vti, dz1, dp1, P.dts, fi, G.xb.v[3], fb, vtibar, dz1bar, dp1bar);
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    abar(A, AB_DZ1, s) += dz1bar[s];
+    abar(A, AB_DP1, s) += dp1bar[s];
+  }
+  // ice melt at and below the first level warmer than tice (record F)
+  bool stop[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (rF.z[Z_TZ][s] > tice_d);
+  col_from_first(L, pred, stop);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!(stop[s] && L.v[s])) continue;
+    double yb[8], g[7];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
+#pragma unroll 1
+    for (int i = 0; i < 7; ++i) {
+      CellT<Dual> cs = dual_cell(rF, s, i);
+      sedi_ice_melt<Dual>(cd, tice_d, P.nml.ql_mlt, cs);
+      g[i] = cell_dot(yb, cs);
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = g[q];
+  }
+  // fall speeds (record F): inputs qg, qi, qs, tz, den and vs_fac, vg_fac, vi_fac
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double bg = abar(A, AB_VTG, s), bs = abar(A, AB_VTS, s), bi = vtibar[s];
+    abar(A, AB_VTG, s) = abar(A, AB_VTS, s) = 0.0;
+    if (!L.v[s]) continue;
+    double g[8];
+#pragma unroll 1
+    for (int i = 0; i < 8; ++i) {
+      Coef<Dual> cdf = cd;
+      if (i == 5) cdf.vs_fac.d = 1.0;
+      if (i == 6) cdf.vg_fac.d = 1.0;
+      if (i == 7) cdf.vi_fac.d = 1.0;
+      Dual vg, vi, vs;
+      fall_speed<Dual>(cdf, seed(rF.z[Z_QG][s], i, 0), seed(rF.z[Z_QI][s], i, 1), seed(rF.z[Z_QS][s], i, 2),
+                       seed(rF.z[Z_TZ][s], i, 3) - tice_d, seed(rF.z[Z_DEN][s], i, 4), vg, vi, vs);
+      g[i] = bg * vg.d + bi * vi.d + bs * vs.d;
+    }
+    G.xb.v[5][s] += g[0];
+    G.xb.v[3][s] += g[1];
+    G.xb.v[4][s] += g[2];
+    G.xb.v[6][s] += g[3];
+    abar(A, AB_DEN, s) += g[4];
+    G.gp[AP_VS_FAC] += g[5];
+    G.gp[AP_VG_FAC] += g[6];
+    G.gp[AP_VI_FAC] += g[7];
+  }
+}
+
+// The reverse of icloud (record RI): melt / freeze, then icloud_main with
+// subgrid_z_proc.  Inputs: the cell (0-6), den (8), denfac (9), vtg (10),
+// vtr (11), vts (12), p1 (13), qi_gen (14).
+template <int S>
+__device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
+                                           const shield_mp_fields& f, const shield_mp_params& P,
+                                           const Coef<Dual>& cd0, const TapeView<S>& T, int it,
+                                           const double (&dp1)[S]) {
+  Shared<S>& sh = A.sh;
+  Rec<S> r;
+  tape_load(T, it, RI, r);
+  const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr, ql_mlt = P.nml.ql_mlt;
+  const double hv = sh.colv[CV_HVAR][0], rh_adj = sh.colv[CV_RHADJ][0], rh_rain = sh.colv[CV_RHRAIN][0];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s]) continue;
+    const int32_t o = Gl.off[s];
+    const double t0 = f.pt[o], dz0 = f.delz[o];
+    const double p1 = dry_density<double>(dp1[s], dz0) * RDGAS * t0;
+    double yb[8], g[15];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
+#pragma unroll 1
+    for (int i = 0; i < 15; ++i) {
+      if (i == 7) continue;
+      Coef<Dual> cd = cd0;
+      if (i == 14) cd.qi_gen.d = 1.0;
+      CellT<Dual> cs = dual_cell(r, s, i);
+      const Dual den = seed(r.z[Z_DEN][s], i, 8);
+      Dual q_liq, q_sol, cvm;
+      icloud_melt_freeze<Dual>(cd, tice_d, t_wfr_d, ql_mlt, cs, den, q_liq, q_sol, cvm);
+      const Dual di = mmax(QVMIN, Dual(hv) * cs.qi);
+      icloud_main<Dual>(cd, tice_d, P.tice0, cs, Dual(hv), rh_adj, rh_rain, den, seed(r.z[Z_DENFAC][s], i, 9),
+                        seed(r.z[Z_VTG][s], i, 10), seed(r.z[Z_VTR][s], i, 11), seed(r.z[Z_VTS][s], i, 12),
+                        seed(p1, i, 13), di, q_liq, q_sol, cvm);
+      g[i] = cell_dot(yb, cs);
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = g[q];
+    abar(A, AB_DEN, s) += g[8];
+    abar(A, AB_DENFAC, s) += g[9];
+    abar(A, AB_VTG, s) += g[10];
+    abar(A, AB_VTR, s) += g[11];
+    abar(A, AB_VTS, s) += g[12];
+    G.gp[AP_QI_GEN] += g[14];
+    // p1 = -dp1 / (g dz0) rdgas t0
+    abar(A, AB_DP1, s) += g[13] * p1 / dp1[s];
+    abar(A, AB_DZ0, s) -= g[13] * p1 / dz0;
+    abar(A, AB_T0, s) += g[13] * p1 / t0;
+  }
+}
+
+
+// ------------------------------------------------------------------ kernel
+template <int S>
+__global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<double> c,
+                                                        shield_mp_fields f, shield_mp_adjoint_fields a,
+                                                        double* tape, int64_t tape_stride, double* part) {
+  __shared__ AdjShared<S> A;
+  Shared<S>& sh = A.sh;
+  const int lane = lane_id();
+  const int nz = (int)sh3.nk;
+  const int64_t ncol = sh3.ni * sh3.nj;
+  const shield_mp_namelist& n = P.nml;
+  const Coef<Dual> cd = dual_coef(c);
+  const TapeView<S> T{tape + (int64_t)blockIdx.x * tape_stride};
+  const double rdt = P.rdt;
+  double gp_acc[AP_N];
+#pragma unroll
+  for (int p = 0; p < AP_N; ++p) gp_acc[p] = 0.0;
+
+  for (int64_t col = blockIdx.x; col < ncol; col += gridDim.x) {
+    Lay<S> L;
+    Glob<S> Gl;
+    L.nz = nz;
+    L.ncols = 1;
+    const int64_t ci = col / sh3.nj, cj = col - ci * sh3.nj;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int idx = 64 * s + lane;
+      const bool v = idx < nz;
+      L.v[s] = v;
+      L.k[s] = v ? idx : 0;
+      L.kb[s] = v ? nz - 1 - idx : 0;
+      L.cl[s] = 0;
+      Gl.off[s] = (int32_t)(ci * sh3.si + cj * sh3.sj + L.k[s]);
+    }
+    const int64_t o2 = ci * sh3.si2 + cj * sh3.sj2;
+    // ------------------------------------------------------------ fields_init
+    if (lane == 0) {
+      const double land = f.land[o2], area = f.area[o2];
+      const double s_leng = sqrt(sqrt(area * 1.0e-10));
+      double h_var = n.dw_land * s_leng * land + n.dw_ocean * s_leng * (1.0 - land);
+      h_var = mmin(0.2, mmax(0.01, h_var));
+      const double rh_adj = 1.0 - h_var - n.rh_inc;
+      sh.colv[CV_HVAR][0] = h_var;
+      sh.colv[CV_RHADJ][0] = rh_adj;
+      sh.colv[CV_RHRAIN][0] = mmax(0.35, rh_adj - n.rh_inr);
+      sh.colv[CV_CCN][0] = (n.ccn_l * land + n.ccn_o * (1.0 - land)) * 1.0e6;
+      sh.colv[CV_RAIN][0] = sh.colv[CV_SNOW][0] = sh.colv[CV_ICE][0] = sh.colv[CV_GRAUPEL][0] = 0.0;
+    }
+    __syncthreads();
+    St<S, double> X;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (L.v[s]) {
+        const int32_t o = Gl.off[s];
+        X.tz[s] = f.pt[o];
+        const double dp0 = f.delp[o];
+        X.dp1[s] = dp0 * (1.0 - f.qv[o]);
+        const double omq = dp0 / X.dp1[s];
+        X.qv[s] = f.qv[o] * omq;
+        X.ql[s] = f.ql[o] * omq;
+        X.qr[s] = f.qr[o] * omq;
+        X.qi[s] = f.qi[o] * omq;
+        X.qs[s] = f.qs[o] * omq;
+        X.qg[s] = f.qg[o] * omq;
+        X.qaz[s] = n.reset_qaz ? 0.0 : f.qaz[o];
+      } else {
+        X.tz[s] = 250.0;
+        X.dp1[s] = 100.0;
+        X.qv[s] = X.ql[s] = X.qr[s] = X.qi[s] = X.qs[s] = X.qg[s] = X.qaz[s] = 0.0;
+      }
+      X.vtr[s] = X.vtg[s] = X.vts[s] = VF_MIN;
+    }
+    double p1_b = 0.0;  // p1 of the surface layer (held by the lane of that cell)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (L.v[s] && L.kb[s] == 0) {
+        double ccn = sh.colv[CV_CCN][0];
+        if (P.use_ccn) {
+          const int32_t o = Gl.off[s];
+          p1_b = dry_density<double>(X.dp1[s], f.delz[o]) * RDGAS * f.pt[o];
+          ccn = ccn * RDGAS * X.tz[s] / p1_b;
+        }
+        sh.colv[CV_CCN][0] = ccn;
+      }
+    }
+    __syncthreads();
+    if (lane == 0) sh.colv[CV_CPRAUT][0] = P.cpaut * pow(sh.colv[CV_CCN][0] * RHOR, -1.0 / 3.0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      sh.cell[CL_M1][64 * s + lane] = 0.0;
+      sh.cell[CL_M1SOL][64 * s + lane] = 0.0;
+      sh.cell[CL_DZ1][64 * s + lane] = 1.0;
+    }
+    __syncthreads();
+    // ------------------------------------------------ sub-steps, recorded
+    for (int it = 0; it < P.ntimes; ++it) {
+      warm_rain_rec(L, X, sh, Gl, f, P, c, true, T, it, RW1_B);
+      sed_rec(L, X, sh, Gl, f, P, c, T, it);
+      warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B);
+      tape_save(T, it, RI, X, sh);
+      icloud(L, X, sh, Gl, f, P, c);
+    }
+    __syncthreads();
+    // ----------------------------------------------------- reverse walk
+    AdjRegs<S> G;
+#pragma unroll
+    for (int p = 0; p < AP_N; ++p) G.gp[p] = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) G.xb.v[q][s] = 0.0;
+#pragma unroll
+      for (int b = 0; b < AB_N; ++b) abar(A, b, s) = 0.0;
+    }
+    const double convt = 86400.0 * rdt * RGRAV;
+    if (lane == 0) {
+      A.colbar[CB_RAIN] = a.rain ? a.rain[o2] * convt : 0.0;
+      A.colbar[CB_SNOW] = a.snow ? a.snow[o2] * convt : 0.0;
+      A.colbar[CB_ICE] = a.ice ? a.ice[o2] * convt : 0.0;
+      A.colbar[CB_GRAUPEL] = a.graupel ? a.graupel[o2] * convt : 0.0;
+      A.colbar[CB_CCN] = A.colbar[CB_CPRAUT] = 0.0;
+    }
+    double ptb[S], qinb[6][S], dp0b[S], uab[S], vab[S], qab[S];
+    // fields_update: tendencies of the final state
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      ptb[s] = dp0b[s] = uab[s] = vab[s] = qab[s] = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) qinb[q][s] = 0.0;
+      if (!L.v[s]) continue;
+      const int32_t o = Gl.off[s];
+      const double dp0 = f.delp[o], dp1 = X.dp1[s];
+      const double omq0 = dp0 / dp1, omq = dp1 / dp0;
+      const double xq[6] = {X.qv[s], X.ql[s], X.qr[s], X.qi[s], X.qs[s], X.qg[s]};
+      const double qin[6] = {f.qv[o], f.ql[o], f.qr[o], f.qi[o], f.qs[o], f.qg[o]};
+      const double gq[6] = {a.qv_dt[o], a.ql_dt[o], a.qr_dt[o], a.qi_dt[o], a.qs_dt[o], a.qg_dt[o]};
+      double omqb = 0.0, omq0b = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        G.xb.v[q][s] += gq[q] * rdt * omq;
+        qinb[q][s] -= gq[q] * rdt * omq0 * omq;
+        omqb += gq[q] * rdt * (xq[q] - qin[q] * omq0);
+        omq0b -= gq[q] * rdt * qin[q] * omq;
+      }
+      abar(A, AB_DP1, s) += omqb / dp0 - omq0b * omq0 / dp1;
+      dp0b[s] += omq0b / dp1 - omqb * omq / dp0;
+      const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
+                         (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
+      const double gt = a.pt_dt[o];
+      G.xb.v[6][s] += gt * rdt * cvm / CP_AIR;
+      ptb[s] -= gt * rdt * cvm / CP_AIR;
+      const double cvmb = gt * rdt * (X.tz[s] - f.pt[o]) / CP_AIR;
+      G.xb.v[0][s] += cvmb * P.c_vap;
+      G.xb.v[1][s] += cvmb * C_LIQ;
+      G.xb.v[2][s] += cvmb * C_LIQ;
+      G.xb.v[3][s] += cvmb * C_ICE;
+      G.xb.v[4][s] += cvmb * C_ICE;
+      G.xb.v[5][s] += cvmb * C_ICE;
+      if (!n.do_qa) {
+        const double ga = a.qa_dt[o];
+        G.xb.v[7][s] += ga * rdt / P.ntimes;
+        qab[s] -= ga * rdt;
+      }
+    }
+    if (n.sedi_transport) {
+      double m1[S], mp[S], al[S], bu[S], bv[S], u1[S], v1[S], an[S], gu[S], gv[S], nu[S], nv[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) m1[s] = sh.cell[CL_M1][64 * s + lane];
+      col_prev(sh.buf, m1, mp);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int32_t o = Gl.off[s];
+        const double u0 = L.v[s] ? f.ua[o] : 0.0, v0 = L.v[s] ? f.va[o] : 0.0;
+        if (L.k[s] == 0) {
+          al[s] = 0.0;
+          bu[s] = u0;
+          bv[s] = v0;
+        } else {
+          const double dp0 = L.v[s] ? f.delp[o] : 100.0;
+          const double r = 1.0 / (dp0 + mp[s]);
+          al[s] = mp[s] * r;
+          bu[s] = dp0 * u0 * r;
+          bv[s] = dp0 * v0 * r;
+        }
+        gu[s] = (L.v[s] && L.k[s] > 0) ? a.udt[o] * rdt : 0.0;
+        gv[s] = (L.v[s] && L.k[s] > 0) ? a.vdt[o] * rdt : 0.0;
+        uab[s] -= gu[s];
+        vab[s] -= gv[s];
+      }
+      scan_fwd(L, al, bu, u1);
+      scan_fwd(L, al, bv, v1);
+      col_next(sh.buf, al, an);
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (L.kb[s] == 0) an[s] = 0.0;
+      scan_bwd(L, an, gu, nu);
+      scan_bwd(L, an, gv, nv);
+      double u1p[S], v1p[S], mb[S], mbn[S];
+      col_prev(sh.buf, u1, u1p);
+      col_prev(sh.buf, v1, v1p);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        mb[s] = 0.0;
+        if (!L.v[s]) continue;
+        const int32_t o = Gl.off[s];
+        if (L.k[s] == 0) {
+          uab[s] += nu[s];
+          vab[s] += nv[s];
+        } else {
+          const double dp0 = f.delp[o], u0 = f.ua[o], v0 = f.va[o];
+          const double r = 1.0 / (dp0 + mp[s]);
+          uab[s] += nu[s] * dp0 * r;
+          vab[s] += nv[s] * dp0 * r;
+          dp0b[s] += (nu[s] * (u0 - u1[s]) + nv[s] * (v0 - v1[s])) * r;
+          mb[s] = (nu[s] * (u1p[s] - u1[s]) + nv[s] * (v1p[s] - v1[s])) * r;  // lands on level k-1
+        }
+      }
+      col_next(sh.buf, mb, mbn);
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (L.v[s] && L.kb[s] > 0) abar(A, AB_M1, s) += mbn[s];
+    }
+    __syncthreads();
+    // ------------------------------------------------ sub-steps, reversed
+    for (int it = P.ntimes - 1; it >= 0; --it) {
+      icloud_adj(L, A, G, Gl, f, P, cd, T, it, X.dp1);
+      __syncthreads();
+      warm_rain_adj(L, A, G, Gl, f, P, c, cd, false, T, it, RW2_B, X.dp1);
+      __syncthreads();
+      sed_adj(L, A, G, P, c, cd, T, it, X.dp1);
+      __syncthreads();
+      warm_rain_adj(L, A, G, Gl, f, P, c, cd, true, T, it, RW1_B, X.dp1);
+      __syncthreads();
+    }
+    // -------------------------------------------------- fields_init, reversed
+    const double ccn = sh.colv[CV_CCN][0], cpraut = sh.colv[CV_CPRAUT][0];
+    const double ccnbar = A.colbar[CB_CCN] - A.colbar[CB_CPRAUT] * cpraut / (3.0 * ccn);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!L.v[s]) continue;
+      const int32_t o = Gl.off[s];
+      const double dp0 = f.delp[o], qv = f.qv[o], dp1 = X.dp1[s];
+      const double omq = dp0 / dp1;
+      const double qin[6] = {qv, f.ql[o], f.qr[o], f.qi[o], f.qs[o], f.qg[o]};
+      double omqb = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        qinb[q][s] += G.xb.v[q][s] * omq;
+        omqb += G.xb.v[q][s] * qin[q];
+      }
+      ptb[s] += G.xb.v[6][s] + abar(A, AB_T0, s);
+      double dp1b = abar(A, AB_DP1, s) - omqb * omq / dp1;
+      dp0b[s] += omqb / dp1;
+      double dz0b = abar(A, AB_DZ0, s);
+      if (L.kb[s] == 0 && P.use_ccn) {
+        // ccn = ccn0 rdgas tz / p1, p1 = -dp1 / (g dz0) rdgas t0, with tz = t0 = pt
+        ptb[s] += ccnbar * ccn / f.pt[o];
+        const double p1b = -ccnbar * ccn / p1_b;
+        dp1b += p1b * p1_b / dp1;
+        dz0b -= p1b * p1_b / f.delz[o];
+        ptb[s] += p1b * p1_b / f.pt[o];
+      }
+      dp0b[s] += dp1b * (1.0 - qv);
+      qinb[0][s] -= dp1b * dp0;
+      a.pt[o] = ptb[s];
+      a.qv[o] = qinb[0][s];
+      a.ql[o] = qinb[1][s];
+      a.qr[o] = qinb[2][s];
+      a.qi[o] = qinb[3][s];
+      a.qs[o] = qinb[4][s];
+      a.qg[o] = qinb[5][s];
+      a.qa[o] = qab[s];
+      a.ua[o] = uab[s];
+      a.va[o] = vab[s];
+      a.delp[o] = dp0b[s];
+      a.delz[o] = dz0b;
+      if (a.qaz) a.qaz[o] = n.reset_qaz ? 0.0 : G.xb.v[7][s];
+    }
+#pragma unroll
+    for (int p = 0; p < AP_N; ++p) gp_acc[p] += G.gp[p];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < AP_N; ++p) {
+    const double v = wave_sum(gp_acc[p]);
+    if (lane == 0) part[(int64_t)blockIdx.x * AP_N + p] = v;
+  }
+}
+
+}  // namespace shield

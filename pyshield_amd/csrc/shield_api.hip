@@ -12,6 +12,9 @@ struct LayoutChoice {
   int S, P;
 };
 LayoutChoice choose_layout(int64_t nz, int mode, int ntimes, bool fused);
+int64_t adjoint_tape_doubles(int64_t nk, int ntimes);
+hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
+                          const shield_mp_adjoint_fields& a, void* ws, size_t ws_bytes, hipStream_t stream);
 }  // namespace shield
 
 using namespace shield;
@@ -173,17 +176,19 @@ const char* shield_strerror(int code) {
     case SHIELD_ERR_LAUNCH: return "HIP launch failed";
     case SHIELD_ERR_TIMESTEP: return "timestep must be positive and finite";
     case SHIELD_ERR_NOGPU: return "no HIP device is visible";
+    case SHIELD_ERR_ADJ_SWITCH:
+      return "the adjoint does not differentiate do_sedi_w, z_slope_ice, fix_negative or de_ice";
+    case SHIELD_ERR_WORKSPACE: return "adjoint workspace smaller than one wavefront's tape";
     default: return "unknown error";
   }
 }
 
-// Checks shared by shield_mp_forward and shield_mp_forward_update; 1 when
-// the domain is empty (nothing to launch).
-static int check_forward(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
-                         int mode, int* empty) {
+// Checks of the switches, shape and input fields shared by the forward and
+// adjoint entry points (shape, p and f non-NULL); *empty = 1 when the domain
+// is empty (nothing to launch).
+static int check_inputs(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                        int* empty) {
   *empty = 0;
-  if (!shape || !p || !f) return SHIELD_ERR_NULL;
-  if (mode != SHIELD_MP_FP64 && mode != SHIELD_MP_MIXED) return SHIELD_ERR_MODE;
   if (p->nml.use_ppm || p->nml.prog_ccn) return SHIELD_ERR_UNSUPPORTED;
   if (!(p->timestep > 0.0) || p->ntimes < 1) return SHIELD_ERR_TIMESTEP;
   if (!shape_ok(shape, 2, 320)) return SHIELD_ERR_SHAPE;
@@ -199,6 +204,20 @@ static int check_forward(const shield_mp_shape* shape, const shield_mp_params* p
                       f->land, f->area};
   for (const void* q : in)
     if (!q) return SHIELD_ERR_NULL;
+  return SHIELD_OK;
+}
+
+// Checks shared by shield_mp_forward and shield_mp_forward_update; 1 when
+// the domain is empty (nothing to launch).
+static int check_forward(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                         int mode, int* empty) {
+  *empty = 0;
+  if (!shape || !p || !f) return SHIELD_ERR_NULL;
+  if (mode != SHIELD_MP_FP64 && mode != SHIELD_MP_MIXED) return SHIELD_ERR_MODE;
+  const int rc = check_inputs(shape, p, f, empty);
+  if (rc != SHIELD_OK || *empty) return rc;
+  const void* in[] = {f->pt, f->qv, f->ql, f->qr, f->qi, f->qs, f->qg, f->qa, f->ua, f->va, f->delp, f->delz,
+                      f->land, f->area};
   const void* out[] = {f->pt_dt, f->qv_dt, f->ql_dt, f->qr_dt, f->qi_dt, f->qs_dt,
                        f->qg_dt, f->qa_dt, f->udt,   f->vdt,   f->qaz};
   for (const void* q : out)
@@ -229,6 +248,36 @@ int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p, c
   shield_update_fields none;
   memset(&none, 0, sizeof(none));
   const hipError_t e = launch_forward(*shape, *p, *f, none, mode, (hipStream_t)stream);
+  return e == hipSuccess ? SHIELD_OK : SHIELD_ERR_LAUNCH;
+}
+
+size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shield_mp_params* p, int64_t waves) {
+  if (!shape || !p || p->ntimes < 1 || waves < 1 || !shape_ok(shape, 2, 128)) return 0;
+  const int64_t ncol = shape->ni * shape->nj;
+  if (waves > ncol) waves = ncol < 1 ? 1 : ncol;
+  return (size_t)waves * (size_t)(adjoint_tape_doubles(shape->nk, p->ntimes) + SHIELD_MP_NPARAMS) * sizeof(double);
+}
+
+int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                      const shield_mp_adjoint_fields* a, void* ws, size_t ws_bytes, void* stream) {
+  if (!shape || !p || !f || !a || !ws) return SHIELD_ERR_NULL;
+  int empty = 0;
+  const int rc = check_inputs(shape, p, f, &empty);
+  if (rc != SHIELD_OK) return rc;
+  if (!f->qaz && !p->nml.reset_qaz) return SHIELD_ERR_NULL;
+  if (shape->nk > 128) return SHIELD_ERR_SHAPE;
+  if (p->do_sedi_w || p->nml.z_slope_ice || p->nml.fix_negative || (p->nml.de_ice && p->accumulate))
+    return SHIELD_ERR_ADJ_SWITCH;
+  if (empty) return SHIELD_OK;
+  const void* need[] = {a->pt_dt, a->qv_dt, a->ql_dt, a->qr_dt, a->qi_dt, a->qs_dt, a->qg_dt, a->qa_dt, a->udt,
+                        a->vdt,   a->pt,    a->qv,    a->ql,    a->qr,    a->qi,    a->qs,    a->qg,    a->qa,
+                        a->ua,    a->va,    a->delp,  a->delz,  a->params};
+  for (const void* q : need)
+    if (!q) return SHIELD_ERR_NULL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SHIELD_ERR_NOGPU;
+  if (ws_bytes < shield_mp_adjoint_workspace_bytes(shape, p, 1)) return SHIELD_ERR_WORKSPACE;
+  const hipError_t e = launch_adjoint(*shape, *p, *f, *a, ws, ws_bytes, (hipStream_t)stream);
   return e == hipSuccess ? SHIELD_OK : SHIELD_ERR_LAUNCH;
 }
 

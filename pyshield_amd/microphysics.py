@@ -278,3 +278,130 @@ def forward_update(fields: dict, update: dict, params, dt: float, rdt: float, q_
     rc = _lib.lib().shield_mp_forward_update(ctypes.byref(shape), ctypes.byref(params), ctypes.byref(f),
                                              ctypes.byref(u), int(mode), ctypes.c_void_p(stream))
     _lib.check(rc, "shield_mp_forward_update")
+
+
+# ---------------------------------------------------------------- adjoint
+AD_STATE_NAMES = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz")
+AD_OUT_NAMES = _lib.ADJ_OUT_NAMES
+
+
+def ad_params(params):
+    """The six differentiable process parameters of a shield_mp_params, in the
+    order of shield_mp_adjoint_fields.params: fac_rc (autoconversion
+    threshold qc0 = fac_rc ccn), vr_fac, vs_fac, vg_fac, vi_fac (fall-speed
+    coefficients), qi_gen (ice generation)."""
+    n = params.nml
+    return [params.fac_rc, n.vr_fac, n.vs_fac, n.vg_fac, n.vi_fac, n.qi_gen]
+
+
+def with_ad_params(params, values):
+    """A copy of ``params`` whose six process parameters are ``values``."""
+    p = _lib.ShieldMpParams()
+    ctypes.memmove(ctypes.byref(p), ctypes.byref(params), ctypes.sizeof(p))
+    v = [float(x) for x in values]
+    p.fac_rc = v[0]
+    p.nml.vr_fac, p.nml.vs_fac, p.nml.vg_fac, p.nml.vi_fac, p.nml.qi_gen = v[1:]
+    return p
+
+
+def adjoint(fields: dict, cotangents: dict, grads: dict, params, workspace=None, waves=2048, stream=None):
+    """Reverse-mode adjoint of one FP64 forward call (shield_mp_adjoint).
+    ``fields``: the forward's inputs (as for forward(); output entries are not
+    used).  ``cotangents``: cotangents of the outputs keyed by AD_OUT_NAMES
+    (the 2D precipitation entries may be missing).  ``grads``: tensors that
+    receive the input cotangents, keyed by AD_STATE_NAMES and "qaz"
+    (optional), and "params" (6 float64 values).  All 3D and 2D tensors share
+    the strides of fields["pt"] and fields["land"].  Returns the workspace
+    (reusable for the next call)."""
+    shape, f = _mp_args(fields)
+    ref3, ref2 = fields["pt"], fields["land"]
+    a = _lib.ShieldMpAdjointFields()
+    for n in AD_OUT_NAMES:
+        t = cotangents.get(n)
+        if t is not None:
+            ref = ref2 if n in FIELDS_2D else ref3
+            _check3("cotangent " + n, t, tuple(ref.shape))
+            if t.stride() != ref.stride():
+                raise ValueError(f"cotangent {n} must have the strides of the forward's fields")
+        setattr(a, "d_" + n, t.data_ptr() if t is not None else None)
+    for n in AD_STATE_NAMES + ("qaz",):
+        t = grads.get(n)
+        if t is not None:
+            _check3("gradient " + n, t, tuple(ref3.shape))
+            if t.stride() != ref3.stride():
+                raise ValueError(f"gradient {n} must have the strides of the forward's fields")
+        setattr(a, "g_" + n, t.data_ptr() if t is not None else None)
+    gp = grads["params"]
+    if gp.dtype != torch.float64 or gp.numel() != 6 or not gp.is_contiguous() or gp.device != ref3.device:
+        raise ValueError("grads['params'] must be a contiguous float64 tensor of 6 values on the fields' device")
+    a.params = gp.data_ptr()
+    L = _lib.lib()
+    nbytes = int(L.shield_mp_adjoint_workspace_bytes(ctypes.byref(shape), ctypes.byref(params), int(waves)))
+    if nbytes == 0:
+        raise _lib.ShieldError("shield_mp_adjoint: unsupported shape (1 <= ni * nj, 2 <= nk <= 128)")
+    if workspace is None or workspace.numel() * 8 < nbytes or workspace.device != ref3.device:
+        workspace = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=ref3.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(ref3.device).cuda_stream
+    rc = L.shield_mp_adjoint(ctypes.byref(shape), ctypes.byref(params), ctypes.byref(f), ctypes.byref(a),
+                             ctypes.c_void_p(workspace.data_ptr()), ctypes.c_size_t(workspace.numel() * 8),
+                             ctypes.c_void_p(stream))
+    _lib.check(rc, "shield_mp_adjoint")
+    return workspace
+
+
+class MicrophysicsFunction(torch.autograd.Function):
+    """One microphysics call as a differentiable function of the twelve state
+    inputs (AD_STATE_NAMES), the incoming qaz and the six process parameters
+    (a float64 tensor in the order of ad_params).  Returns the ten tendencies
+    and the four precipitation rates (AD_OUT_NAMES).  The forward runs the
+    HIP forward in ``mode``; the backward runs the HIP adjoint, which
+    recomputes the FP64 forward of every column and follows the branch it
+    takes at each point.  Tendencies are those of this call (overwritten,
+    not accumulated)."""
+
+    @staticmethod
+    def forward(ctx, params, mode, adp, land, area, qaz, *state):
+        p = with_ad_params(params, adp.detach().cpu().tolist())
+        p.accumulate = 0
+        ref = state[0]
+        ni, nj, nk = ref.shape
+        f = {n: t.detach() for n, t in zip(AD_STATE_NAMES, state)}
+        f.update(land=land.detach(), area=area.detach())
+        f["qaz"] = qaz.detach().clone()
+        f["w"] = torch.zeros_like(ref)
+        out = {n: torch.empty((ni, nj, nk) if n not in FIELDS_2D else (ni, nj), dtype=torch.float64,
+                              device=ref.device) for n in AD_OUT_NAMES}
+        f.update(out)
+        forward(f, p, mode)
+        ctx.p = p
+        ctx.save_for_backward(land, area, qaz, *state)
+        return tuple(out[n] for n in AD_OUT_NAMES)
+
+    @staticmethod
+    def backward(ctx, *gout):
+        land, area, qaz, *state = ctx.saved_tensors
+        ref = state[0]
+        f = {n: t.detach() for n, t in zip(AD_STATE_NAMES, state)}
+        f.update(land=land, area=area, qaz=qaz)
+        cot = {}
+        for n, g in zip(AD_OUT_NAMES, gout):
+            shp = tuple(land.shape) if n in FIELDS_2D else tuple(ref.shape)
+            cot[n] = (g if g is not None else torch.zeros(shp, dtype=torch.float64, device=ref.device)).contiguous()
+        grads = {n: torch.empty_like(ref) for n in AD_STATE_NAMES + ("qaz",)}
+        grads["params"] = torch.zeros(6, dtype=torch.float64, device=ref.device)
+        adjoint(f, cot, grads, ctx.p)
+        gadp = grads["params"].cpu()
+        return (None, None, gadp, None, None, grads["qaz"]) + tuple(grads[n] for n in AD_STATE_NAMES)
+
+
+def microphysics_ad(state: dict, params, adp=None, mode="fp64"):
+    """Differentiable microphysics call on a state dict (AD_STATE_NAMES plus
+    land, area and qaz, all [ni, nj, nk] / [ni, nj] float64 CUDA tensors with
+    k contiguous).  ``adp``: the six process parameters as a CPU float64
+    tensor (default: those of ``params``).  Returns a dict of AD_OUT_NAMES."""
+    if adp is None:
+        adp = torch.tensor(ad_params(params), dtype=torch.float64)
+    outs = MicrophysicsFunction.apply(params, mode, adp, state["land"], state["area"], state["qaz"],
+                                      *[state[n] for n in AD_STATE_NAMES])
+    return dict(zip(AD_OUT_NAMES, outs))

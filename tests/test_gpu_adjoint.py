@@ -1,0 +1,169 @@
+"""The hand-written adjoint (shield_mp_adjoint) against:
+
+* torch autograd through the FP64 torch transcription of the reference column
+  code (oracle/mp_numpy.py under TorchOps): the vector-Jacobian product for
+  random output cotangents, for the twelve input fields, qaz and the six
+  process parameters, at one and two sub-steps;
+* central finite differences of the FP64 GPU forward along random directions
+  (the tangent-linear / adjoint dot-product identity <J v, w> = <v, J^T w>);
+* the torch.autograd.Function wrapper.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import mp_numpy as M
+from pyshield_amd._config import PhysicsConfig
+
+from helpers import norm_err, small_state
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz")
+OUTS = ("pt_dt", "qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa_dt", "udt", "vdt",
+        "rain", "snow", "ice", "graupel")
+PARAMS = ("fac_rc", "vr_fac", "vs_fac", "vg_fac", "vi_fac", "qi_gen")
+# normalised max error of each input cotangent against autograd
+VJP_TOL = 1e-7
+
+
+def _cotangents(st, seed):
+    rng = np.random.default_rng(seed)
+    ni, nj, nz = st["pt"].shape
+    return {n: rng.normal(size=(ni, nj) if n in ("rain", "snow", "ice", "graupel") else (ni, nj, nz))
+            for n in OUTS}
+
+
+def _oracle_vjp(st, qaz, cfg, dt, cot):
+    """Input cotangents and parameter gradient by torch autograd of the oracle."""
+    import torch
+    flat = {k: (v.reshape(-1, v.shape[-1]) if v.ndim == 3 else v.reshape(-1)) for k, v in st.items()}
+    nml = cfg.mp_namelist()
+    ts = {k: torch.tensor(v, dtype=torch.float64, requires_grad=k in STATE) for k, v in flat.items()}
+    tq = torch.tensor(qaz.reshape(flat["pt"].shape), dtype=torch.float64, requires_grad=True)
+    pn = {k: torch.tensor(float(nml[k]), dtype=torch.float64, requires_grad=True) for k in PARAMS[1:]}
+    fac_rc = torch.tensor(M.derived(nml, dt)["fac_rc"], dtype=torch.float64, requires_grad=True)
+    out = M.run(dict(nml, **pn), dt, ts, qaz=tq, xp=M.TorchOps(), derived_overrides={"fac_rc": fac_rc})
+    loss = 0.0
+    for n in OUTS:
+        loss = loss + (out[n] * torch.tensor(cot[n].reshape(out[n].shape))).sum()
+    loss.backward()
+    shp = st["pt"].shape
+    g = {n: ts[n].grad.numpy().reshape(shp) for n in STATE}
+    g["qaz"] = tq.grad.numpy().reshape(shp)
+    g["params"] = np.array([float(fac_rc.grad)] + [float(pn[k].grad) for k in PARAMS[1:]])
+    return g
+
+
+def _gpu_fields(st, qaz):
+    import torch
+    dev = torch.device("cuda:0")
+
+    def t(x):
+        return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev).clone()
+
+    f = {n: t(st[n]) for n in STATE + ("land", "area")}
+    f["qaz"] = t(qaz)
+    f["w"] = torch.zeros_like(f["pt"])
+    return f
+
+
+def _gpu_vjp(st, qaz, cfg, dt, cot):
+    import torch
+    from pyshield_amd import _lib
+    from pyshield_amd.microphysics import adjoint
+    f = _gpu_fields(st, qaz)
+    params = _lib.setup(cfg.mp_namelist(), dt)
+    params.accumulate = 0
+    c = {n: torch.as_tensor(np.ascontiguousarray(v), dtype=torch.float64, device="cuda:0") for n, v in cot.items()}
+    g = {n: torch.full_like(f["pt"], np.nan) for n in STATE + ("qaz",)}
+    g["params"] = torch.zeros(6, dtype=torch.float64, device="cuda:0")
+    adjoint(f, c, g, params, waves=64)
+    torch.cuda.synchronize()
+    return {n: v.cpu().numpy() for n, v in g.items()}
+
+
+@pytest.mark.parametrize("dt,seed", [(150.0, 1000), (225.0, 2001)])
+def test_adjoint_matches_autograd_of_oracle(dt, seed):
+    print(f"case dt={dt} seed={seed}", flush=True)
+    st = small_state(2, 3, 79, seed=seed)
+    qaz = np.random.default_rng(seed).uniform(0.0, 1.0, st["pt"].shape)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79)
+    cot = _cotangents(st, seed + 1)
+    ref = _oracle_vjp(st, qaz, cfg, dt, cot)
+    got = _gpu_vjp(st, qaz, cfg, dt, cot)
+    errs = {n: norm_err(got[n], ref[n], floor=1e-300) for n in STATE + ("qaz",)}
+    perr = np.abs(got["params"] - ref["params"]) / np.maximum(np.abs(ref["params"]), 1e-300)
+    print({k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    print("params", dict(zip(PARAMS, [f"{x:.2e}/{y:.2e}" for x, y in zip(got["params"], ref["params"])])), flush=True)
+    assert all(np.isfinite(got[n]).all() for n in STATE + ("qaz",))
+    assert all(v <= VJP_TOL for v in errs.values()), errs
+    assert np.all(perr <= VJP_TOL), perr
+
+
+def test_adjoint_dot_product_with_finite_differences():
+    """<J v, w> from central differences of the GPU FP64 forward equals
+    <v, J^T w> from the adjoint, along smooth random directions."""
+    import torch
+    from pyshield_amd import _lib
+    from pyshield_amd.microphysics import forward
+    dt = 150.0
+    st = small_state(2, 3, 79, seed=77)
+    qaz = np.zeros(st["pt"].shape)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79)
+    cot = _cotangents(st, 5)
+    got = _gpu_vjp(st, qaz, cfg, dt, cot)
+    params = _lib.setup(cfg.mp_namelist(), dt)
+    params.accumulate = 0
+    rng = np.random.default_rng(11)
+    results = []
+    for trial in range(3):
+        # a direction scaled to each field (relative perturbation of every value)
+        v = {n: rng.normal(size=st[n].shape) * (np.abs(st[n]) + 1e-12) for n in STATE}
+
+        def loss(h):
+            s2 = dict(st)
+            for n in STATE:
+                s2[n] = st[n] + h * v[n]
+            f = _gpu_fields(s2, qaz)
+            ni, nj, nz = st["pt"].shape
+            for n in OUTS:
+                f[n] = torch.zeros((ni, nj) if n in ("rain", "snow", "ice", "graupel") else (ni, nj, nz),
+                                   dtype=torch.float64, device="cuda:0")
+            forward(f, params, "fp64")
+            torch.cuda.synchronize()
+            return sum(float((f[n].cpu().numpy() * cot[n]).sum()) for n in OUTS)
+
+        h = 1e-7
+        fd = (loss(h) - loss(-h)) / (2 * h)
+        ad = sum(float((got[n] * v[n]).sum()) for n in STATE)
+        results.append((fd, ad))
+        print(f"direction {trial}: finite difference {fd:.10e} adjoint {ad:.10e} rel {abs(fd - ad) / abs(ad):.1e}",
+              flush=True)
+    # central differences at h = 1e-7 carry ~1e-6 relative truncation and
+    # round-off; a process threshold crossed inside +-h shows as a larger gap
+    assert sum(abs(fd - ad) / abs(ad) <= 1e-4 for fd, ad in results) >= 2, results
+
+
+def test_autograd_function_matches_adjoint():
+    import torch
+    from pyshield_amd import _lib
+    from pyshield_amd.microphysics import AD_OUT_NAMES, AD_STATE_NAMES, microphysics_ad
+    dt = 150.0
+    st = small_state(2, 2, 79, seed=31)
+    qaz = np.zeros(st["pt"].shape)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79)
+    params = _lib.setup(cfg.mp_namelist(), dt)
+    f = _gpu_fields(st, qaz)
+    for n in AD_STATE_NAMES:
+        f[n].requires_grad_(True)
+    adp = torch.tensor([params.fac_rc, params.nml.vr_fac, params.nml.vs_fac, params.nml.vg_fac, params.nml.vi_fac,
+                        params.nml.qi_gen], dtype=torch.float64, requires_grad=True)
+    out = microphysics_ad(f, params, adp)
+    cot = _cotangents(st, 3)
+    loss = sum((out[n] * torch.as_tensor(cot[n], device="cuda:0")).sum() for n in AD_OUT_NAMES)
+    loss.backward()
+    ref = _gpu_vjp(st, qaz, cfg, dt, cot)
+    for n in AD_STATE_NAMES:
+        assert norm_err(f[n].grad.cpu().numpy(), ref[n], floor=1e-300) <= 1e-12, n
+    np.testing.assert_allclose(adp.grad.numpy(), ref["params"], rtol=1e-12)
