@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the second precision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the FP64 adjoint timing")
+    ap.add_argument("--adjoint-steps", type=int, default=2)
     ap.add_argument("--only-kernel", action="store_true",
                     help="launch only the timed kernel (for profiler runs)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -99,6 +101,29 @@ def time_mode(f, params, mode, steps, warmup):
     wall = time.perf_counter() - t0
     kernel_ms = [a.elapsed_time(b) for a, b in ev]
     return wall, kernel_ms
+
+
+def time_adjoint(f, params, steps):
+    """The FP64 adjoint (which replays the forward of every column) for unit
+    output cotangents; returns the kernel times in ms (HIP events) and the
+    allocated bytes."""
+    import torch
+    from pyshield_amd.microphysics import AD_OUT_NAMES, AD_STATE_NAMES, adjoint
+    ref3, ref2 = f["pt"], f["land"]
+    cot = {n: torch.ones_like(ref2 if n in ("rain", "snow", "ice", "graupel") else ref3) for n in AD_OUT_NAMES}
+    grads = {n: torch.empty_like(ref3) for n in AD_STATE_NAMES + ("qaz",)}
+    grads["params"] = torch.zeros(6, dtype=torch.float64, device=ref3.device)
+    ws = adjoint(f, cot, grads, params, waves=2048)  # warm-up
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        adjoint(f, cot, grads, params, workspace=ws, waves=2048)
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    return ms
 
 
 def cpu_baseline(name, seconds):
@@ -173,6 +198,19 @@ def main():
                  "mixed_over_fp64": (k2 / k_ms) if args.mode == "mixed" else (k_ms / k2)}
         print(f"# {other}: kernel {k2:.3f} ms", file=sys.stderr, flush=True)
 
+    if not args.no_adjoint and not args.only_kernel:
+        p64 = _lib.setup(cfg.mp_namelist(), w.timestep)
+        p64.accumulate = 0
+        ams = float(np.mean(time_adjoint(f, p64, args.adjoint_steps)))
+        # forward inputs re-read (13 F), output cotangents read (10 F), input
+        # cotangents written (13 F); per column land, area and 4 precipitation cotangents
+        adj_bytes = ncol * (w.nz * 8 * (13 + 10 + 13) + 8 * 6)
+        extra.update({"adjoint_kernel_ms": ams, "adjoint_columns_per_s_fp64": ncol / (ams * 1e-3),
+                      "fwd_plus_adjoint_columns_per_s": ncol / ((k_ms + ams) * 1e-3),
+                      "adjoint_algorithmic_bytes": adj_bytes,
+                      "adjoint_roofline_frac": adj_bytes / (ams * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        print(f"# adjoint (fp64): kernel {ams:.3f} ms", file=sys.stderr, flush=True)
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"mp_forward_{w.name}_{args.mode}_traffic.json")
     if os.path.exists(tfile):
@@ -206,7 +244,7 @@ def main():
         "cpu_baseline": cpu,
         "kernel_ms": k_ms,
         "algorithmic_bytes_per_launch": alg_bytes,
-        "adjoint_columns_per_s": None,
+        "adjoint_columns_per_s": extra.get("adjoint_columns_per_s_fp64"),
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

@@ -37,6 +37,8 @@ def test_struct_sizes_match_layout():
     assert ctypes.sizeof(_lib.ShieldMpNamelist) == 37 * 8 + 20 * 4
     assert ctypes.sizeof(_lib.ShieldMpShape) == 7 * 8
     assert ctypes.sizeof(_lib.ShieldMpFields) == len(_lib.FIELD_NAMES) * 8
+    # 14 output cotangents, 13 input cotangents, the parameter gradient
+    assert ctypes.sizeof(_lib.ShieldMpAdjointFields) == (14 + 13 + 1) * 8
     # 21 pointers, two int64 strides, three doubles
     assert ctypes.sizeof(_lib.ShieldUpdateFields) == 21 * 8 + 2 * 8 + 3 * 8
     # 31 pointers, two int64 strides, one double
@@ -168,9 +170,56 @@ def test_forward_rejects_before_launch():
     assert fwd(ctypes.byref(empty), ctypes.byref(p), ctypes.byref(_fake_fields()), 0, None) == 0
 
 
+def _fake_adjoint():
+    a = _lib.ShieldMpAdjointFields()
+    for i, (n, _) in enumerate(a._fields_):
+        setattr(a, n, 0x200000 + 0x1000 * i)
+    return a
+
+
+def test_adjoint_rejects_before_launch():
+    """shield_mp_adjoint validates its arguments before any device call;
+    with valid arguments and no GPU it reports SHIELD_ERR_NOGPU."""
+    L = _lib.lib()
+    adj = L.shield_mp_adjoint
+    p = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
+    shape = _lib.ShieldMpShape(2, 2, 79, 2 * 79, 79, 2, 1)
+    f, a, ws = _fake_fields(), _fake_adjoint(), ctypes.c_void_p(0x900000)
+
+    def call(shape=shape, p=p, f=f, a=a, ws=ws, nbytes=1 << 30):
+        return adj(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(a), ws, nbytes, None)
+
+    assert call(ws=None) == -1
+    a2 = _fake_adjoint()
+    a2.d_pt_dt = None
+    assert call(a=a2) == -1
+    # switches the adjoint does not differentiate
+    for field, value in (("do_sedi_w", 1),):
+        p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
+        setattr(p2, field, value)
+        assert call(p=p2) == -9
+    for field in ("z_slope_ice", "fix_negative"):
+        p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
+        setattr(p2.nml, field, 1)
+        assert call(p=p2) == -9
+    # nk above the adjoint's two slots
+    assert call(shape=_lib.ShieldMpShape(2, 2, 200, 400, 200, 2, 1)) == -2
+    # empty domain is a no-op
+    assert call(shape=_lib.ShieldMpShape(0, 4, 79, 0, 79, 0, 1)) == 0
+    # everything valid, no device here
+    assert call() == -8
+    # workspace: one tape of ntimes x 12 records x 14 fields x 128 cells per
+    # wavefront, plus six partial sums; bounded by the number of columns
+    per_wave = (1 * 12 * 14 * 128 + 6) * 8
+    ws_bytes = L.shield_mp_adjoint_workspace_bytes
+    assert ws_bytes(ctypes.byref(shape), ctypes.byref(p), 3) == 3 * per_wave
+    assert ws_bytes(ctypes.byref(shape), ctypes.byref(p), 100) == 4 * per_wave
+    assert ws_bytes(ctypes.byref(_lib.ShieldMpShape(2, 2, 200, 400, 200, 2, 1)), ctypes.byref(p), 3) == 0
+
+
 def test_strerror_and_layout():
     L = _lib.lib()
-    for code in range(-8, 1):
+    for code in range(-10, 1):
         assert L.shield_strerror(code)
     # without a device the layout with the most busy lanes wins
     assert _lib.layout(79) == (4, 5)       # 4 columns in 5 slots: 316 of 320 lanes busy
