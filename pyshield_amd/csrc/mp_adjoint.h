@@ -13,20 +13,24 @@
 //  1. replays the forward of its column (the FP64 forward's building blocks,
 //     in the forward's order) and records the per-cell state at 12 points of
 //     every sub-step in its own region of the workspace;
-//  2. walks the steps backwards.  A point-wise step (density update, fall
-//     speeds, evaporation / accretion, autoconversion, ice melt, icloud)
-//     gets its vector-Jacobian product from the same process code run on
-//     dual numbers, one pass per input, so every derivative follows the
-//     branch the forward took at the point.  The vertical steps have
-//     hand-written transposes: the implicit fall and the sedimentation
-//     heating (backward affine scans), the liquid slope limiter, the
-//     momentum transport, and the precipitation and flux sums.
+//  2. walks the steps backwards.  The icloud step (ice melt / freeze, the
+//     ice-phase collections and the saturation adjustment of subgrid_z_proc)
+//     has a hand-written reverse (mp_icloud_adj.h).  The other point-wise
+//     steps (density update, fall speeds, evaporation / accretion,
+//     autoconversion, sedimentation ice melt) get their vector-Jacobian
+//     products from the same process code run on dual numbers, one pass per
+//     input, so every derivative follows the branch the forward took at the
+//     point.  The vertical steps have hand-written transposes: the implicit
+//     fall and the sedimentation heating (backward affine scans), the liquid
+//     slope limiter, the momentum transport, and the precipitation and flux
+//     sums.
 // Supported switches: the default ones, plus the const_v*, irain_f,
 // z_slope_liq, do_sedi_heat, sedi_transport, do_qa and reset_qaz variants;
 // do_sedi_w, z_slope_ice, fix_negative and de_ice are rejected by the host.
 #pragma once
 
 #include "mp_forward.h"
+#include "mp_icloud_adj.h"
 
 namespace shield {
 
@@ -895,12 +899,11 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
 template <int S>
 __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
                                            const shield_mp_fields& f, const shield_mp_params& P,
-                                           const Coef<Dual>& cd0, const TapeView<S>& T, int it,
+                                           const Coef<double>& c, const TapeView<S>& T, int it,
                                            const double (&dp1)[S]) {
   Shared<S>& sh = A.sh;
   Rec<S> r;
   tape_load(T, it, RI, r);
-  const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr, ql_mlt = P.nml.ql_mlt;
   const double hv = sh.colv[CV_HVAR][0], rh_adj = sh.colv[CV_RHADJ][0], rh_rain = sh.colv[CV_RHRAIN][0];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -908,36 +911,28 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
     const int32_t o = Gl.off[s];
     const double t0 = f.pt[o], dz0 = f.delz[o];
     const double p1 = dry_density<double>(dp1[s], dz0) * RDGAS * t0;
-    double yb[8], g[15];
+    const IcK k{c, P.nml.tice, P.t_wfr, P.nml.ql_mlt, P.tice0, hv, rh_adj, rh_rain,
+                {r.z[Z_DEN][s], r.z[Z_DENFAC][s], r.z[Z_VTG][s], r.z[Z_VTR][s], r.z[Z_VTS][s], p1}};
+    double yb[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
-#pragma unroll 1
-    for (int i = 0; i < 15; ++i) {
-      if (i == 7) continue;
-      Coef<Dual> cd = cd0;
-      if (i == 14) cd.qi_gen.d = 1.0;
-      CellT<Dual> cs = dual_cell(r, s, i);
-      const Dual den = seed(r.z[Z_DEN][s], i, 8);
-      Dual q_liq, q_sol, cvm;
-      icloud_melt_freeze<Dual>(cd, tice_d, t_wfr_d, ql_mlt, cs, den, q_liq, q_sol, cvm);
-      const Dual di = mmax(QVMIN, Dual(hv) * cs.qi);
-      icloud_main<Dual>(cd, tice_d, P.tice0, cs, Dual(hv), rh_adj, rh_rain, den, seed(r.z[Z_DENFAC][s], i, 9),
-                        seed(r.z[Z_VTG][s], i, 10), seed(r.z[Z_VTR][s], i, 11), seed(r.z[Z_VTS][s], i, 12),
-                        seed(p1, i, 13), di, q_liq, q_sol, cvm);
-      g[i] = cell_dot(yb, cs);
-    }
+    IcInB ib{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    icloud_vjp(k,
+               Cell{r.z[Z_QV][s], r.z[Z_QL][s], r.z[Z_QR][s], r.z[Z_QI][s], r.z[Z_QS][s], r.z[Z_QG][s],
+                    r.z[Z_TZ][s], r.z[Z_QA][s]},
+               yb, ib);
 #pragma unroll
-    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = g[q];
-    abar(A, AB_DEN, s) += g[8];
-    abar(A, AB_DENFAC, s) += g[9];
-    abar(A, AB_VTG, s) += g[10];
-    abar(A, AB_VTR, s) += g[11];
-    abar(A, AB_VTS, s) += g[12];
-    G.gp[AP_QI_GEN] += g[14];
+    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = yb[q];
+    abar(A, AB_DEN, s) += ib.den;
+    abar(A, AB_DENFAC, s) += ib.denfac;
+    abar(A, AB_VTG, s) += ib.vtg;
+    abar(A, AB_VTR, s) += ib.vtr;
+    abar(A, AB_VTS, s) += ib.vts;
+    G.gp[AP_QI_GEN] += ib.qi_gen;
     // p1 = -dp1 / (g dz0) rdgas t0
-    abar(A, AB_DP1, s) += g[13] * p1 / dp1[s];
-    abar(A, AB_DZ0, s) -= g[13] * p1 / dz0;
-    abar(A, AB_T0, s) += g[13] * p1 / t0;
+    abar(A, AB_DP1, s) += ib.p1 * p1 / dp1[s];
+    abar(A, AB_DZ0, s) -= ib.p1 * p1 / dz0;
+    abar(A, AB_T0, s) += ib.p1 * p1 / t0;
   }
 }
 
@@ -1167,7 +1162,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
     __syncthreads();
     // ------------------------------------------------ sub-steps, reversed
     for (int it = P.ntimes - 1; it >= 0; --it) {
-      icloud_adj(L, A, G, Gl, f, P, cd, T, it, X.dp1);
+      icloud_adj(L, A, G, Gl, f, P, c, T, it, X.dp1);
       __syncthreads();
       warm_rain_adj(L, A, G, Gl, f, P, c, cd, false, T, it, RW2_B, X.dp1);
       __syncthreads();

@@ -13,6 +13,10 @@
 
 #include "../../include/shield_mp.h"
 
+// The constants, math and point-wise process functions are host-and-device
+// so a host build can check the process code and its derivatives on the CPU.
+#define SH_FN __host__ __device__ __forceinline__
+
 namespace shield {
 
 // ndsl.constants, GFS set
@@ -67,23 +71,23 @@ constexpr double QRMIN = 1.0e-8;
 constexpr double QVMIN = 1.0e-20;
 
 // ---------------------------------------------------------------- math
-__device__ __forceinline__ double mexp(double x) { return exp(x); }
-__device__ __forceinline__ double mlog(double x) { return log(x); }
-__device__ __forceinline__ double msqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ double mpow(double x, double y) { return pow(x, y); }
-__device__ __forceinline__ double mabs(double x) { return fabs(x); }
-__device__ __forceinline__ float mexp(float x) { return __expf(x); }
-__device__ __forceinline__ float mlog(float x) { return __logf(x); }
+SH_FN double mexp(double x) { return exp(x); }
+SH_FN double mlog(double x) { return log(x); }
+SH_FN double msqrt(double x) { return sqrt(x); }
+SH_FN double mpow(double x, double y) { return pow(x, y); }
+SH_FN double mabs(double x) { return fabs(x); }
+SH_FN float mexp(float x) { return __expf(x); }
+SH_FN float mlog(float x) { return __logf(x); }
 // v_sqrt_f32 (1 ulp); the FP32 rate laws need no correctly rounded root
-__device__ __forceinline__ float msqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float mpow(float x, float y) { return __expf(y * __logf(x)); }
-__device__ __forceinline__ float mabs(float x) { return fabsf(x); }
+SH_FN float msqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+SH_FN float mpow(float x, float y) { return __expf(y * __logf(x)); }
+SH_FN float mabs(float x) { return fabsf(x); }
 // Powers x^p (x > 0, p a literal).  pw_el is the reference's exp(p*log(x)),
 // pw its x**p.  In FP64 they are exactly those expressions.  In FP32 the power
 // is split as x^n * 2^((p-n) log2 x) with n the integer nearest p: rounding
 // the exponent then costs |p-n| |log2 x| ulp instead of |p| |ln x| ulp (for
 // x^(7/3) at x = 1e-4, 3 ulp instead of 21).
-__device__ __forceinline__ float pw_split(float x, double p) {
+SH_FN float pw_split(float x, double p) {
   const double nd = __builtin_rint(p);
   const float r = float(p - nd);
   float y = __builtin_amdgcn_exp2f(r * __builtin_amdgcn_logf(x));
@@ -94,17 +98,17 @@ __device__ __forceinline__ float pw_split(float x, double p) {
   return y;
 }
 template <class R>
-__device__ __forceinline__ R pw_el(R x, double p);
+SH_FN R pw_el(R x, double p);
 template <>
-__device__ __forceinline__ double pw_el<double>(double x, double p) { return exp(p * log(x)); }
+SH_FN double pw_el<double>(double x, double p) { return exp(p * log(x)); }
 template <>
-__device__ __forceinline__ float pw_el<float>(float x, double p) { return pw_split(x, p); }
+SH_FN float pw_el<float>(float x, double p) { return pw_split(x, p); }
 template <class R>
-__device__ __forceinline__ R pw(R x, double p);
+SH_FN R pw(R x, double p);
 template <>
-__device__ __forceinline__ double pw<double>(double x, double p) { return pow(x, p); }
+SH_FN double pw<double>(double x, double p) { return pow(x, p); }
 template <>
-__device__ __forceinline__ float pw<float>(float x, double p) { return pw_split(x, p); }
+SH_FN float pw<float>(float x, double p) { return pw_split(x, p); }
 
 // a / b for the FP64 parts of the process code.  FP64 mode keeps the IEEE
 // division of the reference's expression; mixed mode multiplies by a
@@ -112,7 +116,7 @@ __device__ __forceinline__ float pw<float>(float x, double p) { return pw_split(
 // shares between divisions by the same value (lhl / cvm and lhi / cvm, ...).
 // Measured on gfx950 over b in [1e-3, 1e7]: v_rcp_f64 is within 4.6e-8
 // relative, and one Newton step brings it to 2.2e-15 (two give the same).
-__device__ __forceinline__ double rcp64(double b) {
+SH_FN double rcp64(double b) {
   const double r = __builtin_amdgcn_rcp(b);
   return fma(fma(-b, r, 1.0), r, r);
 }
@@ -131,14 +135,14 @@ template <class R>
 using wide_t = typename Wide<R>::type;
 
 template <class R>
-__device__ __forceinline__ wide_t<R> fdiv(wide_t<R> a, wide_t<R> b) {
+SH_FN wide_t<R> fdiv(wide_t<R> a, wide_t<R> b) {
   if constexpr (sizeof(R) == 4)
     return a * rcp64(b);
   else
     return a / b;
 }
 template <class R>
-__device__ __forceinline__ wide_t<R> recip(wide_t<R> b) {
+SH_FN wide_t<R> recip(wide_t<R> b) {
   if constexpr (sizeof(R) == 4)
     return rcp64(b);
   else
@@ -156,7 +160,7 @@ __device__ __forceinline__ wide_t<R> recip(wide_t<R> b) {
 //  * log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
 //    s = (m - 1) / (m + 1), |s| <= 0.172, summed to s^21 (truncation < 3e-17);
 //    x must be positive and normal.
-__device__ __forceinline__ double exp_sat(double x) {
+SH_FN double exp_sat(double x) {
   constexpr double LOG2E = 1.4426950408889634, LN2_HI = 6.93147180369123816490e-01,
                    LN2_LO = 1.90821492927058770002e-10;
   const double n = __builtin_rint(x * LOG2E);
@@ -175,7 +179,7 @@ __device__ __forceinline__ double exp_sat(double x) {
   p = fma(p, r, 1.0);
   return __builtin_ldexp(p, int(n));
 }
-__device__ __forceinline__ double log_sat(double x) {
+SH_FN double log_sat(double x) {
   constexpr double LN2 = 6.93147180559945286227e-01, RSQRT2 = 0.70710678118654752440;
   int e;
   double m = __builtin_frexp(x, &e);  // [0.5, 1)
@@ -199,20 +203,20 @@ __device__ __forceinline__ double log_sat(double x) {
 }
 
 template <class T>
-__device__ __forceinline__ T mmin(T a, T b) { return a < b ? a : b; }
+SH_FN T mmin(T a, T b) { return a < b ? a : b; }
 template <class T>
-__device__ __forceinline__ T mmax(T a, T b) { return a > b ? a : b; }
+SH_FN T mmax(T a, T b) { return a > b ? a : b; }
 // v_min / v_max for the floating-point cases: one instruction instead of a
 // compare and a select per 32-bit half.  They differ from the selects only
 // for NaN operands and in the sign of a zero result, which no result of the
 // kernel depends on.
-__device__ __forceinline__ float mmin(float a, float b) { return __builtin_fminf(a, b); }
-__device__ __forceinline__ float mmax(float a, float b) { return __builtin_fmaxf(a, b); }
-__device__ __forceinline__ double mmin(double a, double b) { return __builtin_fmin(a, b); }
-__device__ __forceinline__ double mmax(double a, double b) { return __builtin_fmax(a, b); }
+SH_FN float mmin(float a, float b) { return __builtin_fminf(a, b); }
+SH_FN float mmax(float a, float b) { return __builtin_fmaxf(a, b); }
+SH_FN double mmin(double a, double b) { return __builtin_fmin(a, b); }
+SH_FN double mmax(double a, double b) { return __builtin_fmax(a, b); }
 // microphysics_funcs.py:45-49
 template <class T>
-__device__ __forceinline__ T dim(T x, T y) {
+SH_FN T dim(T x, T y) {
   T diff = x - y;
   return diff > T(0) ? diff : T(0);
 }
@@ -253,7 +257,7 @@ static __shared__ float coef_tab_lds[T_N];
 // Coefficient-array entry i (a constant after inlining).  In mixed precision
 // it is read from the LDS copy, in FP64 from the kernel argument.
 template <class R>
-__device__ __forceinline__ R ck(const Coef<R>& c, int i) {
+SH_FN R ck(const Coef<R>& c, int i) {
   if constexpr (sizeof(R) == 4)
     return coef_tab_lds[i];
   else

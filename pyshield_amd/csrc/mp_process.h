@@ -53,7 +53,7 @@ struct is_f32<float> {
 // the cloud water leaves ql < 0, and qv follows), and there the reference's
 // transfer (0, or f q / (1 + f)) stays as it is.
 template <class R>
-__device__ __forceinline__ wide_t<R> cap(wide_t<R> x, wide_t<R> q) {
+SH_FN wide_t<R> cap(wide_t<R> x, wide_t<R> q) {
   if constexpr (is_f32<R>::value) return (x > q && x <= q * (1.0 + 1.0e-5)) ? q : x;
   return x;
 }
@@ -63,19 +63,19 @@ __device__ __forceinline__ wide_t<R> cap(wide_t<R> x, wide_t<R> q) {
 
 // ------------------------------------------------ saturation (f:53-145)
 template <class R>
-__device__ __forceinline__ R wqs1(R ta, R den) {
+SH_FN R wqs1(R ta, R den) {
   return (R(E00) * mexp((R(DC_VAP) * mlog(ta / R(TICE)) + R(LV0) * (ta - R(TICE)) / (ta * R(TICE))) /
                         R(RVGAS))) /
          (R(RVGAS) * ta * den);
 }
 template <class R>
-__device__ __forceinline__ R wqs2(R ta, R den, R& dqdt) {
+SH_FN R wqs2(R ta, R den, R& dqdt) {
   R tmp = wqs1(ta, den);
   dqdt = tmp * (R(DC_VAP) + R(LV0) / ta) / (R(RVGAS) * ta);
   return tmp;
 }
 template <class R>
-__device__ __forceinline__ R iqs1(R ta, R den) {
+SH_FN R iqs1(R ta, R den) {
   R tmp;
   if (ta < R(TICE)) {
     if (ta >= R(T_SAT_MIN)) {
@@ -97,7 +97,7 @@ __device__ __forceinline__ R iqs1(R ta, R den) {
   return tmp;
 }
 template <class R>
-__device__ __forceinline__ R iqs2(R ta, R den, R& dqdt) {
+SH_FN R iqs2(R ta, R den, R& dqdt) {
   R tmp = iqs1(ta, den);
   if (ta < R(TICE)) {
     if (ta >= R(T_SAT_MIN))
@@ -120,7 +120,7 @@ __device__ __forceinline__ R iqs2(R ta, R den, R& dqdt) {
 // mp_common.h (~1e-14 relative).  Below T_SAT_MIN and above
 // TICE + 102 the reference's branches are the same expression at the clamped
 // temperature.
-__device__ __forceinline__ double qs_fast(double t, double d, double l, double den, double& dqdt) {
+SH_FN double qs_fast(double t, double d, double l, double den, double& dqdt) {
   constexpr double rtice = 1.0 / TICE, rrv = 1.0 / RVGAS;
   const double rt = rcp64(t);
   const double e = E00 * exp_sat(d * rrv * log_sat(t * rtice) + l * rrv * (rtice - rt));
@@ -128,7 +128,7 @@ __device__ __forceinline__ double qs_fast(double t, double d, double l, double d
   dqdt = q * (d + l * rt) * rrv * rt;
   return q;
 }
-__device__ __forceinline__ double iqs_fast(double ta, double den, double& dqdt) {
+SH_FN double iqs_fast(double ta, double den, double& dqdt) {
   const bool ice = ta < TICE;
   const double t = ice ? (ta < T_SAT_MIN ? T_SAT_MIN : ta) : (ta > TICE + 102.0 ? TICE + 102.0 : ta);
   return qs_fast(t, ice ? D2ICE : DC_VAP, ice ? LI2 : LV0, den, dqdt);
@@ -136,23 +136,23 @@ __device__ __forceinline__ double iqs_fast(double ta, double den, double& dqdt) 
 // the reference's forms in FP64 mode (and for the adjoint's dual numbers),
 // the folded ones in mixed mode
 template <class R>
-__device__ __forceinline__ wide_t<R> qs_w(wide_t<R> ta, wide_t<R> den, wide_t<R>& dqdt) {
+SH_FN wide_t<R> qs_w(wide_t<R> ta, wide_t<R> den, wide_t<R>& dqdt) {
   if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, dqdt);
   else return wqs2<wide_t<R>>(ta, den, dqdt);
 }
 template <class R>
-__device__ __forceinline__ wide_t<R> qs_w1(wide_t<R> ta, wide_t<R> den) {
+SH_FN wide_t<R> qs_w1(wide_t<R> ta, wide_t<R> den) {
   wide_t<R> g;
   if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, g);
   else return wqs1<wide_t<R>>(ta, den);
 }
 template <class R>
-__device__ __forceinline__ wide_t<R> qs_i(wide_t<R> ta, wide_t<R> den, wide_t<R>& dqdt) {
+SH_FN wide_t<R> qs_i(wide_t<R> ta, wide_t<R> den, wide_t<R>& dqdt) {
   if constexpr (is_f32<R>::value) return iqs_fast(ta, den, dqdt);
   else return iqs2<wide_t<R>>(ta, den, dqdt);
 }
 template <class R>
-__device__ __forceinline__ wide_t<R> qs_i1(wide_t<R> ta, wide_t<R> den) {
+SH_FN wide_t<R> qs_i1(wide_t<R> ta, wide_t<R> den) {
   wide_t<R> g;
   if constexpr (is_f32<R>::value) return iqs_fast(ta, den, g);
   else return iqs1<wide_t<R>>(ta, den);
@@ -160,21 +160,21 @@ __device__ __forceinline__ wide_t<R> qs_i1(wide_t<R> ta, wide_t<R> den) {
 
 // ------------------------------------------------ f:149-179
 template <class R>
-__device__ __forceinline__ R acr3d(R v1, R v2, R q1, R q2, R c, R cac_ik, R cac_i1k, R cac_i2k, R rho) {
+SH_FN R acr3d(R v1, R v2, R q1, R q2, R c, R cac_ik, R cac_i1k, R cac_i2k, R rho) {
   R t1 = msqrt(q1 * rho);
   R s1 = msqrt(q2 * rho);
   R s2 = msqrt(s1);
   return c * mabs(v1 - v2) * q1 * s2 * (cac_ik * t1 + cac_i1k * msqrt(t1) * s2 + cac_i2k * s1);
 }
 template <class R>
-__device__ __forceinline__ R smlt(const Coef<R>& c, R tc, R dqs, R qsrho, R psacw, R psacr, R rho, R rhofac) {
+SH_FN R smlt(const Coef<R>& c, R tc, R dqs, R qsrho, R psacw, R psacr, R rho, R rhofac) {
   const int k = T_CSMLT;
   return (ck(c, k) * tc / rho - ck(c, k + 1) * dqs) *
              (ck(c, k + 2) * msqrt(qsrho) + ck(c, k + 3) * pw<R>(qsrho, 0.65625) * msqrt(rhofac)) +
          ck(c, k + 4) * tc * (psacw + psacr);
 }
 template <class R>
-__device__ __forceinline__ R gmlt(const Coef<R>& c, R tc, R dqs, R qgrho, R pgacw, R pgacr, R rho) {
+SH_FN R gmlt(const Coef<R>& c, R tc, R dqs, R qgrho, R pgacw, R pgacr, R rho) {
   const int k = T_CGMLT;
   return (ck(c, k) * tc / rho - ck(c, k + 1) * dqs) *
              (ck(c, k + 2) * msqrt(qgrho) + ck(c, k + 3) * pw<R>(qgrho, 0.6875) / pw<R>(rho, 0.25)) +
@@ -183,7 +183,7 @@ __device__ __forceinline__ R gmlt(const Coef<R>& c, R tc, R dqs, R qgrho, R pgac
 
 // ------------------------------------------------ f:183-265
 template <class R>
-__device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, CellR<R>& s,
+SH_FN void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, CellR<R>& s,
                                            wide_t<R> den_d, R denfac) {
   using W = wide_t<R>;
   if ((s.tz > t_wfr) && (s.qr > QRMIN)) {
@@ -230,7 +230,7 @@ __device__ __forceinline__ void revap_racc(const Coef<R>& c, double t_wfr, R dt,
 
 // ------------------------------------------------ f:269-341
 template <class R>
-__device__ __forceinline__ void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R tc, R den, R& vtg, R& vti,
+SH_FN void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R tc, R den, R& vtg, R& vti,
                                            R& vts) {
   R rhof = msqrt(mmin(R(10.0), R(SFCRHO) / den));
   if (c.const_vi) {
@@ -265,7 +265,7 @@ __device__ __forceinline__ void fall_speed(const Coef<R>& c, R qg, R qi, R qs, R
 
 // f:344-372 (the r1 half is handled by the caller)
 template <class R>
-__device__ __forceinline__ R rain_fspeed(const Coef<R>& c, bool no_fall, R qrz, R den) {
+SH_FN R rain_fspeed(const Coef<R>& c, bool no_fall, R qrz, R den) {
   if (no_fall) return R(VF_MIN);
   if (c.const_vr) return c.vr_fac;
   R qden = qrz * den;
@@ -276,7 +276,7 @@ __device__ __forceinline__ R rain_fspeed(const Coef<R>& c, bool no_fall, R qrz, 
 
 // ------------------------------------------------ f:375-428
 template <class R>
-__device__ __forceinline__ void autoconv_no_subgrid_var(const Coef<R>& c, double t_wfr, CellR<R>& s, R den, R ccn,
+SH_FN void autoconv_no_subgrid_var(const Coef<R>& c, double t_wfr, CellR<R>& s, R den, R ccn,
                                                         R c_praut) {
   using W = wide_t<R>;
   R qc0 = c.fac_rc * ccn;
@@ -292,7 +292,7 @@ __device__ __forceinline__ void autoconv_no_subgrid_var(const Coef<R>& c, double
   }
 }
 template <class R>
-__device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_wfr, CellR<R>& s, R den, R ccn,
+SH_FN void autoconv_subgrid_var(const Coef<R>& c, double t_wfr, CellR<R>& s, R den, R ccn,
                                                      R c_praut, R dl) {
   using W = wide_t<R>;
   R qc0 = c.fac_rc * ccn;
@@ -313,7 +313,7 @@ __device__ __forceinline__ void autoconv_subgrid_var(const Coef<R>& c, double t_
 
 // ------------------------------------------------ f:431-850
 template <class R>
-__device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var, double rh_adj,
+SH_FN void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var, double rh_adj,
                                                double rh_rain, wide_t<R> den_d, R denfac, wide_t<R> p1) {
   using W = wide_t<R>;
   const R tice = c.tice, t_wfr = c.t_wfr, dts = c.dts;
@@ -571,7 +571,7 @@ __device__ __forceinline__ void subgrid_z_proc(const Coef<R>& c, double tice_d, 
 // Melting of cloud ice in a layer at or below the first level warmer than
 // tice (the caller selects the layers).
 template <class R>
-__device__ __forceinline__ void sedi_ice_melt(const Coef<R>& c, double tice_d, double ql_mlt, CellR<R>& s) {
+SH_FN void sedi_ice_melt(const Coef<R>& c, double tice_d, double ql_mlt, CellR<R>& s) {
   using W = wide_t<R>;
   const W tcd = s.tz - tice_d;
   if ((s.qi > QCMIN) && (tcd > 0.0)) {
@@ -598,7 +598,7 @@ __device__ __forceinline__ void sedi_ice_melt(const Coef<R>& c, double tice_d, d
 // cloud water below t_wfr (pihom); returns the heat-capacity terms the rest of
 // icloud continues from.
 template <class R>
-__device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice_d, double t_wfr_d, double ql_mlt,
+SH_FN void icloud_melt_freeze(const Coef<R>& c, double tice_d, double t_wfr_d, double ql_mlt,
                                                    CellR<R>& s, wide_t<R> den, wide_t<R>& q_liq, wide_t<R>& q_sol,
                                                    wide_t<R>& cvm) {
   using W = wide_t<R>;
@@ -635,7 +635,7 @@ __device__ __forceinline__ void icloud_melt_freeze(const Coef<R>& c, double tice
 
 // ------------------------------------------------ f:853-1260
 template <class R>
-__device__ __forceinline__ void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>& s, R h_var,
+SH_FN void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>& s, R h_var,
                                             double rh_adj, double rh_rain, wide_t<R> den_d, R denfac, R vtgz, R vtrz,
                                             R vtsz, wide_t<R> p1d, R di, wide_t<R> q_liq, wide_t<R> q_sol,
                                             wide_t<R> cvm) {
