@@ -15,7 +15,7 @@
 //     every sub-step in its own region of the workspace;
 //  2. walks the steps backwards.  The icloud step (ice melt / freeze, the
 //     ice-phase collections and the saturation adjustment of subgrid_z_proc)
-//     has a hand-written reverse (mp_icloud_adj.h).  The other point-wise
+//     has a hand-written reverse (mp_point_adj.h).  The other point-wise
 //     steps (density update, fall speeds, evaporation / accretion,
 //     autoconversion, sedimentation ice melt) get their vector-Jacobian
 //     products from the same process code run on dual numbers, one pass per
@@ -30,7 +30,7 @@
 #pragma once
 
 #include "mp_forward.h"
-#include "mp_icloud_adj.h"
+#include "mp_point_adj.h"
 
 namespace shield {
 
@@ -539,37 +539,39 @@ __device__ __forceinline__ double& abar(AdjShared<S>& A, int which, int s) {
 }
 
 // Evaporation / accretion of rain (revap_racc) at record r, in falling
-// columns: inputs qv..tz (0-6), den (8), denfac (9).
+// columns (hand-written reverse, mp_point_adj.h).
 template <int S>
 __device__ __forceinline__ void revap_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Rec<S>& r,
-                                          const bool (&fall)[S], const Coef<Dual>& cd, double t_wfr, double dt5,
+                                          const bool (&fall)[S], const Coef<double>& c, double t_wfr, double dt5,
                                           double hv) {
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!(fall[s] && L.v[s])) continue;
-    double yb[8], g[10];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
-#pragma unroll 1
-    for (int i = 0; i < 10; ++i) {
-      if (i == 7) continue;
-      CellT<Dual> cs = dual_cell(r, s, i);
-      revap_racc<Dual>(cd, t_wfr, Dual(dt5), Dual(hv), cs, seed(r.z[Z_DEN][s], i, 8),
-                       seed(r.z[Z_DENFAC][s], i, 9));
-      g[i] = cell_dot(yb, cs);
-    }
-#pragma unroll
-    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = g[q];
-    abar(A, AB_DEN, s) += g[8];
-    abar(A, AB_DENFAC, s) += g[9];
+    IcX xb{G.xb.v[0][s], G.xb.v[1][s], G.xb.v[2][s], G.xb.v[3][s], G.xb.v[4][s], G.xb.v[5][s], G.xb.v[6][s],
+           G.xb.v[7][s], 0.0, 0.0, 0.0};
+    double denb = 0.0, denfacb = 0.0;
+    revap_vjp(c, t_wfr, dt5, hv,
+              Cell{r.z[Z_QV][s], r.z[Z_QL][s], r.z[Z_QR][s], r.z[Z_QI][s], r.z[Z_QS][s], r.z[Z_QG][s], r.z[Z_TZ][s],
+                   r.z[Z_QA][s]},
+              r.z[Z_DEN][s], r.z[Z_DENFAC][s], xb, denb, denfacb);
+    G.xb.v[0][s] = xb.qv;
+    G.xb.v[1][s] = xb.ql;
+    G.xb.v[2][s] = xb.qr;
+    G.xb.v[3][s] = xb.qi;
+    G.xb.v[4][s] = xb.qs;
+    G.xb.v[5][s] = xb.qg;
+    G.xb.v[6][s] = xb.tz;
+    abar(A, AB_DEN, s) += denb;
+    abar(A, AB_DENFAC, s) += denfacb;
   }
 }
 
-// Autoconversion at record r (with the liquid slope limiter's dl): inputs
-// ql (1), qr (2), den (8), ccn (9), c_praut (10), dl (11), fac_rc (12).
+// Autoconversion at record r, with the liquid slope limiter's dl (hand-written
+// reverse, mp_point_adj.h): cotangents of ql, qr, den, ccn, c_praut, dl and
+// fac_rc.
 template <int S>
 __device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Rec<S>& r,
-                                             const shield_mp_params& P, const Coef<Dual>& cd0, double hv,
+                                             const shield_mp_params& P, const Coef<double>& c, double hv,
                                              double ccn, double cpraut) {
   Shared<S>& sh = A.sh;
   const bool slope = P.nml.z_slope_liq != 0 && P.nml.irain_f == 0;
@@ -584,32 +586,26 @@ __device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, A
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!L.v[s]) continue;
-    double yb[8], g[13];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
-#pragma unroll 1
-    for (int i = 1; i < 13; ++i) {
-      if (i >= 3 && i <= 7) continue;
-      Coef<Dual> cd = cd0;
-      if (i == 12) cd.fac_rc.d = 1.0;
-      CellT<Dual> cs = dual_cell(r, s, i);
-      const Dual den = seed(r.z[Z_DEN][s], i, 8), cc = seed(ccn, i, 9), cp = seed(cpraut, i, 10);
-      if (P.nml.irain_f != 0) {
-        autoconv_no_subgrid_var<Dual>(cd, P.t_wfr, cs, den, cc, cp);
-      } else {
-        const Dual base = mmax(QVMIN, Dual(hv) * cs.ql);
-        const Dual d = slope ? mmax(seed(dl[s], i, 11), base) : base;
-        autoconv_subgrid_var<Dual>(cd, P.t_wfr, cs, den, cc, cp, d);
-      }
-      g[i] = cell_dot(yb, cs);
+    double qlb = G.xb.v[1][s], qrb = G.xb.v[2][s];
+    AcB ab{0.0, 0.0, 0.0, 0.0, 0.0};
+    const double tz = r.z[Z_TZ][s], qlv = r.z[Z_QL][s], den = r.z[Z_DEN][s];
+    if (P.nml.irain_f != 0) {
+      autoconv_vjp(c, P.t_wfr, false, tz, qlv, den, ccn, cpraut, 0.0, qlb, qrb, ab);
+    } else {
+      const double base = mmax(QVMIN, hv * qlv);
+      const bool use_dl = slope && (dl[s] > base);
+      autoconv_vjp(c, P.t_wfr, true, tz, qlv, den, ccn, cpraut, use_dl ? dl[s] : base, qlb, qrb, ab);
+      if (use_dl)
+        dlbar[s] = ab.dl;
+      else if (!(QVMIN > hv * qlv))
+        qlb += ab.dl * hv;
     }
-    G.xb.v[1][s] = g[1];
-    G.xb.v[2][s] = g[2];
-    abar(A, AB_DEN, s) += g[8];
-    ccnbar += g[9];
-    cprautbar += g[10];
-    if (slope) dlbar[s] = g[11];
-    G.gp[AP_FAC_RC] += g[12];
+    G.xb.v[1][s] = qlb;
+    G.xb.v[2][s] = qrb;
+    abar(A, AB_DEN, s) += ab.den;
+    ccnbar += ab.ccn;
+    cprautbar += ab.c_praut;
+    G.gp[AP_FAC_RC] += ab.fac_rc;
   }
   ccnbar = wave_sum(ccnbar);
   cprautbar = wave_sum(cprautbar);
@@ -673,13 +669,13 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
   {
     Rec<S> rE;
     tape_load(T, it, rb + 3, rE);
-    autoconv_adj(L, A, G, rE, P, cd, hv, sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0]);
+    autoconv_adj(L, A, G, rE, P, c, hv, sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0]);
   }
   if (wave_any(fall)) {
     {
       Rec<S> rD;
       tape_load(T, it, rb + 2, rD);
-      revap_adj(L, A, G, rD, fall, cd, t_wfr, dt5, hv);
+      revap_adj(L, A, G, rD, fall, c, t_wfr, dt5, hv);
     }
     Rec<S> rC;
     tape_load(T, it, rb + 1, rC);
@@ -705,7 +701,7 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
       abar(A, AB_DZ1, s) += dz1bar[s];
       abar(A, AB_DP1, s) += dp1bar[s];
     }
-    revap_adj(L, A, G, rB, fall, cd, t_wfr, dt5, hv);
+    revap_adj(L, A, G, rB, fall, c, t_wfr, dt5, hv);
   }
   // rain fall speed: inputs qr, den, vr_fac
 #pragma unroll

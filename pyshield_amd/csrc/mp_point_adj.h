@@ -1,9 +1,11 @@
-// mp_icloud_adj.h -- hand-written reverse mode of the FP64 icloud step of one
-// cell: icloud_melt_freeze (microphysics.py:1290-1338), icloud_main
-// (microphysics_funcs.py:853-1260, without subgrid_z_proc) and subgrid_z_proc
-// (microphysics_funcs.py:431-850).
+// mp_point_adj.h -- hand-written reverse mode of FP64 point-wise processes of
+// one cell: the icloud step -- icloud_melt_freeze (microphysics.py:1290-1338),
+// icloud_main (microphysics_funcs.py:853-1260, without subgrid_z_proc) and
+// subgrid_z_proc (microphysics_funcs.py:431-850) -- and, in warm_rain, the
+// evaporation / accretion of rain revap_racc (microphysics_funcs.py:183-265)
+// and the autoconversion of cloud water (microphysics_funcs.py:375-428).
 //
-// The step is cut into blocks (one process each: a melt, a collection, a
+// The icloud step is cut into blocks (one process each: a melt, a collection, a
 // deposition, ...).  Every block reads the cell state X at its entry -- the
 // eight cell values and the running q_liq, q_sol, cvm -- and derives its latent
 // heats, their ratios to cvm, tc and dqs0 from X, so X is the whole interface
@@ -1684,5 +1686,201 @@ SH_FN void icloud_vjp(const IcK& k, const Cell& s, double (&yb)[8], IcInB& ib, C
   yb[5] = xb.qg;
   yb[6] = xb.tz;
   yb[7] = xb.qa;
+}
+// ---------------------------------------------------------------- warm_rain
+// Vector-Jacobian product of revap_racc (evaporation of rain below
+// saturation, accretion of cloud water by rain) for one cell.  s: the cell
+// before the call; xb: in, the cotangents of the cell after it (qv .. tz;
+// q_liq, q_sol, cvm zero); out, before it.  Adds den's and denfac's
+// cotangents.
+SH_FN void revap_vjp(const Coef<double>& c, double t_wfr, double dt, double h_var, const Cell& s, double den,
+                     double denfac, IcX& xb, double& denb, double& denfacb) {
+  if (!((s.tz > t_wfr) && (s.qr > QRMIN))) return;
+  const double lhl = c.lv00_d + c.d0_vap_d * s.tz;
+  const double q_liq = s.ql + s.qr;
+  const double q_sol = s.qi + s.qs + s.qg;
+  const double cvm = ic_cvm(c, s.qv, q_liq, q_sol);
+  const double lcpk = lhl / cvm;
+  const double tin = s.tz - lcpk * s.ql;
+  const double qpz = s.qv + s.ql;
+  double dqsdt;
+  const double qsat = wqs2<double>(tin, den, dqsdt);
+  const double mq = mmax(qpz, QCMIN);
+  const double dqh0 = mmax(s.ql, h_var * mq);
+  const double dqh = mmin(dqh0, 0.2 * qpz);
+  const double dqv = qsat - s.qv;
+  const double q_minus = qpz - dqh;
+  const double q_plus = qpz + dqh;
+  const bool ev_on = (dqv > QVMIN) && (qsat > q_minus);
+  double dq = 0.0, qden = 0.0, t2 = 0.0, Bq = 0.0, Dd = 0.0, evap = 0.0, a = 0.0, Dn = 0.0, b = 0.0, ev = 0.0;
+  double pe7 = 0.0, cvm1 = 0.0;
+  double qr1 = s.qr;
+  if (ev_on) {
+    if (qsat > q_plus)
+      dq = qsat - qpz;
+    else
+      dq = 0.25 * ((q_minus - qsat) * (q_minus - qsat)) / dqh;
+    qden = s.qr * den;
+    t2 = tin * tin;
+    pe7 = pw_el<double>(qden, 0.725);
+    Bq = ck(c, T_CREVP + 1) * sqrt(qden) + ck(c, T_CREVP + 2) * pe7;
+    Dd = ck(c, T_CREVP + 3) * t2 + ck(c, T_CREVP + 4) * qsat * den;
+    evap = ck(c, T_CREVP + 0) * t2 * dq * Bq / Dd;
+    a = dt * evap;
+    Dn = 1.0 + lcpk * dqsdt;
+    b = dqv / Dn;
+    ev = mmin(s.qr, mmin(a, b));
+    qr1 = s.qr - ev;
+    cvm1 = ic_cvm(c, s.qv + ev, q_liq - ev, q_sol);
+  }
+  // accretion of cloud water by rain
+  if ((qr1 > QRMIN) && (s.ql > 1.0e-6) && (qsat < q_minus)) {
+    const double pe = pw_el<double>(qr1 * den, 0.95);
+    const double sink = dt * denfac * c.cracw * pe;
+    const double D = 1.0 + sink;
+    const double skb = xb.qr - xb.ql;
+    xb.ql += skb * sink / D;
+    const double sinkb = skb * s.ql / (D * D);
+    denfacb += sinkb * dt * c.cracw * pe;
+    const double qdb = sinkb * dt * denfac * c.cracw * 0.95 * pe / (qr1 * den);
+    xb.qr += qdb * den;
+    denb += qdb * qr1;
+  }
+  double lcpkb = 0.0, lhlb = 0.0, tinb = 0.0, qpzb = 0.0, dqhb = 0.0, qsatb = 0.0, dqsdtb = 0.0;
+  if (ev_on) {
+    double evb = heat_rev(c, -1.0, ev, lhl, cvm1, xb, lhlb);
+    evb += xb.qv - xb.qr - xb.q_liq;
+    const double m = mmin(a, b);
+    double mb = 0.0;
+    if (s.qr < m)
+      xb.qr += evb;
+    else
+      mb = evb;
+    double evapb = 0.0, dqvb = 0.0;
+    if (a < b) {
+      evapb = mb * dt;
+    } else {
+      dqvb += mb / Dn;
+      const double Dnb = -mb * b / Dn;
+      lcpkb += Dnb * dqsdt;
+      dqsdtb += Dnb * lcpk;
+    }
+    const double Nb = evapb / Dd, Ddb = -evapb * evap / Dd;
+    const double c0 = ck(c, T_CREVP + 0);
+    const double t2b = Nb * c0 * dq * Bq + Ddb * ck(c, T_CREVP + 3);
+    const double dqb = Nb * c0 * t2 * Bq;
+    const double Bqb = Nb * c0 * t2 * dq;
+    qsatb += Ddb * ck(c, T_CREVP + 4) * den;
+    denb += Ddb * ck(c, T_CREVP + 4) * qsat;
+    const double qdenb = Bqb * (ck(c, T_CREVP + 1) * 0.5 / sqrt(qden) + ck(c, T_CREVP + 2) * 0.725 * pe7 / qden);
+    xb.qr += qdenb * den;
+    denb += qdenb * s.qr;
+    tinb += t2b * 2.0 * tin;
+    if (qsat > q_plus) {
+      qsatb += dqb;
+      qpzb -= dqb;
+    } else {
+      const double e = q_minus - qsat;
+      const double eb = dqb * 0.5 * e / dqh;
+      dqhb -= dqb * dq / dqh;
+      qpzb += eb;  // q_minus = qpz - dqh
+      dqhb -= eb;
+      qsatb -= eb;
+    }
+    qsatb += dqvb;
+    xb.qv -= dqvb;
+  }
+  // dqh = min(max(ql, h_var max(qpz, QCMIN)), 0.2 qpz)
+  if (dqh0 < 0.2 * qpz) {
+    if (s.ql > h_var * mq)
+      xb.ql += dqhb;
+    else if (qpz > QCMIN)
+      qpzb += dqhb * h_var;
+  } else {
+    qpzb += 0.2 * dqhb;
+  }
+  sat_rev(false, tin, den, qsat, dqsdt, qsatb, dqsdtb, tinb, denb);
+  xb.qv += qpzb;
+  xb.ql += qpzb;
+  // tin = tz - lcpk ql
+  xb.tz += tinb;
+  lcpkb -= tinb * s.ql;
+  xb.ql -= tinb * lcpk;
+  lhlb += lcpkb / cvm;
+  const double cb = xb.cvm - lcpkb * lcpk / cvm;
+  xb.qv += cb * c.c_vap_d;
+  const double lb = xb.q_liq + cb * C_LIQ, sb = xb.q_sol + cb * C_ICE;
+  xb.ql += lb;
+  xb.qr += lb;
+  xb.qi += sb;
+  xb.qs += sb;
+  xb.qg += sb;
+  xb.q_liq = xb.q_sol = xb.cvm = 0.0;
+  xb.tz += lhlb * c.d0_vap_d;
+}
+
+// Cotangents of the autoconversion's inputs besides the cell
+struct AcB {
+  double den, ccn, c_praut, dl, fac_rc;
+};
+
+// Vector-Jacobian product of autoconv_no_subgrid_var (subgrid = false) or
+// autoconv_subgrid_var with the width dl (subgrid = true) for one cell.
+// qlb, qrb: in, the cotangents of ql and qr after the call; out, before.
+SH_FN void autoconv_vjp(const Coef<double>& c, double t_wfr, bool subgrid, double tz, double ql, double den,
+                        double ccn, double c_praut, double dl, double& qlb, double& qrb, AcB& ab) {
+  const double qc0 = c.fac_rc * ccn;
+  const double qc = c.use_ccn ? qc0 : qc0 / den;
+  double qcb = 0.0;
+  const double pe = pw_el<double>(ql, 7.0 / 3.0);
+  if (!subgrid) {
+    if (!(tz > t_wfr)) return;
+    const double dq = ql - qc;
+    if (!(dq > 0.0)) return;
+    const double a2 = c.dt_rain * c_praut * den * pe;
+    const double sinkb = qrb - qlb;
+    if (dq < a2) {
+      qlb += sinkb;
+      qcb -= sinkb;
+    } else {
+      ab.c_praut += sinkb * c.dt_rain * den * pe;
+      ab.den += sinkb * c.dt_rain * c_praut * pe;
+      qlb += sinkb * c.dt_rain * c_praut * den * (7.0 / 3.0) * pe / ql;
+    }
+  } else {
+    if (!(tz > t_wfr + DT_FR)) return;
+    const double mx = mmax(1.0e-6, dl);
+    const double dl1 = mmin(mx, 0.5 * ql);
+    const double dq = 0.5 * (ql + dl1 - qc);
+    if (!(dq > 0.0)) return;
+    const double r = dq / dl1;
+    const double m = mmin(1.0, r);
+    const double sinkb = qrb - qlb;
+    const double K = c.dt_rain * c_praut * den * pe;
+    const double mb = sinkb * K;
+    ab.c_praut += sinkb * m * c.dt_rain * den * pe;
+    ab.den += sinkb * m * c.dt_rain * c_praut * pe;
+    qlb += sinkb * m * c.dt_rain * c_praut * den * (7.0 / 3.0) * pe / ql;
+    double dqb = 0.0, dl1b = 0.0;
+    if (!(1.0 < r)) {
+      dqb += mb / dl1;
+      dl1b -= mb * r / dl1;
+    }
+    qlb += 0.5 * dqb;
+    dl1b += 0.5 * dqb;
+    qcb -= 0.5 * dqb;
+    if (mx < 0.5 * ql) {
+      if (!(1.0e-6 > dl)) ab.dl += dl1b;
+    } else {
+      qlb += 0.5 * dl1b;
+    }
+  }
+  double qc0b = qcb;
+  if (!c.use_ccn) {
+    qc0b = qcb / den;
+    ab.den -= qcb * qc / den;
+  }
+  ab.fac_rc += qc0b * ccn;
+  ab.ccn += qc0b * c.fac_rc;
 }
 }  // namespace shield
