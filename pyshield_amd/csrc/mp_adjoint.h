@@ -13,17 +13,17 @@
 //  1. replays the forward of its column (the FP64 forward's building blocks,
 //     in the forward's order) and records the per-cell state at 12 points of
 //     every sub-step in its own region of the workspace;
-//  2. walks the steps backwards.  The icloud step (ice melt / freeze, the
+//  2. walks the steps backwards.  Every point-wise step (density update,
+//     rain and solid fall speeds, evaporation / accretion, autoconversion,
+//     sedimentation ice melt, and the icloud step: ice melt / freeze, the
 //     ice-phase collections and the saturation adjustment of subgrid_z_proc)
-//     has a hand-written reverse (mp_point_adj.h).  The other point-wise
-//     steps (density update, fall speeds, evaporation / accretion,
-//     autoconversion, sedimentation ice melt) get their vector-Jacobian
-//     products from the same process code run on dual numbers, one pass per
-//     input, so every derivative follows the branch the forward took at the
-//     point.  The vertical steps have hand-written transposes: the implicit
-//     fall and the sedimentation heating (backward affine scans), the liquid
-//     slope limiter, the momentum transport, and the precipitation and flux
-//     sums.
+//     has a hand-written reverse (mp_point_adj.h) that re-evaluates the
+//     step's FP64 forward and follows the branch it took at the point; the
+//     liquid slope limiter's three-point stencil uses dual numbers
+//     (mp_dual.h).  The vertical steps have hand-written transposes: the
+//     implicit fall and the sedimentation heating (backward affine scans),
+//     the liquid slope limiter, the momentum transport, and the
+//     precipitation and flux sums.
 // Supported switches: the default ones, plus the const_v*, irain_f,
 // z_slope_liq, do_sedi_heat, sedi_transport, do_qa and reset_qaz variants;
 // do_sedi_w, z_slope_ice, fix_negative and de_ice are rejected by the host.
@@ -130,46 +130,11 @@ __device__ __forceinline__ void scan_bwd(const Lay<S>& L, const double (&a)[S], 
   }
 }
 
-// ------------------------------------------------------------ dual helpers
-__device__ __forceinline__ Dual seed(double v, int i, int j) { return Dual(v, i == j ? 1.0 : 0.0); }
-
-__device__ __forceinline__ Coef<Dual> dual_coef(const Coef<double>& c) {
-  Coef<Dual> d;
-  d.c_air = c.c_air; d.c_vap = c.c_vap; d.d0_vap = c.d0_vap; d.lv00 = c.lv00; d.so3 = c.so3;
-  d.log_10 = c.log_10; d.tice = c.tice; d.tice0 = c.tice0; d.t_wfr = c.t_wfr; d.t_sub = c.t_sub;
-  d.fac_rc = c.fac_rc; d.cracs = c.cracs; d.csacr = c.csacr; d.cgacr = c.cgacr; d.cgacs = c.cgacs;
-  d.csacw = c.csacw; d.csaci = c.csaci; d.cgacw = c.cgacw; d.cgaci = c.cgaci; d.cracw = c.cracw;
-  d.ces0 = c.ces0;
-  for (int i = 0; i < T_N; ++i) d.tab[i] = c.tab[i];
-  d.dts = c.dts; d.rdts = c.rdts; d.dt_rain = c.dt_rain; d.fac_i2s = c.fac_i2s; d.fac_g2v = c.fac_g2v;
-  d.fac_v2g = c.fac_v2g; d.fac_imlt = c.fac_imlt; d.fac_l2v = c.fac_l2v;
-  d.ql_mlt = c.ql_mlt; d.qs_mlt = c.qs_mlt; d.qi_gen = c.qi_gen; d.qi_lim = c.qi_lim; d.qi0_crt = c.qi0_crt;
-  d.qs0_crt = c.qs0_crt; d.qc_crt = c.qc_crt;
-  d.vs_fac = c.vs_fac; d.vg_fac = c.vg_fac; d.vi_fac = c.vi_fac; d.vr_fac = c.vr_fac; d.vs_max = c.vs_max;
-  d.vg_max = c.vg_max; d.vi_max = c.vi_max; d.vr_max = c.vr_max;
-  d.const_vi = c.const_vi; d.const_vs = c.const_vs; d.const_vg = c.const_vg; d.const_vr = c.const_vr;
-  d.fast_sat_adj = c.fast_sat_adj; d.do_qa = c.do_qa; d.rad_snow = c.rad_snow; d.rad_rain = c.rad_rain;
-  d.use_ccn = c.use_ccn; d.pad_ = c.pad_;
-  d.t_sub_d = c.t_sub_d; d.t_wfr_d = c.t_wfr_d; d.qc_crt_d = c.qc_crt_d;
-  d.c_air_d = c.c_air_d; d.c_vap_d = c.c_vap_d; d.d0_vap_d = c.d0_vap_d; d.lv00_d = c.lv00_d;
-  return d;
-}
-
 // cotangents of the eight cell values, in registers
 template <int S>
 struct CellBar {
   double v[8][S];
 };
-__device__ __forceinline__ double cell_dot(const double (&yb)[8], const CellT<Dual>& c) {
-  return yb[0] * c.qv.d + yb[1] * c.ql.d + yb[2] * c.qr.d + yb[3] * c.qi.d + yb[4] * c.qs.d + yb[5] * c.qg.d +
-         yb[6] * c.tz.d + yb[7] * c.qa.d;
-}
-template <int S>
-__device__ __forceinline__ CellT<Dual> dual_cell(const Rec<S>& r, int s, int i) {
-  return CellT<Dual>{seed(r.z[Z_QV][s], i, 0), seed(r.z[Z_QL][s], i, 1), seed(r.z[Z_QR][s], i, 2),
-                     seed(r.z[Z_QI][s], i, 3), seed(r.z[Z_QS][s], i, 4), seed(r.z[Z_QG][s], i, 5),
-                     seed(r.z[Z_TZ][s], i, 6), seed(r.z[Z_QA][s], i, 7)};
-}
 
 // the liquid slope limiter at one interior level (the forward's
 // slope_limiter, microphysics.py:595-620), on any scalar type
@@ -645,7 +610,7 @@ __device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, A
 template <int S>
 __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
                                               const shield_mp_fields& f, const shield_mp_params& P,
-                                              const Coef<double>& c, const Coef<Dual>& cd, bool is_first,
+                                              const Coef<double>& c, bool is_first,
                                               const TapeView<S>& T, int it, int rb, const double (&dp1)[S]) {
   Shared<S>& sh = A.sh;
   const double t_wfr = P.t_wfr;
@@ -709,14 +674,11 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
     const double vb = abar(A, AB_VTR, s);
     abar(A, AB_VTR, s) = 0.0;
     if (!L.v[s] || vb == 0.0) continue;
-    Coef<Dual> cdv = cd;
-    cdv.vr_fac.d = 1.0;
-    const Dual vq = rain_fspeed<Dual>(cd, !fall[s], Dual(rB.z[Z_QR][s], 1.0), Dual(rB.z[Z_DEN][s]));
-    const Dual vd = rain_fspeed<Dual>(cd, !fall[s], Dual(rB.z[Z_QR][s]), Dual(rB.z[Z_DEN][s], 1.0));
-    const Dual vp = rain_fspeed<Dual>(cdv, !fall[s], Dual(rB.z[Z_QR][s]), Dual(rB.z[Z_DEN][s]));
-    G.xb.v[2][s] += vb * vq.d;
-    abar(A, AB_DEN, s) += vb * vd.d;
-    G.gp[AP_VR_FAC] += vb * vp.d;
+    double qrb = 0.0, denb = 0.0, vrb = 0.0;
+    rain_fspeed_vjp(c, !fall[s], rB.z[Z_QR][s], rB.z[Z_DEN][s], vb, qrb, denb, vrb);
+    G.xb.v[2][s] += qrb;
+    abar(A, AB_DEN, s) += denb;
+    G.gp[AP_VR_FAC] += vrb;
   }
   if (is_first) {
     // dz1 = dz0 tz / t0, den = den0 dz0 / dz1 = -dp1 / (g dz1), denfac = sqrt(sfcrho / den)
@@ -741,7 +703,7 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
 // The reverse of sedimentation (records RS_F, RS_G, RS_H).
 template <int S>
 __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const shield_mp_params& P,
-                                        const Coef<double>& c, const Coef<Dual>& cd, const TapeView<S>& T, int it,
+                                        const Coef<double>& c, const TapeView<S>& T, int it,
                                         const double (&dp1)[S]) {
   Shared<S>& sh = A.sh;
   const double tice_d = P.nml.tice;
@@ -848,17 +810,19 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!(stop[s] && L.v[s])) continue;
-    double yb[8], g[7];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
-#pragma unroll 1
-    for (int i = 0; i < 7; ++i) {
-      CellT<Dual> cs = dual_cell(rF, s, i);
-      sedi_ice_melt<Dual>(cd, tice_d, P.nml.ql_mlt, cs);
-      g[i] = cell_dot(yb, cs);
-    }
-#pragma unroll
-    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = g[q];
+    IcX xb{G.xb.v[0][s], G.xb.v[1][s], G.xb.v[2][s], G.xb.v[3][s], G.xb.v[4][s], G.xb.v[5][s], G.xb.v[6][s],
+           G.xb.v[7][s], 0.0, 0.0, 0.0};
+    sedi_ice_melt_vjp(c, tice_d, P.nml.ql_mlt,
+                      Cell{rF.z[Z_QV][s], rF.z[Z_QL][s], rF.z[Z_QR][s], rF.z[Z_QI][s], rF.z[Z_QS][s], rF.z[Z_QG][s],
+                           rF.z[Z_TZ][s], rF.z[Z_QA][s]},
+                      xb);
+    G.xb.v[0][s] = xb.qv;
+    G.xb.v[1][s] = xb.ql;
+    G.xb.v[2][s] = xb.qr;
+    G.xb.v[3][s] = xb.qi;
+    G.xb.v[4][s] = xb.qs;
+    G.xb.v[5][s] = xb.qg;
+    G.xb.v[6][s] = xb.tz;
   }
   // fall speeds (record F): inputs qg, qi, qs, tz, den and vs_fac, vg_fac, vi_fac
 #pragma unroll
@@ -866,26 +830,17 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
     const double bg = abar(A, AB_VTG, s), bs = abar(A, AB_VTS, s), bi = vtibar[s];
     abar(A, AB_VTG, s) = abar(A, AB_VTS, s) = 0.0;
     if (!L.v[s]) continue;
-    double g[8];
-#pragma unroll 1
-    for (int i = 0; i < 8; ++i) {
-      Coef<Dual> cdf = cd;
-      if (i == 5) cdf.vs_fac.d = 1.0;
-      if (i == 6) cdf.vg_fac.d = 1.0;
-      if (i == 7) cdf.vi_fac.d = 1.0;
-      Dual vg, vi, vs;
-      fall_speed<Dual>(cdf, seed(rF.z[Z_QG][s], i, 0), seed(rF.z[Z_QI][s], i, 1), seed(rF.z[Z_QS][s], i, 2),
-                       seed(rF.z[Z_TZ][s], i, 3) - tice_d, seed(rF.z[Z_DEN][s], i, 4), vg, vi, vs);
-      g[i] = bg * vg.d + bi * vi.d + bs * vs.d;
-    }
-    G.xb.v[5][s] += g[0];
-    G.xb.v[3][s] += g[1];
-    G.xb.v[4][s] += g[2];
-    G.xb.v[6][s] += g[3];
-    abar(A, AB_DEN, s) += g[4];
-    G.gp[AP_VS_FAC] += g[5];
-    G.gp[AP_VG_FAC] += g[6];
-    G.gp[AP_VI_FAC] += g[7];
+    FsB fb{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    fall_speed_vjp(c, rF.z[Z_QG][s], rF.z[Z_QI][s], rF.z[Z_QS][s], rF.z[Z_TZ][s] - tice_d, rF.z[Z_DEN][s], bg, bi, bs,
+                   fb);
+    G.xb.v[5][s] += fb.qg;
+    G.xb.v[3][s] += fb.qi;
+    G.xb.v[4][s] += fb.qs;
+    G.xb.v[6][s] += fb.tc;
+    abar(A, AB_DEN, s) += fb.den;
+    G.gp[AP_VS_FAC] += fb.vs_fac;
+    G.gp[AP_VG_FAC] += fb.vg_fac;
+    G.gp[AP_VI_FAC] += fb.vi_fac;
   }
 }
 
@@ -944,7 +899,6 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
   const int nz = (int)sh3.nk;
   const int64_t ncol = sh3.ni * sh3.nj;
   const shield_mp_namelist& n = P.nml;
-  const Coef<Dual> cd = dual_coef(c);
   const TapeView<S> T{tape + (int64_t)blockIdx.x * tape_stride};
   const double rdt = P.rdt;
   double gp_acc[AP_N];
@@ -1160,11 +1114,11 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
     for (int it = P.ntimes - 1; it >= 0; --it) {
       icloud_adj(L, A, G, Gl, f, P, c, T, it, X.dp1);
       __syncthreads();
-      warm_rain_adj(L, A, G, Gl, f, P, c, cd, false, T, it, RW2_B, X.dp1);
+      warm_rain_adj(L, A, G, Gl, f, P, c, false, T, it, RW2_B, X.dp1);
       __syncthreads();
-      sed_adj(L, A, G, P, c, cd, T, it, X.dp1);
+      sed_adj(L, A, G, P, c, T, it, X.dp1);
       __syncthreads();
-      warm_rain_adj(L, A, G, Gl, f, P, c, cd, true, T, it, RW1_B, X.dp1);
+      warm_rain_adj(L, A, G, Gl, f, P, c, true, T, it, RW1_B, X.dp1);
       __syncthreads();
     }
     // -------------------------------------------------- fields_init, reversed

@@ -1,9 +1,10 @@
 // mp_point_adj.h -- hand-written reverse mode of FP64 point-wise processes of
 // one cell: the icloud step -- icloud_melt_freeze (microphysics.py:1290-1338),
 // icloud_main (microphysics_funcs.py:853-1260, without subgrid_z_proc) and
-// subgrid_z_proc (microphysics_funcs.py:431-850) -- and, in warm_rain, the
-// evaporation / accretion of rain revap_racc (microphysics_funcs.py:183-265)
-// and the autoconversion of cloud water (microphysics_funcs.py:375-428).
+// subgrid_z_proc (microphysics_funcs.py:431-850) -- in warm_rain, the
+// evaporation / accretion of rain revap_racc (microphysics_funcs.py:183-265),
+// the autoconversion of cloud water (microphysics_funcs.py:375-428) and the
+// rain fall speed, and in sedimentation the ice melt and the fall speeds.
 //
 // The icloud step is cut into blocks (one process each: a melt, a collection, a
 // deposition, ...).  Every block reads the cell state X at its entry -- the
@@ -149,7 +150,7 @@ SH_FN void ic_heat_rev(const Coef<double>& c, const IcX& x, const IcHeat& h, dou
 
 // ------------------------------------------------------------ melt / freeze
 // icloud_melt_freeze: reads the cell, writes q_liq, q_sol, cvm
-SH_FN int blk_melt_freeze(const IcK& k, IcX& x) {
+SH_FN int blk_melt_freeze(const IcK& k, IcX& x, bool freeze = true) {
   const Coef<double>& c = k.c;
   const double lhi = LI00 + DC_ICE * x.tz;
   x.q_liq = x.ql + x.qr;
@@ -167,7 +168,7 @@ SH_FN int blk_melt_freeze(const IcK& k, IcX& x) {
     x.cvm = ic_cvm(c, x.qv, x.q_liq, x.q_sol);
     x.tz = x.tz - (melt * lhi) / x.cvm;
     return 1;
-  } else if ((x.tz < k.t_wfr) && (x.ql > QCMIN)) {
+  } else if (freeze && (x.tz < k.t_wfr) && (x.ql > QCMIN)) {
     const double dtmp = k.t_wfr - x.tz;
     const double factor = mmin(1.0, dtmp / DT_FR);
     const double sink = mmin(x.ql * factor, dtmp / icpk);
@@ -185,10 +186,10 @@ SH_FN int blk_melt_freeze(const IcK& k, IcX& x) {
   return 0;
 }
 
-SH_FN void blk_melt_freeze_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
+SH_FN void blk_melt_freeze_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib, bool freeze = true) {
   const Coef<double>& c = k.c;
   IcX y = x;
-  const int br = blk_melt_freeze(k, y);
+  const int br = blk_melt_freeze(k, y, freeze);
   const double lhi = LI00 + DC_ICE * x.tz;
   const double q_liq0 = x.ql + x.qr, q_sol0 = x.qi + x.qs + x.qg;
   const double cvm0 = ic_cvm(c, x.qv, q_liq0, q_sol0);
@@ -1882,5 +1883,105 @@ SH_FN void autoconv_vjp(const Coef<double>& c, double t_wfr, bool subgrid, doubl
   }
   ab.fac_rc += qc0b * ccn;
   ab.ccn += qc0b * c.fac_rc;
+}
+// ------------------------------------------------------------ sedimentation
+// Vector-Jacobian product of sedi_ice_melt (microphysics.py:702-746), the
+// melt branch of icloud_melt_freeze.  xb: in, the cotangents of the cell after
+// the melt (q_liq, q_sol, cvm zero); out, before it.
+SH_FN void sedi_ice_melt_vjp(const Coef<double>& c, double tice, double ql_mlt, const Cell& s, IcX& xb) {
+  const IcK k{c, tice, 0.0, ql_mlt, 0.0, 0.0, 0.0, 0.0, {1.0, 1.0, 0.0, 0.0, 0.0, 0.0}};
+  IcInB ib{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const IcX x{s.qv, s.ql, s.qr, s.qi, s.qs, s.qg, s.tz, s.qa, 0.0, 0.0, 0.0};
+  blk_melt_freeze_rev(k, x, xb, ib, false);
+}
+
+// Cotangents of the fall speeds' inputs (fall_speed, microphysics_funcs.py:269-341)
+struct FsB {
+  double qg, qi, qs, tc, den, vs_fac, vg_fac, vi_fac;
+};
+
+// sqrt(min(10, SFCRHO / den)) and its cotangent onto den
+SH_FN double rhof_of(double den) { return sqrt(mmin(10.0, SFCRHO / den)); }
+SH_FN void rhof_rev(double den, double rb, double& denb) {
+  const double x = SFCRHO / den;
+  if (!(10.0 < x)) denb -= rb * 0.5 / sqrt(x) * x / den;
+}
+
+// Vector-Jacobian product of fall_speed with cotangents bg, bi, bs of vtg,
+// vti, vts
+SH_FN void fall_speed_vjp(const Coef<double>& c, double qg, double qi, double qs, double tc, double den, double bg,
+                          double bi, double bs, FsB& fb) {
+  const double rhof = rhof_of(den);
+  double rhofb = 0.0;
+  if (c.const_vi) {
+    fb.vi_fac += bi;
+  } else if (!(qi < THI)) {
+    const double vi0 = 0.01 * c.vi_fac;
+    const double lg = log(qi * den);
+    const double poly = tc * (AA * tc + BB) + CC;
+    const double v1 = (3.0 + lg / c.log_10) * poly + DD_FS * tc + EE;
+    const double e = exp(c.log_10 * v1);
+    const double v2 = vi0 * e * 0.8;
+    const double mx = mmax(VF_MIN, v2);
+    if (!(c.vi_max < mx) && !(VF_MIN > v2)) {
+      fb.vi_fac += bi * 0.01 * e * 0.8;
+      const double v1b = bi * vi0 * 0.8 * e * c.log_10;
+      const double lgb = v1b * poly / c.log_10;
+      fb.tc += v1b * (3.0 + lg / c.log_10) * (2.0 * AA * tc + BB) + v1b * DD_FS;
+      fb.qi += lgb / qi;
+      fb.den += lgb / den;
+    }
+  }
+  if (c.const_vs) {
+    fb.vs_fac += bs;
+  } else if (!(qs < THS)) {
+    const double x = qs * den / NORMS;
+    const double p = pw_el<double>(x, 0.0625);
+    const double v = c.vs_fac * VCONS * rhof * p;
+    if (!(c.vs_max < mmax(VF_MIN, v)) && !(VF_MIN > v)) {
+      fb.vs_fac += bs * VCONS * rhof * p;
+      rhofb += bs * c.vs_fac * VCONS * p;
+      const double xb = bs * c.vs_fac * VCONS * rhof * 0.0625 * p / x;
+      fb.qs += xb * den / NORMS;
+      fb.den += xb * qs / NORMS;
+    }
+  }
+  if (c.const_vg) {
+    fb.vg_fac += bg;
+  } else if (!(qg < THG)) {
+    const double x = qg * den / NORMG;
+    const double r8 = sqrt(sqrt(sqrt(x)));
+    const double v = c.vg_fac * VCONG * rhof * r8;
+    if (!(c.vg_max < mmax(VF_MIN, v)) && !(VF_MIN > v)) {
+      fb.vg_fac += bg * VCONG * rhof * r8;
+      rhofb += bg * c.vg_fac * VCONG * r8;
+      const double xb = bg * c.vg_fac * VCONG * rhof * 0.125 * r8 / x;
+      fb.qg += xb * den / NORMG;
+      fb.den += xb * qg / NORMG;
+    }
+  }
+  rhof_rev(den, rhofb, fb.den);
+}
+
+// Vector-Jacobian product of rain_fspeed (microphysics_funcs.py:344-372) with
+// cotangent vb: adds to qrb, denb, vr_facb
+SH_FN void rain_fspeed_vjp(const Coef<double>& c, bool no_fall, double qr, double den, double vb, double& qrb,
+                           double& denb, double& vr_facb) {
+  if (no_fall) return;
+  if (c.const_vr) {
+    vr_facb += vb;
+    return;
+  }
+  if (qr < THR) return;
+  const double rhof = rhof_of(den);
+  const double x = qr * den / NORMR;
+  const double p = pw_el<double>(x, 0.2);
+  const double v = c.vr_fac * VCONR * rhof * p;
+  if (c.vr_max < mmax(VR_MIN, v) || VR_MIN > v) return;
+  vr_facb += vb * VCONR * rhof * p;
+  rhof_rev(den, vb * c.vr_fac * VCONR * p, denb);
+  const double xb = vb * c.vr_fac * VCONR * rhof * 0.2 * p / x;
+  qrb += xb * den / NORMR;
+  denb += xb * qr / NORMR;
 }
 }  // namespace shield

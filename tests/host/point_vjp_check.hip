@@ -122,3 +122,65 @@ extern "C" void autoconv_check(const shield_mp_params* Pp, int subgrid, int n, c
     oh[0] = qlb; oh[1] = qrb; oh[2] = ab.den; oh[3] = ab.ccn; oh[4] = ab.c_praut; oh[5] = ab.dl; oh[6] = ab.fac_rc;
   }
 }
+
+// fall_speed: dual passes over qg, qi, qs, tc, den, vs_fac, vg_fac, vi_fac against fall_speed_vjp;
+// rain_fspeed: over qr, den, vr_fac against rain_fspeed_vjp (v: qg, qi, qs, tc, den, qr)
+extern "C" void speed_check(const shield_mp_params* Pp, int n, const double* v, const double* ybs, double* out_dual,
+                            double* out_hand) {
+  const shield_mp_params& P = *Pp;
+  const Coef<double> c = make_coef<double>(P);
+  for (int j = 0; j < n; ++j) {
+    const double* x = v + 6 * j;
+    const double* yb = ybs + 4 * j;  // cotangents of vtg, vti, vts, vtr
+    double* od = out_dual + 11 * j;
+    for (int i = 0; i < 8; ++i) {
+      Coef<Dual> cd = make_coef<Dual>(P);
+      if (i == 5) cd.vs_fac.d = 1.0;
+      if (i == 6) cd.vg_fac.d = 1.0;
+      if (i == 7) cd.vi_fac.d = 1.0;
+      auto sd = [&](double w, int q) { return Dual(w, i == q ? 1.0 : 0.0); };
+      Dual vg, vi, vs;
+      fall_speed<Dual>(cd, sd(x[0], 0), sd(x[1], 1), sd(x[2], 2), sd(x[3], 3), sd(x[4], 4), vg, vi, vs);
+      od[i] = yb[0] * vg.d + yb[1] * vi.d + yb[2] * vs.d;
+    }
+    for (int i = 0; i < 3; ++i) {
+      Coef<Dual> cd = make_coef<Dual>(P);
+      if (i == 2) cd.vr_fac.d = 1.0;
+      const Dual vr = rain_fspeed<Dual>(cd, false, Dual(x[5], i == 0 ? 1.0 : 0.0), Dual(x[4], i == 1 ? 1.0 : 0.0));
+      od[8 + i] = yb[3] * vr.d;
+    }
+    FsB fb{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    fall_speed_vjp(c, x[0], x[1], x[2], x[3], x[4], yb[0], yb[1], yb[2], fb);
+    double* oh = out_hand + 11 * j;
+    oh[0] = fb.qg; oh[1] = fb.qi; oh[2] = fb.qs; oh[3] = fb.tc; oh[4] = fb.den; oh[5] = fb.vs_fac;
+    oh[6] = fb.vg_fac; oh[7] = fb.vi_fac;
+    double qrb = 0.0, denb = 0.0, vrb = 0.0;
+    rain_fspeed_vjp(c, false, x[5], x[4], yb[3], qrb, denb, vrb);
+    oh[8] = qrb; oh[9] = denb; oh[10] = vrb;
+  }
+}
+
+// sedi_ice_melt: dual passes over qv..tz against sedi_ice_melt_vjp
+extern "C" void melt_check(const shield_mp_params* Pp, int n, const double* cells, const double* ybs,
+                           double* out_dual, double* out_hand) {
+  const shield_mp_params& P = *Pp;
+  const Coef<double> c = make_coef<double>(P);
+  const Coef<Dual> cd = make_coef<Dual>(P);
+  for (int j = 0; j < n; ++j) {
+    const double* z = cells + 8 * j;
+    const double* yb = ybs + 8 * j;
+    double* od = out_dual + 7 * j;
+    for (int i = 0; i < 7; ++i) {
+      auto sd = [&](double w, int q) { return Dual(w, i == q ? 1.0 : 0.0); };
+      CellT<Dual> cs{sd(z[0], 0), sd(z[1], 1), sd(z[2], 2), sd(z[3], 3), sd(z[4], 4), sd(z[5], 5), sd(z[6], 6),
+                     Dual(z[7])};
+      sedi_ice_melt<Dual>(cd, P.nml.tice, P.nml.ql_mlt, cs);
+      od[i] = yb[0] * cs.qv.d + yb[1] * cs.ql.d + yb[2] * cs.qr.d + yb[3] * cs.qi.d + yb[4] * cs.qs.d +
+              yb[5] * cs.qg.d + yb[6] * cs.tz.d;
+    }
+    IcX xb{yb[0], yb[1], yb[2], yb[3], yb[4], yb[5], yb[6], yb[7], 0.0, 0.0, 0.0};
+    sedi_ice_melt_vjp(c, P.nml.tice, P.nml.ql_mlt, Cell{z[0], z[1], z[2], z[3], z[4], z[5], z[6], z[7]}, xb);
+    double* oh = out_hand + 7 * j;
+    oh[0] = xb.qv; oh[1] = xb.ql; oh[2] = xb.qr; oh[3] = xb.qi; oh[4] = xb.qs; oh[5] = xb.qg; oh[6] = xb.tz;
+  }
+}

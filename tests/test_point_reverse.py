@@ -8,7 +8,9 @@ per input), on random cells that reach every branch:
   block-wise forward inside the reverse must also reproduce the process code
   bit for bit (every branch of the reverse is the forward's);
 * revap_racc (evaporation and accretion of rain);
-* the autoconversion, with and without the subgrid width.
+* the autoconversion, with and without the subgrid width;
+* the solid and rain fall speeds (including their clamps) and the
+  sedimentation ice melt.
 
 The products must agree to round-off, normalised by each input's largest
 cotangent.  The dual-number path itself is checked against torch autograd of
@@ -45,7 +47,7 @@ def harness(tmp_path_factory):
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     lib = ctypes.CDLL(out)
-    for fn in (lib.icl_check, lib.revap_check, lib.autoconv_check):
+    for fn in (lib.icl_check, lib.revap_check, lib.autoconv_check, lib.speed_check, lib.melt_check):
         fn.restype = None
     return lib
 
@@ -140,3 +142,35 @@ def test_autoconversion_reverse_matches_dual_numbers(harness, subgrid, use_ccn):
     assert all(v <= 1e-12 for v in errs.values()), errs
     # the threshold's cotangents are exercised
     assert np.abs(od[:, 3]).max() > 0.0 and np.abs(od[:, 6]).max() > 0.0
+
+
+@pytest.mark.parametrize("overrides", [{}, {"const_vi": True, "const_vs": True, "const_vg": True, "const_vr": True}])
+def test_fall_speed_reverses_match_dual_numbers(harness, overrides):
+    nml = PhysicsConfig(dt_atmos=150, npz=79).mp_namelist()
+    nml.update(overrides)
+    params = _lib.setup(nml, 150.0)
+    n = 8000
+    rng = np.random.default_rng(61)
+    # species across the thresholds and up to the speed caps
+    v = np.stack([10 ** rng.uniform(-9.5, -1.5, n), 10 ** rng.uniform(-9.5, -2, n), 10 ** rng.uniform(-9.5, -1.5, n),
+                  rng.uniform(-90, 30, n), rng.uniform(0.02, 1.3, n), 10 ** rng.uniform(-9.5, -1.5, n)], 1)
+    yb = rng.normal(size=(n, 4))
+    od, oh = np.zeros((n, 11)), np.zeros((n, 11))
+    harness.speed_check(ctypes.byref(params), ctypes.c_int(n), _p(v), _p(yb), _p(od), _p(oh))
+    errs = _norm_errs(oh, od, ["qg", "qi", "qs", "tc", "den", "vs_fac", "vg_fac", "vi_fac", "qr", "den_r",
+                               "vr_fac"])
+    print(json.dumps({k: f"{v:.1e}" for k, v in errs.items()}))
+    assert np.isfinite(oh).all()
+    assert all(v <= 1e-12 for v in errs.values()), errs
+
+
+def test_sedimentation_ice_melt_reverse_matches_dual_numbers(harness):
+    params = _lib.setup(PhysicsConfig(dt_atmos=150, npz=79).mp_namelist(), 150.0)
+    n = 8000
+    cells, _, yb = _cells(n, 71)
+    cells[:, 6] = np.random.default_rng(72).uniform(265, 290, n)
+    od, oh = np.zeros((n, 7)), np.zeros((n, 7))
+    harness.melt_check(ctypes.byref(params), ctypes.c_int(n), _p(cells), _p(yb), _p(od), _p(oh))
+    errs = _norm_errs(oh, od, INPUTS[:7])
+    print(json.dumps({k: f"{v:.1e}" for k, v in errs.items()}))
+    assert all(v <= 1e-12 for v in errs.values()), errs
