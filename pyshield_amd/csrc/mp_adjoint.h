@@ -11,8 +11,9 @@
 // Layout: one column per wavefront (S = ceil(nz / 64) slots), and a
 // persistent grid whose wavefronts walk the columns.  Each wavefront
 //  1. replays the forward of its column (the FP64 forward's building blocks,
-//     in the forward's order) and records the per-cell state at 12 points of
-//     every sub-step in its own region of the workspace;
+//     in the forward's order) and records, at 12 points of every sub-step,
+//     the per-cell values the reverse reads there (77 per cell and sub-step)
+//     in its own region of the workspace;
 //  2. walks the steps backwards.  Every point-wise step (density update,
 //     rain and solid fall speeds, evaporation / accretion, autoconversion,
 //     sedimentation ice melt, and the icloud step: ice melt / freeze, the
@@ -41,8 +42,38 @@ enum { Z_QV, Z_QL, Z_QR, Z_QI, Z_QS, Z_QG, Z_TZ, Z_QA, Z_DEN, Z_DENFAC, Z_DZ1, Z
 // record points of one sub-step: warm_rain (first) before the evaporation,
 // before the fall, before the second evaporation, before autoconversion;
 // sedimentation at entry, after the ice melt, after the falls; the same four
-// for warm_rain (second); icloud at entry
-enum { RW1_B, RW1_C, RW1_D, RW1_E, RS_F, RS_G, RS_H, RW2_B, RW2_C, RW2_D, RW2_E, RI, REC_N };
+// for warm_rain (second); icloud at entry; and the density, its surface ratio
+// and the layer thickness, which are fixed after the density update that
+// opens the sub-step
+enum { RW1_B, RW1_C, RW1_D, RW1_E, RS_F, RS_G, RS_H, RW2_B, RW2_C, RW2_D, RW2_E, RI, RAUX, REC_N };
+
+// The fields a record keeps: those the reverse step reads from it.  The cell
+// values (qv .. tz) at most points; the rain fall speed beside them before
+// the rain fall; the solid species and their fall speeds after the ice melt;
+// temperature and cloud water before autoconversion.  The reverse reads den,
+// denfac and dz1 from RAUX, icloud's fall speeds from RW2_C and RS_G, and
+// never the cloud-fraction accumulator.
+constexpr uint32_t zbit(int z) { return 1u << z; }
+constexpr uint32_t ZM_CELL = zbit(Z_QV) | zbit(Z_QL) | zbit(Z_QR) | zbit(Z_QI) | zbit(Z_QS) | zbit(Z_QG) | zbit(Z_TZ);
+constexpr uint32_t ZM_AUX = zbit(Z_DEN) | zbit(Z_DENFAC) | zbit(Z_DZ1);
+__host__ __device__ constexpr uint32_t rec_fields(int rec) {
+  return (rec == RW1_C || rec == RW2_C)   ? ZM_CELL | zbit(Z_VTR)
+         : (rec == RW1_E || rec == RW2_E) ? zbit(Z_QL) | zbit(Z_TZ)
+         : (rec == RS_G) ? zbit(Z_QI) | zbit(Z_QS) | zbit(Z_QG) | zbit(Z_VTG) | zbit(Z_VTS)
+         : (rec == RAUX) ? ZM_AUX
+                         : ZM_CELL;
+}
+__host__ __device__ constexpr int nbits(uint32_t m) { return __builtin_popcount(m); }
+// offset of record rec in a sub-step's block, in fields (a loop, not
+// recursion, so it folds to a constant wherever rec is one)
+__host__ __device__ constexpr int rec_offset(int rec) {
+  int o = 0;
+  for (int r = 0; r < rec; ++r) o += nbits(rec_fields(r));
+  return o;
+}
+constexpr int REC_FIELDS = rec_offset(REC_N);
+static_assert(REC_FIELDS == 77, "fields per sub-step");
+
 // per-cell cotangents of the auxiliary fields, kept in LDS
 enum { AB_DEN, AB_DENFAC, AB_DZ1, AB_VTR, AB_VTG, AB_VTS, AB_M1, AB_M1SOL, AB_DP1, AB_T0, AB_DZ0, AB_N };
 
@@ -56,33 +87,46 @@ enum { CB_RAIN, CB_SNOW, CB_ICE, CB_GRAUPEL, CB_CCN, CB_CPRAUT };
 
 template <int S>
 struct TapeView {
-  double* base;  // this wavefront's region: ntimes * REC_N * Z_N * 64 S doubles
+  double* base;  // this wavefront's region: ntimes * REC_FIELDS * 64 S doubles
+  // field z of record rec (z must be one of rec_fields(rec))
   __device__ __forceinline__ double* at(int it, int rec, int z) const {
-    return base + ((size_t)(it * REC_N + rec) * Z_N + z) * (64 * S);
+    const int k = rec_offset(rec) + nbits(rec_fields(rec) & (zbit(z) - 1u));
+    return base + ((size_t)it * REC_FIELDS + k) * (64 * S);
   }
 };
 
+// value of field z of the replay's state
+template <int S>
+__device__ __forceinline__ double rec_value(const St<S, double>& X, Shared<S>& sh, int z, int s) {
+  switch (z) {
+    case Z_QV: return X.qv[s];
+    case Z_QL: return X.ql[s];
+    case Z_QR: return X.qr[s];
+    case Z_QI: return X.qi[s];
+    case Z_QS: return X.qs[s];
+    case Z_QG: return X.qg[s];
+    case Z_TZ: return X.tz[s];
+    case Z_QA: return X.qaz[s];
+    case Z_DEN: return sh.at(CL_DEN, s);
+    case Z_DENFAC: return sh.at(CL_DENFAC, s);
+    case Z_DZ1: return sh.at(CL_DZ1, s);
+    case Z_VTR: return X.vtr[s];
+    case Z_VTG: return X.vtg[s];
+    default: return X.vts[s];
+  }
+}
+
+// record the fields of `mask` (by default all that record rec keeps)
 template <int S>
 __device__ __forceinline__ void tape_save(const TapeView<S>& T, int it, int rec, const St<S, double>& X,
-                                          Shared<S>& sh) {
+                                          Shared<S>& sh, uint32_t mask = 0xFFFFFFFFu) {
   const int lane = lane_id();
+  mask &= rec_fields(rec);
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = 64 * s + lane;
-    T.at(it, rec, Z_QV)[i] = X.qv[s];
-    T.at(it, rec, Z_QL)[i] = X.ql[s];
-    T.at(it, rec, Z_QR)[i] = X.qr[s];
-    T.at(it, rec, Z_QI)[i] = X.qi[s];
-    T.at(it, rec, Z_QS)[i] = X.qs[s];
-    T.at(it, rec, Z_QG)[i] = X.qg[s];
-    T.at(it, rec, Z_TZ)[i] = X.tz[s];
-    T.at(it, rec, Z_QA)[i] = X.qaz[s];
-    T.at(it, rec, Z_DEN)[i] = sh.cell[CL_DEN][i];
-    T.at(it, rec, Z_DENFAC)[i] = sh.cell[CL_DENFAC][i];
-    T.at(it, rec, Z_DZ1)[i] = sh.cell[CL_DZ1][i];
-    T.at(it, rec, Z_VTR)[i] = X.vtr[s];
-    T.at(it, rec, Z_VTG)[i] = X.vtg[s];
-    T.at(it, rec, Z_VTS)[i] = X.vts[s];
+  for (int z = 0; z < Z_N; ++z) {
+    if (!(mask & zbit(z))) continue;
+#pragma unroll
+    for (int s = 0; s < S; ++s) T.at(it, rec, z)[64 * s + lane] = rec_value(X, sh, z, s);
   }
 }
 
@@ -91,13 +135,34 @@ template <int S>
 struct Rec {
   double z[Z_N][S];
 };
+// Load record rec, with den, denfac and dz1 from LDS (where the reverse walk
+// put the sub-step's RAUX record); fields the record does not keep read as
+// NaN.
 template <int S>
-__device__ __forceinline__ void tape_load(const TapeView<S>& T, int it, int rec, Rec<S>& r) {
+__device__ __forceinline__ void tape_load(const TapeView<S>& T, int it, int rec, Rec<S>& r, Shared<S>& sh) {
   const int lane = lane_id();
+  const uint32_t m = rec_fields(rec);
 #pragma unroll
   for (int q = 0; q < Z_N; ++q)
 #pragma unroll
-    for (int s = 0; s < S; ++s) r.z[q][s] = T.at(it, rec, q)[64 * s + lane];
+    for (int s = 0; s < S; ++s) {
+      if (q == Z_DEN || q == Z_DENFAC || q == Z_DZ1)
+        r.z[q][s] = sh.cell[q == Z_DEN ? CL_DEN : q == Z_DENFAC ? CL_DENFAC : CL_DZ1][64 * s + lane];
+      else
+        r.z[q][s] = (m & zbit(q)) ? T.at(it, rec, q)[64 * s + lane] : __builtin_nan("");
+    }
+}
+
+// put the sub-step's den, denfac and dz1 into LDS for tape_load
+template <int S>
+__device__ __forceinline__ void aux_load(const TapeView<S>& T, int it, Shared<S>& sh) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    sh.cell[CL_DEN][64 * s + lane] = T.at(it, RAUX, Z_DEN)[64 * s + lane];
+    sh.cell[CL_DENFAC][64 * s + lane] = T.at(it, RAUX, Z_DENFAC)[64 * s + lane];
+    sh.cell[CL_DZ1][64 * s + lane] = T.at(it, RAUX, Z_DZ1)[64 * s + lane];
+  }
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -343,6 +408,7 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
       sh.at(CL_DEN, s) = den;
       sh.at(CL_DENFAC, s) = sqrt(SFCRHO / den);
     }
+    tape_save(T, it, RAUX, X, sh);
   }
   tape_save(T, it, rb, X, sh);
   bool pred[S], fall[S];
@@ -379,8 +445,8 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
       }
     }
   } else {
-    tape_save(T, it, rb + 1, X, sh);
-    tape_save(T, it, rb + 2, X, sh);
+    // no rain falls: the reverse reads only the fall speed (icloud's)
+    tape_save(T, it, rb + 1, X, sh, zbit(Z_VTR));
   }
   tape_save(T, it, rb + 3, X, sh);
   const double hv = sh.colv[CV_HVAR][0];
@@ -517,7 +583,7 @@ __device__ __forceinline__ void revap_adj(const Lay<S>& L, AdjShared<S>& A, AdjR
     double denb = 0.0, denfacb = 0.0;
     revap_vjp(c, t_wfr, dt5, hv,
               Cell{r.z[Z_QV][s], r.z[Z_QL][s], r.z[Z_QR][s], r.z[Z_QI][s], r.z[Z_QS][s], r.z[Z_QG][s], r.z[Z_TZ][s],
-                   r.z[Z_QA][s]},
+                   0.0},
               r.z[Z_DEN][s], r.z[Z_DENFAC][s], xb, denb, denfacb);
     G.xb.v[0][s] = xb.qv;
     G.xb.v[1][s] = xb.ql;
@@ -617,7 +683,7 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
   const double dt5 = 0.5 * c.dt_rain;
   const double hv = sh.colv[CV_HVAR][0];
   Rec<S> rB;
-  tape_load(T, it, rb, rB);
+  tape_load(T, it, rb, rB, sh);
   bool pred[S], fall[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (rB.z[Z_QR][s] > QRMIN);
@@ -633,17 +699,17 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
   }
   {
     Rec<S> rE;
-    tape_load(T, it, rb + 3, rE);
+    tape_load(T, it, rb + 3, rE, sh);
     autoconv_adj(L, A, G, rE, P, c, hv, sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0]);
   }
   if (wave_any(fall)) {
     {
       Rec<S> rD;
-      tape_load(T, it, rb + 2, rD);
+      tape_load(T, it, rb + 2, rD, sh);
       revap_adj(L, A, G, rD, fall, c, t_wfr, dt5, hv);
     }
     Rec<S> rC;
-    tape_load(T, it, rb + 1, rC);
+    tape_load(T, it, rb + 1, rC, sh);
     double qr[S], vt[S], dz1[S], qn[S], flux[S], vtbar[S], dz1bar[S], dp1bar[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -708,8 +774,8 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
   Shared<S>& sh = A.sh;
   const double tice_d = P.nml.tice;
   Rec<S> rF, rG;
-  tape_load(T, it, RS_F, rF);
-  tape_load(T, it, RS_G, rG);
+  tape_load(T, it, RS_F, rF, sh);
+  tape_load(T, it, RS_G, rG, sh);
   double vti[S], dz1[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -757,7 +823,7 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
   }
   if (P.nml.do_sedi_heat) {
     Rec<S> rH;
-    tape_load(T, it, RS_H, rH);
+    tape_load(T, it, RS_H, rH, sh);
     bool all[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) all[s] = L.v[s];
@@ -814,7 +880,7 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
            G.xb.v[7][s], 0.0, 0.0, 0.0};
     sedi_ice_melt_vjp(c, tice_d, P.nml.ql_mlt,
                       Cell{rF.z[Z_QV][s], rF.z[Z_QL][s], rF.z[Z_QR][s], rF.z[Z_QI][s], rF.z[Z_QS][s], rF.z[Z_QG][s],
-                           rF.z[Z_TZ][s], rF.z[Z_QA][s]},
+                           rF.z[Z_TZ][s], 0.0},
                       xb);
     G.xb.v[0][s] = xb.qv;
     G.xb.v[1][s] = xb.ql;
@@ -854,7 +920,18 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
                                            const double (&dp1)[S]) {
   Shared<S>& sh = A.sh;
   Rec<S> r;
-  tape_load(T, it, RI, r);
+  tape_load(T, it, RI, r, sh);
+  {
+    // the fall speeds icloud used: the solid ones of sedimentation, the rain
+    // one of the second warm_rain
+    const int lane = lane_id();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      r.z[Z_VTG][s] = T.at(it, RS_G, Z_VTG)[64 * s + lane];
+      r.z[Z_VTS][s] = T.at(it, RS_G, Z_VTS)[64 * s + lane];
+      r.z[Z_VTR][s] = T.at(it, RW2_C, Z_VTR)[64 * s + lane];
+    }
+  }
   const double hv = sh.colv[CV_HVAR][0], rh_adj = sh.colv[CV_RHADJ][0], rh_rain = sh.colv[CV_RHRAIN][0];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -870,7 +947,7 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
     IcInB ib{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     icloud_vjp(k,
                Cell{r.z[Z_QV][s], r.z[Z_QL][s], r.z[Z_QR][s], r.z[Z_QI][s], r.z[Z_QS][s], r.z[Z_QG][s],
-                    r.z[Z_TZ][s], r.z[Z_QA][s]},
+                    r.z[Z_TZ][s], 0.0},
                yb, ib);
 #pragma unroll
     for (int q = 0; q < 7; ++q) G.xb.v[q][s] = yb[q];
@@ -1112,6 +1189,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
     __syncthreads();
     // ------------------------------------------------ sub-steps, reversed
     for (int it = P.ntimes - 1; it >= 0; --it) {
+      aux_load(T, it, sh);
       icloud_adj(L, A, G, Gl, f, P, c, T, it, X.dp1);
       __syncthreads();
       warm_rain_adj(L, A, G, Gl, f, P, c, false, T, it, RW2_B, X.dp1);

@@ -16,7 +16,7 @@ int adjoint_slots(int64_t nk) { return (int)((nk + 63) / 64); }
 
 // doubles of tape per wavefront
 int64_t adjoint_tape_doubles(int64_t nk, int ntimes) {
-  return (int64_t)ntimes * REC_N * Z_N * 64 * adjoint_slots(nk);
+  return (int64_t)ntimes * REC_FIELDS * 64 * adjoint_slots(nk);
 }
 
 hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
