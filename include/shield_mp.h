@@ -266,8 +266,8 @@ typedef struct shield_mp_adjoint_fields {
   double* params;
 } shield_mp_adjoint_fields;
 
-/* Bytes of workspace for `waves` concurrent wavefronts (each walks columns
- * and records its column's per-sub-step state there); 0 on a bad shape.     */
+/* Bytes of workspace for `waves` concurrent wavefronts (each walks groups of
+ * columns and records their per-sub-step state there); 0 on a bad shape.    */
 size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shield_mp_params* p, int64_t waves);
 
 /* Reverse-mode adjoint of one FP64 shield_mp_forward call with the inputs

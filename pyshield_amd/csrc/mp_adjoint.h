@@ -8,9 +8,11 @@
 // vr_fac, vs_fac, vg_fac, vi_fac (fall-speed coefficients) and qi_gen (ice
 // generation / nucleation).
 //
-// Layout: one column per wavefront (S = ceil(nz / 64) slots), and a
-// persistent grid whose wavefronts walk the columns.  Each wavefront
-//  1. replays the forward of its column (the FP64 forward's building blocks,
+// Layout: P columns per wavefront in S slots (the levels of a column on
+// consecutive lanes, as in the forward kernel; 2 columns in 3 slots at
+// nz = 79), and a persistent grid whose wavefronts walk the columns.  Each
+// wavefront
+//  1. replays the forward of its columns (the FP64 forward's building blocks,
 //     in the forward's order) and records, at 12 points of every sub-step,
 //     the per-cell values the reverse reads there (77 per cell and sub-step)
 //     in its own region of the workspace;
@@ -81,7 +83,8 @@ template <int S>
 struct AdjShared {
   Shared<S> sh;
   double bar[AB_N][64 * S];
-  double colbar[8];  // rain, snow, ice, graupel (per unit of the accumulated flux), ccn, c_praut
+  // per column: rain, snow, ice, graupel (per unit of the accumulated flux), ccn, c_praut
+  double colbar[8][pmax_for(S)];
 };
 enum { CB_RAIN, CB_SNOW, CB_ICE, CB_GRAUPEL, CB_CCN, CB_CPRAUT };
 
@@ -426,7 +429,7 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
     for (int s = 0; s < S; ++s) {
       if (fall[s]) {
         Cell cs = cell_of(X, s);
-        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][0], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
+        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][L.cl[s]], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
         set_cell(X, s, cs);
       }
     }
@@ -440,7 +443,7 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
     for (int s = 0; s < S; ++s) {
       if (fall[s]) {
         Cell cs = cell_of(X, s);
-        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][0], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
+        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][L.cl[s]], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
         set_cell(X, s, cs);
       }
     }
@@ -449,12 +452,12 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
     tape_save(T, it, rb + 1, X, sh, zbit(Z_VTR));
   }
   tape_save(T, it, rb + 3, X, sh);
-  const double hv = sh.colv[CV_HVAR][0];
   if (P.nml.irain_f != 0) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       Cell cs = cell_of(X, s);
-      autoconv_no_subgrid_var<double>(c, t_wfr, cs, sh.at(CL_DEN, s), sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0]);
+      autoconv_no_subgrid_var<double>(c, t_wfr, cs, sh.at(CL_DEN, s), sh.colv[CV_CCN][L.cl[s]],
+                                      sh.colv[CV_CPRAUT][L.cl[s]]);
       set_cell(X, s, cs);
     }
   } else {
@@ -463,14 +466,16 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       Cell cs = cell_of(X, s);
+      const double hv = sh.colv[CV_HVAR][L.cl[s]];
       const double d = P.nml.z_slope_liq ? mmax(dl[s], mmax(QVMIN, hv * cs.ql)) : mmax(QVMIN, hv * cs.ql);
-      autoconv_subgrid_var<double>(c, t_wfr, cs, sh.at(CL_DEN, s), sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0], d);
+      autoconv_subgrid_var<double>(c, t_wfr, cs, sh.at(CL_DEN, s), sh.colv[CV_CCN][L.cl[s]],
+                                   sh.colv[CV_CPRAUT][L.cl[s]], d);
       set_cell(X, s, cs);
     }
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (L.v[s] && L.kb[s] == 0) sh.colv[CV_RAIN][0] += surf[s];
+    if (L.v[s] && L.kb[s] == 0) sh.colv[CV_RAIN][L.cl[s]] += surf[s];
     double& m1 = sh.cell[CL_M1][64 * s + lane];
     m1 = is_first ? m1 + flux[s] : m1 + flux[s] + sh.cell[CL_M1SOL][64 * s + lane];
   }
@@ -518,7 +523,7 @@ __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Share
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       m1_sol[s] = flux[s];
-      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_ICE][0] += surf[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_ICE][L.cl[s]] += surf[s];
     }
   }
 #pragma unroll
@@ -529,7 +534,7 @@ __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Share
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       m1_sol[s] = m1_sol[s] + flux[s];
-      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_SNOW][0] += surf[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_SNOW][L.cl[s]] += surf[s];
     }
   }
 #pragma unroll
@@ -540,7 +545,7 @@ __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Share
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       m1_sol[s] = m1_sol[s] + flux[s];
-      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_GRAUPEL][0] += surf[s];
+      if (L.v[s] && L.kb[s] == 0) sh.colv[CV_GRAUPEL][L.cl[s]] += surf[s];
     }
   }
   __syncthreads();
@@ -573,15 +578,14 @@ __device__ __forceinline__ double& abar(AdjShared<S>& A, int which, int s) {
 // columns (hand-written reverse, mp_point_adj.h).
 template <int S>
 __device__ __forceinline__ void revap_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Rec<S>& r,
-                                          const bool (&fall)[S], const Coef<double>& c, double t_wfr, double dt5,
-                                          double hv) {
+                                          const bool (&fall)[S], const Coef<double>& c, double t_wfr, double dt5) {
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!(fall[s] && L.v[s])) continue;
     IcX xb{G.xb.v[0][s], G.xb.v[1][s], G.xb.v[2][s], G.xb.v[3][s], G.xb.v[4][s], G.xb.v[5][s], G.xb.v[6][s],
            G.xb.v[7][s], 0.0, 0.0, 0.0};
     double denb = 0.0, denfacb = 0.0;
-    revap_vjp(c, t_wfr, dt5, hv,
+    revap_vjp(c, t_wfr, dt5, A.sh.colv[CV_HVAR][L.cl[s]],
               Cell{r.z[Z_QV][s], r.z[Z_QL][s], r.z[Z_QR][s], r.z[Z_QI][s], r.z[Z_QS][s], r.z[Z_QG][s], r.z[Z_TZ][s],
                    0.0},
               r.z[Z_DEN][s], r.z[Z_DENFAC][s], xb, denb, denfacb);
@@ -602,8 +606,7 @@ __device__ __forceinline__ void revap_adj(const Lay<S>& L, AdjShared<S>& A, AdjR
 // fac_rc.
 template <int S>
 __device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Rec<S>& r,
-                                             const shield_mp_params& P, const Coef<double>& c, double hv,
-                                             double ccn, double cpraut) {
+                                             const shield_mp_params& P, const Coef<double>& c) {
   Shared<S>& sh = A.sh;
   const bool slope = P.nml.z_slope_liq != 0 && P.nml.irain_f == 0;
   double ql[S], dl[S], dlbar[S];
@@ -613,10 +616,13 @@ __device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, A
     dlbar[s] = 0.0;
   }
   if (slope) slope_limiter(L, sh, ql, dl);
-  double ccnbar = 0.0, cprautbar = 0.0;
+  double ccnbar[S], cprautbar[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
+    ccnbar[s] = cprautbar[s] = 0.0;
     if (!L.v[s]) continue;
+    const int cl = L.cl[s];
+    const double hv = sh.colv[CV_HVAR][cl], ccn = sh.colv[CV_CCN][cl], cpraut = sh.colv[CV_CPRAUT][cl];
     double qlb = G.xb.v[1][s], qrb = G.xb.v[2][s];
     AcB ab{0.0, 0.0, 0.0, 0.0, 0.0};
     const double tz = r.z[Z_TZ][s], qlv = r.z[Z_QL][s], den = r.z[Z_DEN][s];
@@ -634,15 +640,26 @@ __device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, A
     G.xb.v[1][s] = qlb;
     G.xb.v[2][s] = qrb;
     abar(A, AB_DEN, s) += ab.den;
-    ccnbar += ab.ccn;
-    cprautbar += ab.c_praut;
+    ccnbar[s] = ab.ccn;
+    cprautbar[s] = ab.c_praut;
     G.gp[AP_FAC_RC] += ab.fac_rc;
   }
-  ccnbar = wave_sum(ccnbar);
-  cprautbar = wave_sum(cprautbar);
-  if (lane_id() == 0) {
-    A.colbar[CB_CCN] += ccnbar;
-    A.colbar[CB_CPRAUT] += cprautbar;
+  // per-column sums
+  for (int cc = 0; cc < L.ncols; ++cc) {
+    double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (L.v[s] && L.cl[s] == cc) {
+        a1 += ccnbar[s];
+        a2 += cprautbar[s];
+      }
+    }
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if (lane_id() == 0) {
+      A.colbar[CB_CCN][cc] += a1;
+      A.colbar[CB_CPRAUT][cc] += a2;
+    }
   }
   __syncthreads();
   if (slope) {
@@ -681,7 +698,6 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
   Shared<S>& sh = A.sh;
   const double t_wfr = P.t_wfr;
   const double dt5 = 0.5 * c.dt_rain;
-  const double hv = sh.colv[CV_HVAR][0];
   Rec<S> rB;
   tape_load(T, it, rb, rB, sh);
   bool pred[S], fall[S];
@@ -695,18 +711,18 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
     const double m1b = abar(A, AB_M1, s);
     fluxbar[s] = m1b;
     if (!is_first) abar(A, AB_M1SOL, s) += m1b;
-    if (L.v[s] && L.kb[s] == 0) fluxbar[s] += A.colbar[CB_RAIN];
+    if (L.v[s] && L.kb[s] == 0) fluxbar[s] += A.colbar[CB_RAIN][L.cl[s]];
   }
   {
     Rec<S> rE;
     tape_load(T, it, rb + 3, rE, sh);
-    autoconv_adj(L, A, G, rE, P, c, hv, sh.colv[CV_CCN][0], sh.colv[CV_CPRAUT][0]);
+    autoconv_adj(L, A, G, rE, P, c);
   }
   if (wave_any(fall)) {
     {
       Rec<S> rD;
       tape_load(T, it, rb + 2, rD, sh);
-      revap_adj(L, A, G, rD, fall, c, t_wfr, dt5, hv);
+      revap_adj(L, A, G, rD, fall, c, t_wfr, dt5);
     }
     Rec<S> rC;
     tape_load(T, it, rb + 1, rC, sh);
@@ -732,7 +748,7 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
       abar(A, AB_DZ1, s) += dz1bar[s];
       abar(A, AB_DP1, s) += dp1bar[s];
     }
-    revap_adj(L, A, G, rB, fall, c, t_wfr, dt5, hv);
+    revap_adj(L, A, G, rB, fall, c, t_wfr, dt5);
   }
   // rain fall speed: inputs qr, den, vr_fac
 #pragma unroll
@@ -836,7 +852,7 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
     double fb[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_GRAUPEL] : 0.0);
+      fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_GRAUPEL][L.cl[s]] : 0.0);
       vtbar[s] = 0.0;
     }
     fall_adj(L, sh, qg, vtg, dz1, dp1, P.dts, fg, G.xb.v[5], fb, vtbar, dz1bar, dp1bar);
@@ -847,7 +863,7 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
     double fb[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_SNOW] : 0.0);
+      fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_SNOW][L.cl[s]] : 0.0);
       vtbar[s] = 0.0;
     }
     fall_adj(L, sh, qs, vts, dz1, dp1, P.dts, fs, G.xb.v[4], fb, vtbar, dz1bar, dp1bar);
@@ -860,7 +876,7 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
   if (any_i) {
     double fb[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_ICE] : 0.0);
+    for (int s = 0; s < S; ++s) fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_ICE][L.cl[s]] : 0.0);
     fall_adj(L, sh, qi, vti, dz1, dp1, P.dts, fi, G.xb.v[3], fb, vtibar, dz1bar, dp1bar);
   }
 #pragma unroll
@@ -932,10 +948,11 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
       r.z[Z_VTR][s] = T.at(it, RW2_C, Z_VTR)[64 * s + lane];
     }
   }
-  const double hv = sh.colv[CV_HVAR][0], rh_adj = sh.colv[CV_RHADJ][0], rh_rain = sh.colv[CV_RHRAIN][0];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!L.v[s]) continue;
+    const int cl = L.cl[s];
+    const double hv = sh.colv[CV_HVAR][cl], rh_adj = sh.colv[CV_RHADJ][cl], rh_rain = sh.colv[CV_RHRAIN][cl];
     const int32_t o = Gl.off[s];
     const double t0 = f.pt[o], dz0 = f.delz[o];
     const double p1 = dry_density<double>(dp1[s], dz0) * RDGAS * t0;
@@ -969,7 +986,8 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
 template <int S>
 __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<double> c,
                                                         shield_mp_fields f, shield_mp_adjoint_fields a,
-                                                        double* tape, int64_t tape_stride, double* part) {
+                                                        double* tape, int64_t tape_stride, double* part,
+                                                        int cols_per_wave) {
   __shared__ AdjShared<S> A;
   Shared<S>& sh = A.sh;
   const int lane = lane_id();
@@ -982,35 +1000,46 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
 #pragma unroll
   for (int p = 0; p < AP_N; ++p) gp_acc[p] = 0.0;
 
-  for (int64_t col = blockIdx.x; col < ncol; col += gridDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * cols_per_wave;
+  for (int64_t c0 = (int64_t)blockIdx.x * cols_per_wave; c0 < ncol; c0 += stride) {
+    const int ncols = (int)((ncol - c0) < cols_per_wave ? (ncol - c0) : cols_per_wave);
     Lay<S> L;
     Glob<S> Gl;
     L.nz = nz;
-    L.ncols = 1;
-    const int64_t ci = col / sh3.nj, cj = col - ci * sh3.nj;
+    L.ncols = ncols;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int idx = 64 * s + lane;
-      const bool v = idx < nz;
+      const int cl = idx / nz;
+      const int k = idx - cl * nz;
+      const bool v = cl < ncols;
       L.v[s] = v;
-      L.k[s] = v ? idx : 0;
-      L.kb[s] = v ? nz - 1 - idx : 0;
-      L.cl[s] = 0;
+      L.k[s] = v ? k : 0;
+      L.kb[s] = v ? nz - 1 - k : 0;
+      L.cl[s] = v ? cl : 0;
+      const int64_t col = c0 + L.cl[s];
+      const int64_t ci = col / sh3.nj, cj = col - ci * sh3.nj;
       Gl.off[s] = (int32_t)(ci * sh3.si + cj * sh3.sj + L.k[s]);
     }
-    const int64_t o2 = ci * sh3.si2 + cj * sh3.sj2;
+    // the 2D offset of column `lane` (lanes below ncols)
+    int64_t o2 = 0;
+    if (lane < ncols) {
+      const int64_t col = c0 + lane;
+      const int64_t ci = col / sh3.nj, cj = col - ci * sh3.nj;
+      o2 = ci * sh3.si2 + cj * sh3.sj2;
+    }
     // ------------------------------------------------------------ fields_init
-    if (lane == 0) {
+    if (lane < ncols) {
       const double land = f.land[o2], area = f.area[o2];
       const double s_leng = sqrt(sqrt(area * 1.0e-10));
       double h_var = n.dw_land * s_leng * land + n.dw_ocean * s_leng * (1.0 - land);
       h_var = mmin(0.2, mmax(0.01, h_var));
       const double rh_adj = 1.0 - h_var - n.rh_inc;
-      sh.colv[CV_HVAR][0] = h_var;
-      sh.colv[CV_RHADJ][0] = rh_adj;
-      sh.colv[CV_RHRAIN][0] = mmax(0.35, rh_adj - n.rh_inr);
-      sh.colv[CV_CCN][0] = (n.ccn_l * land + n.ccn_o * (1.0 - land)) * 1.0e6;
-      sh.colv[CV_RAIN][0] = sh.colv[CV_SNOW][0] = sh.colv[CV_ICE][0] = sh.colv[CV_GRAUPEL][0] = 0.0;
+      sh.colv[CV_HVAR][lane] = h_var;
+      sh.colv[CV_RHADJ][lane] = rh_adj;
+      sh.colv[CV_RHRAIN][lane] = mmax(0.35, rh_adj - n.rh_inr);
+      sh.colv[CV_CCN][lane] = (n.ccn_l * land + n.ccn_o * (1.0 - land)) * 1.0e6;
+      sh.colv[CV_RAIN][lane] = sh.colv[CV_SNOW][lane] = sh.colv[CV_ICE][lane] = sh.colv[CV_GRAUPEL][lane] = 0.0;
     }
     __syncthreads();
     St<S, double> X;
@@ -1036,21 +1065,22 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
       }
       X.vtr[s] = X.vtg[s] = X.vts[s] = VF_MIN;
     }
-    double p1_b = 0.0;  // p1 of the surface layer (held by the lane of that cell)
+    double p1_b[S];  // p1 of the surface layer (in the slot and lane of that cell)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+      p1_b[s] = 0.0;
       if (L.v[s] && L.kb[s] == 0) {
-        double ccn = sh.colv[CV_CCN][0];
+        double ccn = sh.colv[CV_CCN][L.cl[s]];
         if (P.use_ccn) {
           const int32_t o = Gl.off[s];
-          p1_b = dry_density<double>(X.dp1[s], f.delz[o]) * RDGAS * f.pt[o];
-          ccn = ccn * RDGAS * X.tz[s] / p1_b;
+          p1_b[s] = dry_density<double>(X.dp1[s], f.delz[o]) * RDGAS * f.pt[o];
+          ccn = ccn * RDGAS * X.tz[s] / p1_b[s];
         }
-        sh.colv[CV_CCN][0] = ccn;
+        sh.colv[CV_CCN][L.cl[s]] = ccn;
       }
     }
     __syncthreads();
-    if (lane == 0) sh.colv[CV_CPRAUT][0] = P.cpaut * pow(sh.colv[CV_CCN][0] * RHOR, -1.0 / 3.0);
+    if (lane < ncols) sh.colv[CV_CPRAUT][lane] = P.cpaut * pow(sh.colv[CV_CCN][lane] * RHOR, -1.0 / 3.0);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       sh.cell[CL_M1][64 * s + lane] = 0.0;
@@ -1079,12 +1109,12 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
       for (int b = 0; b < AB_N; ++b) abar(A, b, s) = 0.0;
     }
     const double convt = 86400.0 * rdt * RGRAV;
-    if (lane == 0) {
-      A.colbar[CB_RAIN] = a.rain ? a.rain[o2] * convt : 0.0;
-      A.colbar[CB_SNOW] = a.snow ? a.snow[o2] * convt : 0.0;
-      A.colbar[CB_ICE] = a.ice ? a.ice[o2] * convt : 0.0;
-      A.colbar[CB_GRAUPEL] = a.graupel ? a.graupel[o2] * convt : 0.0;
-      A.colbar[CB_CCN] = A.colbar[CB_CPRAUT] = 0.0;
+    if (lane < ncols) {
+      A.colbar[CB_RAIN][lane] = a.rain ? a.rain[o2] * convt : 0.0;
+      A.colbar[CB_SNOW][lane] = a.snow ? a.snow[o2] * convt : 0.0;
+      A.colbar[CB_ICE][lane] = a.ice ? a.ice[o2] * convt : 0.0;
+      A.colbar[CB_GRAUPEL][lane] = a.graupel ? a.graupel[o2] * convt : 0.0;
+      A.colbar[CB_CCN][lane] = A.colbar[CB_CPRAUT][lane] = 0.0;
     }
     double ptb[S], qinb[6][S], dp0b[S], uab[S], vab[S], qab[S];
     // fields_update: tendencies of the final state
@@ -1200,11 +1230,12 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
       __syncthreads();
     }
     // -------------------------------------------------- fields_init, reversed
-    const double ccn = sh.colv[CV_CCN][0], cpraut = sh.colv[CV_CPRAUT][0];
-    const double ccnbar = A.colbar[CB_CCN] - A.colbar[CB_CPRAUT] * cpraut / (3.0 * ccn);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       if (!L.v[s]) continue;
+      const int cl = L.cl[s];
+      const double ccn = sh.colv[CV_CCN][cl], cpraut = sh.colv[CV_CPRAUT][cl];
+      const double ccnbar = A.colbar[CB_CCN][cl] - A.colbar[CB_CPRAUT][cl] * cpraut / (3.0 * ccn);
       const int32_t o = Gl.off[s];
       const double dp0 = f.delp[o], qv = f.qv[o], dp1 = X.dp1[s];
       const double omq = dp0 / dp1;
@@ -1222,10 +1253,10 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
       if (L.kb[s] == 0 && P.use_ccn) {
         // ccn = ccn0 rdgas tz / p1, p1 = -dp1 / (g dz0) rdgas t0, with tz = t0 = pt
         ptb[s] += ccnbar * ccn / f.pt[o];
-        const double p1b = -ccnbar * ccn / p1_b;
-        dp1b += p1b * p1_b / dp1;
-        dz0b -= p1b * p1_b / f.delz[o];
-        ptb[s] += p1b * p1_b / f.pt[o];
+        const double p1b = -ccnbar * ccn / p1_b[s];
+        dp1b += p1b * p1_b[s] / dp1;
+        dz0b -= p1b * p1_b[s] / f.delz[o];
+        ptb[s] += p1b * p1_b[s] / f.pt[o];
       }
       dp0b[s] += dp1b * (1.0 - qv);
       qinb[0][s] -= dp1b * dp0;

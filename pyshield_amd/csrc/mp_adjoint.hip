@@ -4,6 +4,10 @@
 
 namespace shield {
 
+struct AdjLayout {
+  int S, P;  // slots, columns per wavefront
+};
+
 __global__ __launch_bounds__(64) void adj_param_reduce(const double* part, int64_t nw, double* out) {
   const int p = threadIdx.x;
   if (p >= AP_N) return;
@@ -12,29 +16,51 @@ __global__ __launch_bounds__(64) void adj_param_reduce(const double* part, int64
   out[p] = acc;
 }
 
-int adjoint_slots(int64_t nk) { return (int)((nk + 63) / 64); }
+// P columns in S slots: the most busy lanes over S = 1..3 (the fewest slots
+// on a tie); 2 columns in 3 slots at nk = 79
+AdjLayout adjoint_layout(int64_t nk) {
+  AdjLayout best{0, 0};
+  double util = -1.0;
+  for (int S = 1; S <= 3; ++S) {
+    const int64_t p0 = (64 * S) / nk;
+    if (p0 < 1) continue;
+    const int P = (int)(p0 > pmax_for(S) ? pmax_for(S) : p0);
+    const double u = double(P * nk) / (64.0 * S);
+    if (u > util + 1e-9) {
+      util = u;
+      best = AdjLayout{S, P};
+    }
+  }
+  return best;
+}
+
+int adjoint_cols_per_wave(int64_t nk) { return adjoint_layout(nk).P; }
 
 // doubles of tape per wavefront
 int64_t adjoint_tape_doubles(int64_t nk, int ntimes) {
-  return (int64_t)ntimes * REC_FIELDS * 64 * adjoint_slots(nk);
+  return (int64_t)ntimes * REC_FIELDS * 64 * adjoint_layout(nk).S;
 }
 
 hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
                           const shield_mp_adjoint_fields& a, void* ws, size_t ws_bytes, hipStream_t stream) {
   const int64_t ncol = sh.ni * sh.nj;
+  const AdjLayout lay = adjoint_layout(sh.nk);
+  const int64_t groups = (ncol + lay.P - 1) / lay.P;
   const int64_t tape = adjoint_tape_doubles(sh.nk, p.ntimes);
   const int64_t per_wave = (tape + AP_N) * (int64_t)sizeof(double);
   int64_t nw = (int64_t)(ws_bytes / (size_t)per_wave);
-  if (nw > ncol) nw = ncol;
+  if (nw > groups) nw = groups;
   if (nw < 1) return hipErrorInvalidValue;
   double* base = static_cast<double*>(ws);
   double* part = base + nw * tape;
   const Coef<double> c = make_coef<double>(p);
   const dim3 grid((unsigned)nw), block(64);
-  if (adjoint_slots(sh.nk) == 1)
-    hipLaunchKernelGGL(mp_adjoint_kernel<1>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part);
+  if (lay.S == 1)
+    hipLaunchKernelGGL(mp_adjoint_kernel<1>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part, lay.P);
+  else if (lay.S == 2)
+    hipLaunchKernelGGL(mp_adjoint_kernel<2>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part, lay.P);
   else
-    hipLaunchKernelGGL(mp_adjoint_kernel<2>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part);
+    hipLaunchKernelGGL(mp_adjoint_kernel<3>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part, lay.P);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(adj_param_reduce, dim3(1), dim3(64), 0, stream, part, nw, a.params);

@@ -13,6 +13,7 @@ struct LayoutChoice {
 };
 LayoutChoice choose_layout(int64_t nz, int mode, int ntimes, bool fused);
 int64_t adjoint_tape_doubles(int64_t nk, int ntimes);
+int adjoint_cols_per_wave(int64_t nk);
 hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
                           const shield_mp_adjoint_fields& a, void* ws, size_t ws_bytes, hipStream_t stream);
 }  // namespace shield
@@ -253,8 +254,9 @@ int shield_mp_forward(const shield_mp_shape* shape, const shield_mp_params* p, c
 
 size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shield_mp_params* p, int64_t waves) {
   if (!shape || !p || p->ntimes < 1 || waves < 1 || !shape_ok(shape, 2, 128)) return 0;
-  const int64_t ncol = shape->ni * shape->nj;
-  if (waves > ncol) waves = ncol < 1 ? 1 : ncol;
+  const int64_t ncol = shape->ni * shape->nj, P = adjoint_cols_per_wave(shape->nk);
+  const int64_t groups = (ncol + P - 1) / P;
+  if (waves > groups) waves = groups < 1 ? 1 : groups;
   return (size_t)waves * (size_t)(adjoint_tape_doubles(shape->nk, p->ntimes) + SHIELD_MP_NPARAMS) * sizeof(double);
 }
 

@@ -211,12 +211,17 @@ def test_adjoint_rejects_before_launch():
     # fake addresses would reach the kernel)
     if _lib.device_count() == 0:
         assert call() == -8
-    # workspace: one tape of ntimes x 77 recorded fields x 128 cells per
-    # wavefront, plus six partial sums; bounded by the number of columns
-    per_wave = (1 * 77 * 128 + 6) * 8
+    # workspace: one tape of ntimes x 77 recorded fields x 192 cells (2
+    # columns of 79 levels in 3 slots) per wavefront, plus six partial sums;
+    # bounded by the number of column pairs
+    per_wave = (1 * 77 * 192 + 6) * 8
     ws_bytes = L.shield_mp_adjoint_workspace_bytes
-    assert ws_bytes(ctypes.byref(shape), ctypes.byref(p), 3) == 3 * per_wave
-    assert ws_bytes(ctypes.byref(shape), ctypes.byref(p), 100) == 4 * per_wave
+    assert ws_bytes(ctypes.byref(shape), ctypes.byref(p), 1) == per_wave
+    assert ws_bytes(ctypes.byref(shape), ctypes.byref(p), 3) == 2 * per_wave
+    assert ws_bytes(ctypes.byref(shape), ctypes.byref(p), 100) == 2 * per_wave
+    # 127 levels: one column in 2 slots
+    tall = _lib.ShieldMpShape(2, 2, 127, 2 * 127, 127, 2, 1)
+    assert ws_bytes(ctypes.byref(tall), ctypes.byref(p), 100) == 4 * (1 * 77 * 128 + 6) * 8
     assert ws_bytes(ctypes.byref(_lib.ShieldMpShape(2, 2, 200, 400, 200, 2, 1)), ctypes.byref(p), 3) == 0
 
 

@@ -83,12 +83,15 @@ def _gpu_vjp(st, qaz, cfg, dt, cot):
     return {n: v.cpu().numpy() for n, v in g.items()}
 
 
-@pytest.mark.parametrize("dt,seed", [(150.0, 1000), (225.0, 2001)])
-def test_adjoint_matches_autograd_of_oracle(dt, seed):
-    print(f"case dt={dt} seed={seed}", flush=True)
-    st = small_state(2, 3, 79, seed=seed)
+# (dt, seed, ni, nj, nz): 79 levels run 2 columns per wavefront in 3 slots (9
+# columns leave a last wavefront with one), 40 levels 3 columns in 2 slots
+@pytest.mark.parametrize("dt,seed,ni,nj,nz", [(150.0, 1000, 2, 3, 79), (225.0, 2001, 2, 3, 79),
+                                              (150.0, 3003, 3, 3, 79), (150.0, 4004, 5, 1, 40)])
+def test_adjoint_matches_autograd_of_oracle(dt, seed, ni, nj, nz):
+    print(f"case dt={dt} seed={seed} shape=({ni}, {nj}, {nz})", flush=True)
+    st = small_state(ni, nj, nz, seed=seed)
     qaz = np.random.default_rng(seed).uniform(0.0, 1.0, st["pt"].shape)
-    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=nz)
     cot = _cotangents(st, seed + 1)
     ref = _oracle_vjp(st, qaz, cfg, dt, cot)
     got = _gpu_vjp(st, qaz, cfg, dt, cot)
