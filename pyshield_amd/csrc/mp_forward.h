@@ -115,7 +115,11 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X
   for (int s = 0; s < S; ++s) {
     const double r = recip<R>(dz[s] + dd[s]);
     a[s] = ddp[s] * r;
-    b[s] = qq[s] * r;
+    // FP64: the reference's division, exact where nothing falls in from above
+    if constexpr (sizeof(R) == 8)
+      b[s] = qq[s] / (dz[s] + dd[s]);
+    else
+      b[s] = qq[s] * r;
   }
   scan_fwd(L, a, b, qm);
   // The reference sums the flux out of each layer bottom from the top,
@@ -167,6 +171,9 @@ __device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X
 template <int S, class R>
 __device__ __forceinline__ void sedi_heat(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const double (&flux)[S],
                                           double cq, const bool (&apply)[S]) {
+  // separately rounded products and sums, as the reference's: in a cell no
+  // flux passes, (cvn tz) / cvn then rounds as it does there
+#pragma clang fp contract(off)
   const int lane = lane_id();
   double fp[S], a[S], b[S], tn[S];
   col_prev(sh.buf, flux, fp);
@@ -178,6 +185,12 @@ __device__ __forceinline__ void sedi_heat(const Lay<S>& L, St<S, R>& X, Shared<S
     if (L.k[s] == 0) {
       a[s] = 0.0;
       b[s] = X.tz[s] + fdiv<R>(flux[s] * dgz, cvn + flux[s] * cq);
+    } else if constexpr (sizeof(R) == 8) {
+      // FP64: the reference's division; with no flux through the cell it
+      // returns (cvn tz) / cvn, which a reciprocal can miss by an ulp of tz
+      const double d = cvn + cq * flux[s];
+      a[s] = fp[s] * cq * recip<R>(d);
+      b[s] = ((cvn + cq * (flux[s] - fp[s])) * X.tz[s] + dgz * (fp[s] + flux[s])) / d;
     } else {
       const double r = recip<R>(cvn + cq * flux[s]);
       a[s] = fp[s] * cq * r;
@@ -794,8 +807,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
         const double dp0 = L.v[s] ? f.delp[o] : 100.0;
         const double r = recip<R>(dp0 + mp[s]);
         a[s] = mp[s] * r;
-        bu[s] = dp0 * u0 * r;
-        bv[s] = dp0 * v0 * r;
+        if constexpr (sizeof(R) == 8) {
+          // FP64: the reference's division, exact where no mass falls in
+          bu[s] = dp0 * u0 / (dp0 + mp[s]);
+          bv[s] = dp0 * v0 / (dp0 + mp[s]);
+        } else {
+          bu[s] = dp0 * u0 * r;
+          bv[s] = dp0 * v0 * r;
+        }
       }
     }
     scan_fwd(L, a, bu, u1);

@@ -1,0 +1,43 @@
+"""The Microph translate test on the HIP kernels: a synthetic savepoint in the
+reference's serialized layout, expected tendencies from the FP64 oracle, the
+reference's GPU-backend thresholds (overrides/standard.yaml, gt:gpu)."""
+
+import numpy as np
+import pytest
+
+import oracle
+from pyshield_amd import translate as T
+from pyshield_amd._config import PhysicsConfig
+from pyshield_amd.synthetic import make_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dt", [150.0, 225.0])
+def test_translate_microph_against_oracle_savepoint(dt):
+    cn, nz = 6, 79
+    st = make_state(cn, cn, nz, 5000 + int(dt))
+    print(f"case dt={dt} tile {cn}x{cn}x{nz}", flush=True)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=nz)
+    names = {"mph_qv1": "qv", "mph_ql1": "ql", "mph_qr1": "qr", "mph_qi1": "qi", "mph_qs1": "qs",
+             "mph_qg1": "qg", "mph_qa1": "qa", "mph_uin": "ua", "mph_vin": "va", "mph_delp": "delp",
+             "mph_w": "w", "mph_dz": "delz", "mph_pt": "pt"}
+    inputs = {ser: T.ijk_to_microph(st[n]) for ser, n in names.items()}
+    inputs["mph_land"] = st["land"].reshape(cn * cn, 1)
+    ref = oracle.run(cfg.mp_namelist(), dt, st)
+    expected = {ser: T.ijk_to_microph(ref[n]) for n, ser in T.MICROPH_OUT.items()}
+    out = T.TranslateMicroph(cfg, st["area"]).compute(inputs)
+    # A tendency is a difference of FP64 states over dt.  One ulp of a 275 K
+    # temperature is 5.7e-14 K, 3.8e-16 K/s at dt = 150 s: the 2.2e-8
+    # relative tolerance of a 1.7e-8 K/s tendency; one ulp of a 10 m/s wind
+    # over dt is 1.2e-17 m/s^2, that of a 5e-10 m/s^2 one.  Two FP64
+    # implementations that round a state by an ulp differ beyond the
+    # tolerance below those magnitudes, which is why the reference's GPU
+    # thresholds carry near-zero allowances: ql, qr, qg (overrides
+    # standard.yaml, gt:gpu) and udt, vdt at 1e-8 (baroclinic.yaml,
+    # dace:gpu).  pt_dt gets one by the same count, 1e-7 K/s (1.5e-5 K per
+    # call).
+    near_zero = dict(T.MICROPH_NEAR_ZERO, mph_udt=1e-8, mph_vdt=1e-8, mph_pt_dt=1e-7)
+    rep = T.compare(out, expected, near_zero=near_zero)
+    print({k: f"{v[1]:.1e}" for k, v in rep.items()}, flush=True)
+    assert all(v[0] for v in rep.values()), rep
