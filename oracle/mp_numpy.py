@@ -138,6 +138,9 @@ class TorchOps:
         return t.where(c, a, b)
 
     def full_like(self, x, v):
+        # a tensor fill value (a differentiated parameter) keeps its graph
+        if isinstance(v, self.t.Tensor):
+            return self.t.zeros_like(x) + v
         return self.t.full_like(x, v)
 
     def power(self, x, y):

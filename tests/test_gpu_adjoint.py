@@ -49,9 +49,14 @@ def _oracle_vjp(st, qaz, cfg, dt, cot):
         loss = loss + (out[n] * torch.tensor(cot[n].reshape(out[n].shape))).sum()
     loss.backward()
     shp = st["pt"].shape
-    g = {n: ts[n].grad.numpy().reshape(shp) for n in STATE}
-    g["qaz"] = tq.grad.numpy().reshape(shp)
-    g["params"] = np.array([float(fac_rc.grad)] + [float(pn[k].grad) for k in PARAMS[1:]])
+
+    def grad(t):
+        # an input the switches take out of the call gets no gradient
+        return np.zeros(shp) if t.grad is None else t.grad.numpy().reshape(shp)
+
+    g = {n: grad(ts[n]) for n in STATE}
+    g["qaz"] = grad(tq)
+    g["params"] = np.array([0.0 if p.grad is None else float(p.grad) for p in [fac_rc] + [pn[k] for k in PARAMS[1:]]])
     return g
 
 
@@ -102,6 +107,39 @@ def test_adjoint_matches_autograd_of_oracle(dt, seed, ni, nj, nz):
     assert all(np.isfinite(got[n]).all() for n in STATE + ("qaz",))
     assert all(v <= VJP_TOL for v in errs.values()), errs
     assert np.all(perr <= VJP_TOL), perr
+
+
+# every namelist switch the adjoint supports, away from its default
+SWITCHES = [
+    {"do_qa": True},
+    {"const_vi": True, "const_vs": True, "const_vg": True, "const_vr": True},
+    {"irain_f": 1},
+    {"z_slope_liq": False},
+    {"do_sedi_heat": False, "sedi_transport": False},
+    {"fast_sat_adj": True, "rad_snow": False, "rad_rain": False},
+    {"reset_qaz": True},
+]
+
+
+@pytest.mark.parametrize("switches", SWITCHES, ids=lambda d: "+".join(sorted(d)))
+def test_adjoint_switches_match_autograd_of_oracle(switches):
+    dt, seed = 150.0, 6006
+    print(f"case switches={switches}", flush=True)
+    st = small_state(2, 3, 79, seed=seed)
+    qaz = np.random.default_rng(seed).uniform(0.0, 1.0, st["pt"].shape)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79, **switches)
+    cot = _cotangents(st, seed + 1)
+    ref = _oracle_vjp(st, qaz, cfg, dt, cot)
+    got = _gpu_vjp(st, qaz, cfg, dt, cot)
+    errs = {n: norm_err(got[n], ref[n], floor=1e-300) for n in STATE + ("qaz",)}
+    perr = np.abs(got["params"] - ref["params"]) / np.maximum(np.abs(ref["params"]), 1e-300)
+    print({k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert all(np.isfinite(got[n]).all() for n in STATE + ("qaz",))
+    assert all(v <= VJP_TOL for v in errs.values()), errs
+    # a parameter the switches take out of the call has no gradient
+    live = np.abs(ref["params"]) > 0.0
+    assert np.all(perr[live] <= VJP_TOL), perr
+    assert np.all(np.abs(got["params"][~live]) == 0.0), got["params"]
 
 
 def test_adjoint_dot_product_with_finite_differences():
