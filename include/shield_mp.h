@@ -288,6 +288,45 @@ int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, c
  * when f->omga is set, prepare_microphysics, in one kernel.  nwat = 6.      */
 int shield_physics_prologue(const shield_mp_shape* shape, const shield_prologue_fields* f, void* stream);
 
+/* Fields of shield_apply_physics_to_dycore for one tile on one rank (all
+ * four tile edges local), C-contiguous, k = 0 the top, i in [0, ni), j in
+ * [0, nj):
+ *   layer fields [ni][nj][nk]; interface fields pe, peln, pk [ni][nj][nk+1];
+ *   2D fields [ni][nj];
+ *   u [ni][nj+1][nk] and v [ni+1][nj][nk], the D-grid winds;
+ *   u_dt, v_dt [ni+2][nj+2][nk], the A-grid wind tendencies with a one-point
+ *   halo (compute point (i, j) at [i+1][j+1]), filled by the caller;
+ *   vlon_*, vlat_* [ni+2][nj+2] with the same halo; es1_* [ni][nj+1];
+ *   ew2_* [ni+1][nj]; edge_vect_w, _e [nj]; edge_vect_s, _n [ni].          */
+typedef struct shield_dycore_fields {
+  const double *qvapor, *qliquid, *qrain, *qsnow, *qice, *qgraupel;
+  double *pt, *t_dt;
+  const double* delp;
+  double *pe, *peln, *pk, *ps;
+  const double *ua, *va;
+  double *u_srf, *v_srf;
+  double *u, *v, *u_dt, *v_dt;
+  const double *vlon1, *vlon2, *vlon3, *vlat1, *vlat2, *vlat3;
+  const double *es1_1, *es1_2, *es1_3, *ew2_1, *ew2_2, *ew2_3;
+  const double *edge_vect_w, *edge_vect_e, *edge_vect_s, *edge_vect_n;
+  double dt;      /* the physics time step (dt_atmos)                       */
+  double con_cp;  /* heat capacity of moist_cv (CP_AIR in the reference)     */
+  double kappa;   /* RDGAS / CP_AIR                                         */
+  int32_t grid_type, npx, npy, pad_;
+} shield_dycore_fields;
+
+/* The steps of ApplyPhysicsToDycore.__call__ (fv_update_phys.py:148-195)
+ * that pySHiELD holds, for one tile on one rank: moist_cv (:32-50),
+ * update_pressure_and_surface_winds (:53-74) and AGrid2DGridPhysics
+ * (update_dwind_phys.py:166-701; the doubly periodic form when
+ * grid_type > 3).  The u_dt / v_dt halo exchange before the wind update and
+ * CubedToLatLon after it (NDSL) are the caller's.  Only shape->ni, nj, nk
+ * are read.  The grid fields may be NULL when grid_type > 3.  Errors:
+ * SHIELD_ERR_NULL; SHIELD_ERR_SHAPE (ni or nj below 2, nk below 1, or
+ * npx - 1 != ni or npy - 1 != nj when grid_type <= 3); SHIELD_ERR_ALIAS (a
+ * written field equal to another field); SHIELD_ERR_TIMESTEP.               */
+int shield_apply_physics_to_dycore(const shield_mp_shape* shape, const shield_dycore_fields* f, void* stream);
+
 /* ------------------------------------------------------------ utilities */
 const char* shield_strerror(int code);
 /* Number of HIP devices visible (0 without a GPU; never fails).            */

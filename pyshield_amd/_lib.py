@@ -23,7 +23,7 @@ EXPORTS = (
     "shield_prepare_microphysics", "shield_update_physics_state_with_tendencies",
     "shield_fill_gfs_delp", "shield_prepare_tendencies_and_update_tracers",
     "shield_strerror", "shield_device_count", "shield_mp_layout", "shield_physics_prologue",
-    "shield_mp_adjoint", "shield_mp_adjoint_workspace_bytes",
+    "shield_mp_adjoint", "shield_mp_adjoint_workspace_bytes", "shield_apply_physics_to_dycore",
 )
 
 _D = ctypes.c_double
@@ -119,6 +119,18 @@ class ShieldUpdateFields(ctypes.Structure):
                 ("q_min", _D)]
 
 
+DYCORE_PTR_NAMES = ("qvapor", "qliquid", "qrain", "qsnow", "qice", "qgraupel", "pt", "t_dt", "delp", "pe", "peln",
+                    "pk", "ps", "ua", "va", "u_srf", "v_srf", "u", "v", "u_dt", "v_dt", "vlon1", "vlon2",
+                    "vlon3", "vlat1", "vlat2", "vlat3", "es1_1", "es1_2", "es1_3", "ew2_1", "ew2_2", "ew2_3",
+                    "edge_vect_w", "edge_vect_e", "edge_vect_s", "edge_vect_n")
+
+
+class ShieldDycoreFields(ctypes.Structure):
+    _fields_ = [(n, _VP) for n in DYCORE_PTR_NAMES] + [
+        ("dt", _D), ("con_cp", _D), ("kappa", _D), ("grid_type", _I32), ("npx", _I32), ("npy", _I32),
+        ("pad_", _I32)]
+
+
 _lib = None
 
 
@@ -143,6 +155,8 @@ def lib():
                                                ctypes.c_int, _VP]
         L.shield_physics_prologue.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldPrologueFields),
                                               _VP]
+        L.shield_apply_physics_to_dycore.argtypes = [ctypes.POINTER(ShieldMpShape),
+                                                     ctypes.POINTER(ShieldDycoreFields), _VP]
         L.shield_prepare_microphysics.argtypes = [ctypes.POINTER(ShieldMpShape), _VP, _VP, _VP, _VP, _VP,
                                                   _VP, _VP, _VP * 10, _VP]
         L.shield_update_physics_state_with_tendencies.argtypes = [ctypes.POINTER(ShieldMpShape), _VP * 10,

@@ -112,34 +112,50 @@ def _tensor(x):
 
 
 class UpdateAtmosphereState:
-    """UpdateAtmosphereState (pySHiELD/update/update_atmos_state.py:235-349)
-    without its ApplyPhysicsToDycore step: fill_gfs_delp of the updated vapour
-    (or, dycore-only, of the dycore vapour) and
-    prepare_tendencies_and_update_tracers, each one kernel on the GPU.
+    """UpdateAtmosphereState (pySHiELD/update/update_atmos_state.py:235-349):
+    fill_gfs_delp of the updated vapour (or, dycore-only, of the dycore
+    vapour), prepare_tendencies_and_update_tracers and, with
+    ``apply_tendencies``, ApplyPhysicsToDycore for one tile on one rank
+    (``pyshield_amd.dycore``), each one kernel on the GPU.
 
     Fields are the nk physical layers ([X, Y, Z] tensors or objects holding
     one in ``.data``); ``phy_state.prsi`` has nk + 1 interfaces.  The reference
     runs fill_gfs_delp over the halo columns too; it fills whatever columns
-    the passed tensors span.  ``apply_tendencies=True`` needs the cubed-sphere
-    A-to-D wind interpolation and a halo exchange, which are not part of this
-    library, and raises NotImplementedError.
+    the passed tensors span.  With ``apply_tendencies``, ``grid_info`` holds
+    the driver grid data and grid_type, ``namelist`` npx and npy, and the two
+    steps the reference takes from NDSL come as callables:
+    ``halo_update(u_dt, v_dt)`` (the one-point halo of the tendencies, from
+    ``comm`` in the reference) and ``cubed_to_latlon(u, v, ua, va)``; u_dt and
+    v_dt are the compute domain [ni, nj, nk] (ApplyPhysicsToDycore keeps the
+    halo'd copy).
     """
 
     def __init__(self, stencil_factory=None, grid_data=None, namelist=None, comm=None, grid_info=None,
                  state=None, quantity_factory=None, dycore_only: bool = False, apply_tendencies: bool = False,
-                 tendency_state=None):
-        if apply_tendencies:
-            raise NotImplementedError("ApplyPhysicsToDycore (fv_update_phys.py) is not part of pyshield_amd")
+                 tendency_state=None, halo_update=None, cubed_to_latlon=None):
         if namelist is None:
             raise ValueError("namelist (with dt_atmos) is required")
         self._rdt = 1.0 / float(namelist.dt_atmos)
         self._dycore_only = dycore_only
+        self._apply_tendencies = apply_tendencies
+        if apply_tendencies:
+            if grid_info is None:
+                raise ValueError("apply_tendencies needs grid_info (the driver grid data)")
+            from .dycore import ApplyPhysicsToDycore
+            self._apply_physics_to_dycore = ApplyPhysicsToDycore(namelist, grid_info, halo_update=halo_update,
+                                                                 cubed_to_latlon=cubed_to_latlon)
 
     def __call__(self, dycore_state, phy_state, u_dt, v_dt, pt_dt, dt: float):
         T = _tensor
         if self._dycore_only:
             fill_gfs_delp(T(dycore_state.delp), T(dycore_state.qvapor), 1.0e-9)
-            return
+        else:
+            self._physics_tendencies(dycore_state, phy_state, u_dt, v_dt, pt_dt)
+        if self._apply_tendencies:
+            self._apply_physics_to_dycore(dycore_state, u_dt, v_dt, pt_dt, dt=dt)
+
+    def _physics_tendencies(self, dycore_state, phy_state, u_dt, v_dt, pt_dt):
+        T = _tensor
         fill_gfs_delp(T(dycore_state.delp), T(phy_state.physics_updated_specific_humidity), 1.0e-9)
         prepare_tendencies_and_update_tracers(
             T(u_dt), T(v_dt), T(pt_dt), T(phy_state.physics_updated_ua), T(phy_state.physics_updated_va),

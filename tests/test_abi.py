@@ -115,10 +115,13 @@ def test_forward_update_rejects_bad_arguments():
 def test_update_atmosphere_state_scope():
     from types import SimpleNamespace
     from pyshield_amd.update import UpdateAtmosphereState
-    nml = SimpleNamespace(dt_atmos=225)
-    with pytest.raises(NotImplementedError):
-        UpdateAtmosphereState(namelist=nml, apply_tendencies=True)
+    nml = SimpleNamespace(dt_atmos=225, npx=5, npy=5)
+    with pytest.raises(ValueError):
+        UpdateAtmosphereState(namelist=nml, apply_tendencies=True)   # needs the grid data
     assert UpdateAtmosphereState(namelist=nml)._rdt == pytest.approx(1.0 / 225.0)
+    # the doubly periodic form needs no grid fields
+    op = UpdateAtmosphereState(namelist=nml, apply_tendencies=True, grid_info=SimpleNamespace(grid_type=4))
+    assert op._apply_physics_to_dycore.grid == {}
 
 
 def test_setup_errors():
@@ -238,3 +241,52 @@ def test_strerror_and_layout():
         _lib.layout(1000)
     with pytest.raises(_lib.ShieldError):
         _lib.layout(79, 7)                 # unknown precision mode
+
+
+def _fake_dycore(ni=4, nj=4):
+    f = _lib.ShieldDycoreFields()
+    for i, n in enumerate(_lib.DYCORE_PTR_NAMES):
+        setattr(f, n, 0x1000 * (i + 1))
+    f.dt, f.con_cp, f.kappa = 150.0, 1004.6, 287.05 / 1004.6
+    f.grid_type, f.npx, f.npy = 0, ni + 1, nj + 1
+    return f
+
+
+def test_dycore_struct_and_rejects():
+    # 37 pointers, three doubles, four int32
+    assert ctypes.sizeof(_lib.ShieldDycoreFields) == 37 * 8 + 3 * 8 + 4 * 4
+    L = _lib.lib()
+    call = L.shield_apply_physics_to_dycore
+    shape = _lib.ShieldMpShape(4, 4, 10, 40, 10, 4, 1)
+    assert call(None, ctypes.byref(_fake_dycore()), None) == -1
+    f = _fake_dycore()
+    f.delp = None
+    assert call(ctypes.byref(shape), ctypes.byref(f), None) == -1
+    f = _fake_dycore()
+    f.edge_vect_n = None
+    assert call(ctypes.byref(shape), ctypes.byref(f), None) == -1
+    f = _fake_dycore()
+    f.npx = 9                                   # one tile per rank: npx - 1 == ni
+    assert call(ctypes.byref(shape), ctypes.byref(f), None) == -2
+    assert call(ctypes.byref(_lib.ShieldMpShape(1, 4, 10, 40, 10, 4, 1)), ctypes.byref(_fake_dycore(1, 4)),
+                None) == -2
+    f = _fake_dycore()
+    f.u = f.u_dt                                # the wind written over its tendency
+    assert call(ctypes.byref(shape), ctypes.byref(f), None) == -5
+    f = _fake_dycore()
+    f.u_srf = f.ua
+    assert call(ctypes.byref(shape), ctypes.byref(f), None) == -5
+    f = _fake_dycore()
+    f.dt = 0.0
+    assert call(ctypes.byref(shape), ctypes.byref(f), None) == -7
+    # an empty tile is a no-op
+    assert call(ctypes.byref(_lib.ShieldMpShape(0, 4, 10, 40, 10, 4, 1)), ctypes.byref(_fake_dycore()), None) == 0
+    if _lib.device_count() == 0:
+        # every check passes: without a GPU the call reports it
+        assert call(ctypes.byref(shape), ctypes.byref(_fake_dycore()), None) == -8
+        f = _fake_dycore()
+        f.grid_type = 4
+        for n in _lib.DYCORE_PTR_NAMES[21:]:
+            setattr(f, n, None)                 # the doubly periodic form needs no grid
+        f.npx = 100
+        assert call(ctypes.byref(shape), ctypes.byref(f), None) == -8
