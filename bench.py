@@ -10,8 +10,10 @@ physics.py:133-143); the zeroing is fused into the kernel.
 Default workload: C384 (6 x 384 x 384 columns x 79 levels, dt 150 s),
 mixed precision.  The JSON line also carries the plain FP64 kernel's rate on
 the same state, the dominant kernel's HBM roofline fraction (algorithmic
-bytes / HIP-event kernel time / 8 TB/s) and the CPU baseline (the C oracle on
-the host cores, on a bounded sample of the same workload).
+bytes / HIP-event kernel time / 8 TB/s), the FP64 adjoint's time, the
+ApplyPhysicsToDycore kernels' time on one tile (384 x 384 x 79 at C384) and
+the CPU baseline (the C oracle on the host cores, on a bounded sample of the
+same workload).
 """
 
 import argparse
@@ -42,6 +44,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the FP64 adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=2)
+    ap.add_argument("--no-dycore", action="store_true", help="skip the ApplyPhysicsToDycore timing")
     ap.add_argument("--only-kernel", action="store_true",
                     help="launch only the timed kernel (for profiler runs)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -157,6 +160,49 @@ def cpu_baseline(name, seconds):
                       f"FP64, median of {reps} calls"}
 
 
+# ApplyPhysicsToDycore per cell: the column step reads 9 layer fields and
+# writes pt, t_dt and the 3 interface fields; the wind step reads and writes
+# u, v, reads u_dt, v_dt and clears them (FP64)
+B_DYCORE_CELL = 8 * (9 + 2 + 3 + 4 + 2 + 2)
+
+
+def time_dycore(n, nk, steps):
+    """shield_apply_physics_to_dycore on one n x n tile of nk levels
+    (synthetic fields and grid metrics; its three kernels, HIP events on the
+    launch stream); returns the times in ms."""
+    import torch
+    from pyshield_amd.dycore import GRID_NAMES, apply_physics_to_dycore
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(3)
+
+    def u(shape, lo, hi):
+        return torch.rand(shape, generator=g, device=dev, dtype=torch.float64) * (hi - lo) + lo
+
+    f = {q: u((n, n, nk), 0.0, 2e-3) for q in ("qvapor", "qliquid", "qrain", "qsnow", "qice", "qgraupel")}
+    f.update(pt=u((n, n, nk), 200, 300), t_dt=u((n, n, nk), -1e-4, 1e-4), delp=u((n, n, nk), 50, 2000),
+             ua=u((n, n, nk), -10, 10), va=u((n, n, nk), -10, 10), u=u((n, n + 1, nk), -10, 10),
+             v=u((n + 1, n, nk), -10, 10), u_dt=u((n + 2, n + 2, nk), -1e-3, 1e-3),
+             v_dt=u((n + 2, n + 2, nk), -1e-3, 1e-3))
+    for q in ("pe", "peln", "pk"):
+        f[q] = torch.full((n, n, nk + 1), 300.0, dtype=torch.float64, device=dev)
+    for q in ("ps", "u_srf", "v_srf"):
+        f[q] = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    shapes = {**{q: (n + 2, n + 2) for q in GRID_NAMES[:6]}, **{q: (n, n + 1) for q in GRID_NAMES[6:9]},
+              **{q: (n + 1, n) for q in GRID_NAMES[9:12]}, **{q: (n,) for q in GRID_NAMES[12:]}}
+    grid = {q: u(shp, -1, 1) if not q.startswith("edge") else u(shp, 0, 1) for q, shp in shapes.items()}
+    apply_physics_to_dycore(f, grid, 150.0, n + 1, n + 1)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        apply_physics_to_dycore(f, grid, 150.0, n + 1, n + 1)
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    return ms
+
+
 def main():
     args = parse()
     if args.copy_bytes:
@@ -210,6 +256,13 @@ def main():
                       "adjoint_algorithmic_bytes": adj_bytes,
                       "adjoint_roofline_frac": adj_bytes / (ams * 1e-3) / 1e9 / HBM_PEAK_GBS})
         print(f"# adjoint (fp64): kernel {ams:.3f} ms", file=sys.stderr, flush=True)
+
+    if not args.no_dycore and not args.only_kernel:
+        dms = float(np.mean(time_dycore(w.ni, w.nz, 5)))
+        dbytes = w.ni * w.nj * w.nz * B_DYCORE_CELL
+        extra.update({"dycore_tile_ms": dms, "dycore_algorithmic_bytes": dbytes,
+                      "dycore_roofline_frac": dbytes / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        print(f"# apply_physics_to_dycore ({w.ni}x{w.nj}x{w.nz} tile): {dms:.3f} ms", file=sys.stderr, flush=True)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"mp_forward_{w.name}_{args.mode}_traffic.json")

@@ -322,9 +322,11 @@ typedef struct shield_dycore_fields {
  * grid_type > 3).  The u_dt / v_dt halo exchange before the wind update and
  * CubedToLatLon after it (NDSL) are the caller's.  Only shape->ni, nj, nk
  * are read.  The grid fields may be NULL when grid_type > 3.  Errors:
- * SHIELD_ERR_NULL; SHIELD_ERR_SHAPE (ni or nj below 2, nk below 1, or
+ * SHIELD_ERR_NULL; SHIELD_ERR_SHAPE (ni or nj below 2, nk outside
+ * [1, SHIELD_DYCORE_MAX_NK], or
  * npx - 1 != ni or npy - 1 != nj when grid_type <= 3); SHIELD_ERR_ALIAS (a
  * written field equal to another field); SHIELD_ERR_TIMESTEP.               */
+#define SHIELD_DYCORE_MAX_NK 4096
 int shield_apply_physics_to_dycore(const shield_mp_shape* shape, const shield_dycore_fields* f, void* stream);
 
 /* ------------------------------------------------------------ utilities */

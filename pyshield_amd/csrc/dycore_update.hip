@@ -4,7 +4,7 @@
 // AGrid2DGridPhysics (pySHiELD/update/update_dwind_phys.py).
 //
 // Two kernels and a clear:
-//  * a column kernel, one thread per column: the point-wise temperature
+//  * a column kernel over blocks of whole columns: the point-wise temperature
 //    update and the interface-pressure recurrence summed from the top;
 //  * a D-grid kernel, one thread per (i, j, k) face point with k fastest, so
 //    a wavefront reads consecutive levels of the same faces: each thread
@@ -22,20 +22,32 @@ using namespace shield;
 namespace {
 
 constexpr int kBlock = 256;
+// LDS of the column kernel's delp / pe stage; bounds nk (one column at least)
+constexpr int64_t kStageBytes = 40960;
 
 struct Dims {
   int64_t ni, nj, nk;
 };
 
 // moist_cv (fv_update_phys.py:32-50) and update_pressure_and_surface_winds
-// (:53-74)
-__global__ __launch_bounds__(kBlock) void dycore_column_kernel(Dims d, shield_dycore_fields f) {
-  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (col >= d.ni * d.nj) return;
+// (:53-74).  A block takes `ncb` whole columns, whose cells are one
+// contiguous run of ncb * nk values (and their interfaces one run of
+// ncb * (nk + 1)), so every pass below reads and writes consecutive
+// addresses with consecutive threads:
+//  1. moist_cv, one thread per cell; delp staged in LDS;
+//  2. the interface pressures, one thread per column summing its delp from
+//     the top in the reference's order (pe[k] = pe[k-1] + delp[k-1]) in LDS;
+//  3. pe, peln = log pe, pk = exp(kappa peln), one thread per interface.
+__global__ __launch_bounds__(kBlock) void dycore_column_kernel(Dims d, shield_dycore_fields f, int ncb) {
+  extern __shared__ double stage[];  // [ncb][nk]: delp, then pe of interfaces 1..nk
   const int nk = (int)d.nk;
-  const int64_t b = col * nk, bi = col * (nk + 1);
-  for (int k = 0; k < nk; ++k) {
-    const int64_t o = b + k;
+  const int64_t ncol = d.ni * d.nj;
+  const int64_t c0 = (int64_t)blockIdx.x * ncb;
+  const int nc = (int)(ncol - c0 < ncb ? ncol - c0 : ncb);
+  const int ncell = nc * nk;
+  const int64_t b = c0 * nk;
+  for (int e = threadIdx.x; e < ncell; e += kBlock) {
+    const int64_t o = b + e;
     const double qv = f.qvapor[o];
     const double ql = f.qliquid[o] + f.qrain[o];
     const double qs = f.qice[o] + f.qsnow[o] + f.qgraupel[o];
@@ -43,18 +55,33 @@ __global__ __launch_bounds__(kBlock) void dycore_column_kernel(Dims d, shield_dy
     const double cvm = (1.0 - (qv + gz)) * CV_AIR + qv * CV_VAP + ql * C_LIQ + qs * C_ICE;
     f.pt[o] = f.pt[o] + f.t_dt[o] * f.dt * f.con_cp / cvm;
     f.t_dt[o] = 0.0;
+    stage[e] = f.delp[o];
   }
-  double pe = f.pe[bi];
-  for (int k = 1; k <= nk; ++k) {
-    pe = pe + f.delp[b + k - 1];
-    f.pe[bi + k] = pe;
+  __syncthreads();
+  if ((int)threadIdx.x < nc) {
+    const int64_t col = c0 + threadIdx.x;
+    double* s = stage + threadIdx.x * nk;
+    double pe = f.pe[col * (nk + 1)];
+    for (int k = 0; k < nk; ++k) {
+      pe = pe + s[k];
+      s[k] = pe;
+    }
+    f.ps[col] = pe;
+    f.u_srf[col] = f.ua[col * nk + nk - 1];
+    f.v_srf[col] = f.va[col * nk + nk - 1];
+  }
+  __syncthreads();
+  const int nint = nc * (nk + 1);
+  const int64_t bi = c0 * (nk + 1);
+  for (int e = threadIdx.x; e < nint; e += kBlock) {
+    const int cl = e / (nk + 1), k = e - cl * (nk + 1);
+    if (k == 0) continue;  // the top interface is the caller's
+    const double pe = stage[cl * nk + k - 1];
     const double peln = log(pe);
-    f.peln[bi + k] = peln;
-    f.pk[bi + k] = exp(f.kappa * peln);
+    f.pe[bi + e] = pe;
+    f.peln[bi + e] = peln;
+    f.pk[bi + e] = exp(f.kappa * peln);
   }
-  f.ps[col] = pe;
-  f.u_srf[col] = f.ua[b + nk - 1];
-  f.v_srf[col] = f.va[b + nk - 1];
 }
 
 struct Wind {
@@ -148,13 +175,17 @@ __global__ __launch_bounds__(kBlock) void dycore_clear_kernel(double* u_dt, doub
 
 }  // namespace
 
+static_assert(kStageBytes / (8 * SHIELD_DYCORE_MAX_NK) >= 1, "one column of delp must fit the LDS stage");
+
 namespace shield {
 
 hipError_t launch_dycore_update(const shield_mp_shape& sh, const shield_dycore_fields& f, hipStream_t stream) {
   const Dims d{sh.ni, sh.nj, sh.nk};
   const int64_t ncol = sh.ni * sh.nj;
-  hipLaunchKernelGGL(dycore_column_kernel, dim3((unsigned)((ncol + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
-                     d, f);
+  // columns per block: up to 64, within kStageBytes of LDS
+  const int ncb = (int)(kStageBytes / (8 * sh.nk) < 64 ? kStageBytes / (8 * sh.nk) : 64);
+  hipLaunchKernelGGL(dycore_column_kernel, dim3((unsigned)((ncol + ncb - 1) / ncb)), dim3(kBlock),
+                     (size_t)ncb * sh.nk * sizeof(double), stream, d, f, ncb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   Wind w;
@@ -205,7 +236,7 @@ extern "C" int shield_apply_physics_to_dycore(const shield_mp_shape* shape, cons
     for (const void* q : grid)
       if (q && out[a] == q) return SHIELD_ERR_ALIAS;
   }
-  if (shape->ni < 0 || shape->nj < 0 || shape->nk < 1) return SHIELD_ERR_SHAPE;
+  if (shape->ni < 0 || shape->nj < 0 || shape->nk < 1 || shape->nk > SHIELD_DYCORE_MAX_NK) return SHIELD_ERR_SHAPE;
   if (shape->ni == 0 || shape->nj == 0) return SHIELD_OK;
   // one tile on one rank: the edge split points of the wind update lie
   // inside the tile

@@ -78,7 +78,7 @@ def _check(got, want, where):
 
 
 @pytest.mark.parametrize("ni,nj,nk,grid_type", [(8, 8, 79, 0), (7, 10, 5, 0), (3, 3, 4, 0), (2, 2, 1, 0),
-                                                (48, 48, 79, 0), (6, 5, 7, 4)])
+                                                (48, 48, 79, 0), (6, 5, 7, 4), (9, 7, 300, 0)])
 def test_apply_physics_to_dycore_matches_oracle(ni, nj, nk, grid_type):
     import torch
     from pyshield_amd.dycore import apply_physics_to_dycore
