@@ -11,8 +11,8 @@ Default workload: C384 (6 x 384 x 384 columns x 79 levels, dt 150 s),
 mixed precision.  The JSON line also carries the plain FP64 kernel's rate on
 the same state, the dominant kernel's HBM roofline fraction (algorithmic
 bytes / HIP-event kernel time / 8 TB/s), the FP64 adjoint's time, the
-ApplyPhysicsToDycore kernels' time on one tile (384 x 384 x 79 at C384) and
-the CPU baseline (the C oracle on the host cores, on a bounded sample of the
+ApplyPhysicsToDycore kernels' and the physics prologue's times on one tile
+(384 x 384 x 79 at C384) and the CPU baseline (the C oracle on the host cores, on a bounded sample of the
 same workload).
 """
 
@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the FP64 adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=2)
-    ap.add_argument("--no-dycore", action="store_true", help="skip the ApplyPhysicsToDycore timing")
+    ap.add_argument("--no-dycore", action="store_true",
+                    help="skip the ApplyPhysicsToDycore and physics-prologue timings")
     ap.add_argument("--only-kernel", action="store_true",
                     help="launch only the timed kernel (for profiler runs)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -203,6 +204,55 @@ def time_dycore(n, nk, steps):
     return ms
 
 
+# physics prologue per cell: reads delz, delp, 7 tracers, qsgs_tke, pt, omga
+# (12) and writes delp, 7 tracers, qsgs_tke, dm, delprsi, del_gz, phil, dz,
+# wmp and 10 zeroed tendencies (24), plus 3 interface fields (FP64)
+B_PROLOGUE_CELL = 8 * (12 + 24 + 3)
+
+
+def time_prologue(n, nk, steps):
+    """shield_physics_prologue on one n x n tile of nk levels (synthetic
+    state, restored before each call outside the timed region); returns the
+    times in ms."""
+    import torch
+    from pyshield_amd.physics import SPECIES, physics_prologue
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(5)
+
+    def u(lo, hi, shape=(n, n, nk)):
+        return torch.rand(shape, generator=g, device=dev, dtype=torch.float64) * (hi - lo) + lo
+
+    src = {"delz": u(-600.0, -20.0), "delp": u(50.0, 2000.0), "qsgs_tke": u(0.0, 1.0), "pt": u(200.0, 300.0),
+           "omga": u(-1.0, 1.0)}
+    for sp in SPECIES:
+        src[sp] = u(1e-6, 2e-2) if sp == SPECIES[0] else u(0.0, 1e-3)
+    work = {k: v.clone() for k, v in src.items()}
+    out = {k: torch.zeros((n, n, nk), dtype=torch.float64, device=dev)
+           for k in ("dm", "delprsi", "del_gz", "phil", "dz", "wmp")}
+    iface = {k: torch.zeros((n, n, nk + 1), dtype=torch.float64, device=dev) for k in ("phii", "prsi", "prsik")}
+    tend = [torch.zeros((n, n, nk), dtype=torch.float64, device=dev) for _ in range(10)]
+
+    def call():
+        physics_prologue(work["delz"], work["delp"], {sp: work[sp] for sp in SPECIES}, work["qsgs_tke"],
+                         work["pt"], iface["phii"], iface["prsi"], iface["prsik"], out["dm"], out["delprsi"],
+                         out["del_gz"], out["phil"], 300.0, omga=work["omga"], dz=out["dz"], wmp=out["wmp"],
+                         tend=tend)
+
+    call()
+    ms = []
+    for _ in range(steps):
+        for k, v in src.items():
+            work[k].copy_(v)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        call()
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    return ms
+
+
 def main():
     args = parse()
     if args.copy_bytes:
@@ -263,6 +313,13 @@ def main():
         extra.update({"dycore_tile_ms": dms, "dycore_algorithmic_bytes": dbytes,
                       "dycore_roofline_frac": dbytes / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS})
         print(f"# apply_physics_to_dycore ({w.ni}x{w.nj}x{w.nz} tile): {dms:.3f} ms", file=sys.stderr, flush=True)
+
+    if not args.no_dycore and not args.only_kernel:
+        pms = float(np.mean(time_prologue(w.ni, w.nz, 5)))
+        pbytes = w.ni * w.nj * w.nz * B_PROLOGUE_CELL
+        extra.update({"prologue_tile_ms": pms, "prologue_algorithmic_bytes": pbytes,
+                      "prologue_roofline_frac": pbytes / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        print(f"# physics_prologue ({w.ni}x{w.nj}x{w.nz} tile): {pms:.3f} ms", file=sys.stderr, flush=True)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"mp_forward_{w.name}_{args.mode}_traffic.json")

@@ -285,7 +285,9 @@ int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, c
 /* The column-local part of Physics.__call__ before the microphysics
  * (physics.py:278-330): atmos_phys_driver_statein (physics.py:26-102),
  * get_prs_fv3 (get_prs_fv3.py:7-20), get_phi_fv3 (get_phi_fv3.py:7-25) and,
- * when f->omga is set, prepare_microphysics, in one kernel.  nwat = 6.      */
+ * when f->omga is set, prepare_microphysics, in one kernel.  nwat = 6;
+ * 1 <= nk <= SHIELD_PROLOGUE_MAX_NK.                                        */
+#define SHIELD_PROLOGUE_MAX_NK 1024
 int shield_physics_prologue(const shield_mp_shape* shape, const shield_prologue_fields* f, void* stream);
 
 /* Fields of shield_apply_physics_to_dycore for one tile on one rank (all

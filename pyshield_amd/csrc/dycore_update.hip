@@ -6,11 +6,11 @@
 // Two kernels and a clear:
 //  * a column kernel over blocks of whole columns: the point-wise temperature
 //    update and the interface-pressure recurrence summed from the top;
-//  * a D-grid kernel, one thread per (i, j, k) face point with k fastest, so
-//    a wavefront reads consecutive levels of the same faces: each thread
-//    rebuilds the 3D tendency vectors of the two A points beside its face
-//    (and, on a tile edge, of its neighbour face), blends along the edge and
-//    projects on the face's unit vector;
+//  * a D-grid kernel over tiles of faces and levels: the block builds the 3D
+//    tendency vectors of the A points its faces read in LDS, then each
+//    thread averages the two beside its face (and, on a tile edge, those of
+//    its neighbour face), blends along the edge and projects on the face's
+//    unit vector (the doubly periodic form: a thread per point, no tiles);
 //  * u_dt and v_dt cleared after the D-grid kernel has read them, as the
 //    reference's set_winds_zero (grid_type <= 3).
 #include <math.h>
@@ -88,52 +88,87 @@ struct Wind {
   int64_t ni, nj, nk;
   double dt5;
   int im2, jm2;  // edge split points in compute coordinates
-  // the 3D tendency vector component c at A point (ia, ja) and level k
-  __device__ __forceinline__ double v3(const shield_dycore_fields& f, int c, int64_t ia, int64_t ja,
-                                       int64_t k) const {
-    const int64_t a = (ia + 1) * (nj + 2) + (ja + 1);
-    const double* lon = c == 0 ? f.vlon1 : c == 1 ? f.vlon2 : f.vlon3;
-    const double* lat = c == 0 ? f.vlat1 : c == 1 ? f.vlat2 : f.vlat3;
-    return f.u_dt[a * nk + k] * lon[a] + f.v_dt[a * nk + k] * lat[a];
-  }
 };
 
-// AGrid2DGridPhysics.__call__ (update_dwind_phys.py:490-701)
-__global__ __launch_bounds__(kBlock) void dycore_dwind_kernel(Wind w, shield_dycore_fields f, int grid_type) {
-  const int64_t n = (w.ni + 1) * (w.nj + 1) * w.nk;
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n) return;
-  const int64_t k = t % w.nk, ij = t / w.nk;
-  const int64_t i = ij / (w.nj + 1), j = ij - i * (w.nj + 1);
+// The D-grid kernel's block: TI x TJ faces (i, j) by KC levels, k fastest so
+// a wavefront's lanes read 16-level runs of 4 faces.  The block first builds
+// the 3D tendency vector u_dt vlon + v_dt vlat of each of the (TI + 2) x
+// (TJ + 2) A points its faces and their edge neighbours read (A points
+// i0 - 1 .. i0 + TI, j0 - 1 .. j0 + TJ) in LDS, once per point and level.
+constexpr int TI = 4, TJ = 4, KC = 16, AI = TI + 2, AJ = TJ + 2;
+static_assert(TI * TJ * KC == kBlock, "one thread per face and level of the tile");
+
+// Workgroups are dispatched to the XCDs round robin (block b to XCD b % 8);
+// the (tile, level chunk) pairs are numbered with the chunk fastest and each
+// XCD takes one contiguous range of them, so a tile's chunks and its
+// neighbours, which read the same halo A points and cache lines, share an L2.
+constexpr int kXcds = 8;
+
+// AGrid2DGridPhysics.__call__ for grid_type <= 3 (update_dwind_phys.py:490-701)
+__global__ __launch_bounds__(kBlock) void dycore_dwind_kernel(Wind w, shield_dycore_fields f, int64_t per_xcd) {
+  __shared__ double v3s[3][AI * AJ * KC];
+  __shared__ double met[6][AI * AJ];  // vlon1..3, vlat1..3 of the tile's A points
   const int64_t ni = w.ni, nj = w.nj, nk = w.nk;
-  if (grid_type > 3) {
-    // doubly_periodic_wind_update (:153-163), compute domain
-    if (i < ni && j < nj) {
-      auto A = [&](int64_t ia, int64_t ja) { return ((ia + 1) * (nj + 2) + (ja + 1)) * nk + k; };
-      f.u[(i * (nj + 1) + j) * nk + k] += w.dt5 * (f.u_dt[A(i, j - 1)] + f.u_dt[A(i, j)]);
-      f.v[(i * nj + j) * nk + k] += w.dt5 * (f.v_dt[A(i - 1, j)] + f.v_dt[A(i, j)]);
-    }
-    return;
+  const int64_t nkc = (nk + KC - 1) / KC, ntj = (nj + 1 + TJ - 1) / TJ, nti = (ni + 1 + TI - 1) / TI;
+  const int64_t job = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (job >= nti * ntj * nkc) return;  // the whole block
+  const int64_t k0 = (job % nkc) * KC, tile = job / nkc;
+  const int64_t i0 = (tile / ntj) * TI, j0 = (tile % ntj) * TJ;
+  // the grid metrics do not depend on k: one load per A point
+  if (threadIdx.x < AI * AJ) {
+    const int p = threadIdx.x;
+    const int64_t ia = i0 - 1 + p / AJ, ja = j0 - 1 + p % AJ;
+    const bool in = ia <= ni && ja <= nj;
+    const int64_t a = in ? (ia + 1) * (nj + 2) + (ja + 1) : 0;
+    met[0][p] = in ? f.vlon1[a] : 0.0;
+    met[1][p] = in ? f.vlon2[a] : 0.0;
+    met[2][p] = in ? f.vlon3[a] : 0.0;
+    met[3][p] = in ? f.vlat1[a] : 0.0;
+    met[4][p] = in ? f.vlat2[a] : 0.0;
+    met[5][p] = in ? f.vlat3[a] : 0.0;
   }
+  __syncthreads();
+  // update_dwind_prep_stencil (:21-46): v3 at the tile's A points; points
+  // outside the tendencies' one-point halo are never read by a face
+  for (int e = threadIdx.x; e < AI * AJ * KC; e += kBlock) {
+    const int ek = e % KC, p = e / KC;
+    const int64_t ia = i0 - 1 + p / AJ, ja = j0 - 1 + p % AJ, k = k0 + ek;
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    if (ia <= ni && ja <= nj && k < nk) {
+      const int64_t a = (ia + 1) * (nj + 2) + (ja + 1);
+      const double ud = f.u_dt[a * nk + k], vd = f.v_dt[a * nk + k];
+      x0 = ud * met[0][p] + vd * met[3][p];
+      x1 = ud * met[1][p] + vd * met[4][p];
+      x2 = ud * met[2][p] + vd * met[5][p];
+    }
+    v3s[0][e] = x0;
+    v3s[1][e] = x1;
+    v3s[2][e] = x2;
+  }
+  __syncthreads();
+  const int kk = threadIdx.x % KC, r = threadIdx.x / KC;
+  const int ti = r / TJ, tj = r % TJ;
+  const int64_t i = i0 + ti, j = j0 + tj, k = k0 + kk;
+  if (k >= nk || i > ni || j > nj) return;
+  // the vector component c at A point (i0 - 1 + pi, j0 - 1 + pj)
+  auto V = [&](int c, int pi, int pj) { return v3s[c][(pi * AJ + pj) * KC + kk]; };
   if (i < ni) {
-    // u face (i, j), j in [0, nj]: ue = v3 of the A points south and north
-    // (update_dwind_prep_stencil, :21-46)
+    // u face (i, j), j in [0, nj]: the A points south (j - 1) and north (j)
     double e[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) e[c] = w.v3(f, c, i, j - 1, k) + w.v3(f, c, i, j, k);
+    for (int c = 0; c < 3; ++c) e[c] = V(c, ti + 1, tj) + V(c, ti + 1, tj + 1);
     if (j == 0 || j == nj) {
-      // south and north tile edges: blend with the face west (i + 1 side
-      // below im2) or east (above), :49-106 and the ranges of :236-433
+      // south and north tile edges: blend with the face at i + 1 below im2,
+      // at i - 1 above (:49-106 and the ranges of :236-433)
       const bool north = j == nj;
       const double* ev = north ? f.edge_vect_n : f.edge_vect_s;
       const bool lower = (north ? 0 < w.im2 : 0 <= w.im2) && i <= (w.im2 < ni - 1 ? w.im2 : ni - 1);
       const bool upper = (north ? ni - 1 >= w.im2 : ni - 1 > w.im2) && i >= (w.im2 + 1 > 0 ? w.im2 + 1 : 0);
       if (lower || upper) {
-        const int64_t nb = lower ? i + 1 : i - 1;
+        const int pb = lower ? ti + 2 : ti;
         const double g = ev[i];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          e[c] = g * (w.v3(f, c, nb, j - 1, k) + w.v3(f, c, nb, j, k)) + (1.0 - g) * e[c];
+        for (int c = 0; c < 3; ++c) e[c] = g * (V(c, pb, tj) + V(c, pb, tj + 1)) + (1.0 - g) * e[c];
       }
     }
     // update_uwind_stencil (:123-135)
@@ -141,28 +176,40 @@ __global__ __launch_bounds__(kBlock) void dycore_dwind_kernel(Wind w, shield_dyc
     f.u[fu * nk + k] += w.dt5 * (e[0] * f.es1_1[fu] + e[1] * f.es1_2[fu] + e[2] * f.es1_3[fu]);
   }
   if (j < nj) {
-    // v face (i, j), i in [0, ni]: ve = v3 of the A points west and east
+    // v face (i, j), i in [0, ni]: the A points west (i - 1) and east (i)
     double e[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) e[c] = w.v3(f, c, i - 1, j, k) + w.v3(f, c, i, j, k);
+    for (int c = 0; c < 3; ++c) e[c] = V(c, ti, tj + 1) + V(c, ti + 1, tj + 1);
     if (i == 0 || i == ni) {
-      // west and east tile edges: blend with the face north (j + 1 side
-      // below jm2) or south (above)
+      // west and east tile edges: blend with the face at j + 1 below jm2,
+      // at j - 1 above
       const double* ev = i == 0 ? f.edge_vect_w : f.edge_vect_e;
       const bool lower = 0 <= w.jm2 && j <= (w.jm2 < nj - 1 ? w.jm2 : nj - 1);
       const bool upper = nj - 1 > w.jm2 && j >= (w.jm2 + 1 > 0 ? w.jm2 + 1 : 0);
       if (lower || upper) {
-        const int64_t nb = lower ? j + 1 : j - 1;
+        const int pb = lower ? tj + 2 : tj;
         const double g = ev[j];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          e[c] = g * (w.v3(f, c, i - 1, nb, k) + w.v3(f, c, i, nb, k)) + (1.0 - g) * e[c];
+        for (int c = 0; c < 3; ++c) e[c] = g * (V(c, ti, pb) + V(c, ti + 1, pb)) + (1.0 - g) * e[c];
       }
     }
     // update_vwind_stencil (:138-150)
     const int64_t fv = i * nj + j;
     f.v[fv * nk + k] += w.dt5 * (e[0] * f.ew2_1[fv] + e[1] * f.ew2_2[fv] + e[2] * f.ew2_3[fv]);
   }
+}
+
+// doubly_periodic_wind_update (:153-163) over the compute domain, one thread
+// per (i, j, k) with k fastest
+__global__ __launch_bounds__(kBlock) void dycore_periodic_kernel(Wind w, shield_dycore_fields f) {
+  const int64_t ni = w.ni, nj = w.nj, nk = w.nk;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int64_t k = t % nk, ij = t / nk;
+  const int64_t i = ij / nj, j = ij - i * nj;
+  auto A = [&](int64_t ia, int64_t ja) { return ((ia + 1) * (nj + 2) + (ja + 1)) * nk + k; };
+  f.u[(i * (nj + 1) + j) * nk + k] += w.dt5 * (f.u_dt[A(i, j - 1)] + f.u_dt[A(i, j)]);
+  f.v[(i * nj + j) * nk + k] += w.dt5 * (f.v_dt[A(i - 1, j)] + f.v_dt[A(i, j)]);
 }
 
 // set_winds_zero (:12-18) over the tendencies and their halo
@@ -197,11 +244,18 @@ hipError_t launch_dycore_update(const shield_mp_shape& sh, const shield_dycore_f
   // halo (update_dwind_phys.py:187-188)
   w.im2 = (f.npx - 1) / 2 + 2 - 3;
   w.jm2 = (f.npy - 1) / 2 + 2 - 3;
-  const int64_t nw = (sh.ni + 1) * (sh.nj + 1) * sh.nk;
-  hipLaunchKernelGGL(dycore_dwind_kernel, dim3((unsigned)((nw + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, w,
-                     f, (int)f.grid_type);
+  if (f.grid_type > 3) {
+    const int64_t np = sh.ni * sh.nj * sh.nk;
+    hipLaunchKernelGGL(dycore_periodic_kernel, dim3((unsigned)((np + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
+                       w, f);
+    return hipGetLastError();
+  }
+  const int64_t njob = ((sh.ni + 1 + TI - 1) / TI) * ((sh.nj + 1 + TJ - 1) / TJ) * ((sh.nk + KC - 1) / KC);
+  const int64_t per_xcd = (njob + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL(dycore_dwind_kernel, dim3((unsigned)(per_xcd * kXcds)), dim3(kBlock), 0, stream, w, f,
+                     per_xcd);
   e = hipGetLastError();
-  if (e != hipSuccess || f.grid_type > 3) return e;
+  if (e != hipSuccess) return e;
   const int64_t nh = (sh.ni + 2) * (sh.nj + 2) * sh.nk;
   hipLaunchKernelGGL(dycore_clear_kernel, dim3((unsigned)((nh + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
                      f.u_dt, f.v_dt, nh);

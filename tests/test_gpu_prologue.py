@@ -33,17 +33,32 @@ def _t(x):
     return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda:0").clone()
 
 
-def test_prologue_matches_restatement():
+def _halo_view(x, h=2, pad=3):
+    """x inside a larger NaN-filled buffer (halo h in i and j, pad extra
+    levels), returned as a strided view."""
+    import torch
+    ni, nj, nk = x.shape
+    big = torch.full((ni + 2 * h, nj + 2 * h, nk + pad), float("nan"), dtype=torch.float64, device="cuda:0")
+    big[h:h + ni, h:h + nj, :nk] = torch.as_tensor(np.ascontiguousarray(x), device="cuda:0")
+    return big[h:h + ni, h:h + nj, :nk]
+
+
+# (shape, strided): the default tile; ragged blocks of columns (23 columns of
+# 140 levels); fields inside halo'd buffers
+@pytest.mark.parametrize("shape,strided", [((6, 5, 79), False), ((23, 1, 140), False), ((4, 7, 79), True)])
+def test_prologue_matches_restatement(shape, strided):
     import torch
     from pyshield_amd.physics import physics_prologue
-    st, q, tke, omga = _state()
+    print(f"prologue case {shape} strided={strided}", flush=True)
+    st, q, tke, omga = _state(*shape)
     ni, nj, nz = st["pt"].shape
-    z = lambda k: torch.zeros((ni, nj, k), dtype=torch.float64, device="cuda:0")  # noqa: E731
-    g = {"delz": _t(st["delz"]), "delp": _t(st["delp"]), "qsgs_tke": _t(tke), "pt": _t(st["pt"]),
+    mk = _halo_view if strided else _t
+    z = lambda k: mk(np.zeros((ni, nj, k)))  # noqa: E731
+    g = {"delz": mk(st["delz"]), "delp": mk(st["delp"]), "qsgs_tke": mk(tke), "pt": mk(st["pt"]),
          "phii": z(nz + 1), "prsi": z(nz + 1), "prsik": z(nz + 1), "dm": z(nz), "delprsi": z(nz),
-         "del_gz": z(nz), "phil": z(nz), "omga": _t(omga), "dz": z(nz), "wmp": z(nz)}
-    gq = {n: _t(q[n]) for n in PR.SPECIES}
-    tend = [_t(np.full((ni, nj, nz), 5.0)) for _ in range(10)]
+         "del_gz": z(nz), "phil": z(nz), "omga": mk(omga), "dz": z(nz), "wmp": z(nz)}
+    gq = {n: mk(q[n]) for n in PR.SPECIES}
+    tend = [mk(np.full((ni, nj, nz), 5.0)) for _ in range(10)]
     physics_prologue(g["delz"], g["delp"], gq, g["qsgs_tke"], g["pt"], g["phii"], g["prsi"], g["prsik"], g["dm"],
                      g["delprsi"], g["del_gz"], g["phil"], PTOP, omga=g["omga"], dz=g["dz"], wmp=g["wmp"],
                      tend=tend)
