@@ -253,6 +253,30 @@ def time_prologue(n, nk, steps):
     return ms
 
 
+def time_fill(n, nk, steps):
+    """fill_gfs_delp on one n x n tile of nk levels (vapour near q_min so the
+    fills run; restored before each call); returns the times in ms."""
+    import torch
+    from pyshield_amd.update import fill_gfs_delp
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(7)
+    q0 = torch.rand((n, n, nk), generator=g, device=dev, dtype=torch.float64) * 3e-9 - 1e-9
+    delp = torch.rand((n, n, nk), generator=g, device=dev, dtype=torch.float64) * 1900.0 + 100.0
+    q = q0.clone()
+    fill_gfs_delp(delp, q, 1.0e-9)
+    ms = []
+    for _ in range(steps):
+        q.copy_(q0)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fill_gfs_delp(delp, q, 1.0e-9)
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    return ms
+
+
 def main():
     args = parse()
     if args.copy_bytes:
@@ -315,6 +339,9 @@ def main():
         print(f"# apply_physics_to_dycore ({w.ni}x{w.nj}x{w.nz} tile): {dms:.3f} ms", file=sys.stderr, flush=True)
 
     if not args.no_dycore and not args.only_kernel:
+        fms = float(np.mean(time_fill(w.ni, w.nz, 5)))
+        extra.update({"fill_gfs_delp_tile_ms": fms})
+        print(f"# fill_gfs_delp ({w.ni}x{w.nj}x{w.nz} tile): {fms:.3f} ms", file=sys.stderr, flush=True)
         pms = float(np.mean(time_prologue(w.ni, w.nz, 5)))
         pbytes = w.ni * w.nj * w.nz * B_PROLOGUE_CELL
         extra.update({"prologue_tile_ms": pms, "prologue_algorithmic_bytes": pbytes,

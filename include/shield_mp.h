@@ -200,7 +200,8 @@ int shield_update_physics_state_with_tendencies(const shield_mp_shape* shape,
 
 /* fill_gfs_delp (pySHiELD/update/update_atmos_state.py:21-37): vertical
  * filling of q below q_min, in place, over the nk layers of the reference's
- * nk+1-level domain.                                                         */
+ * nk+1-level domain; 1 <= nk <= SHIELD_FILL_MAX_NK.                         */
+#define SHIELD_FILL_MAX_NK 2048
 int shield_fill_gfs_delp(const shield_mp_shape* shape, const double* delp,
                          double* q, double q_min, void* stream);
 

@@ -192,7 +192,7 @@ class PhysicsConfig:
             # The reference reads a Fortran namelist with f90nml, which is not
             # available here; the override path is rejected loudly.
             raise NotImplementedError(
-                "namelist_override needs f90nml, which this build does not ship"
+                "namelist_override needs f90nml, which is not installed"
             )
 
     @classmethod

@@ -6,8 +6,8 @@
 //
 // The point-wise stencils run one thread per cell over the flattened
 // (column, k) index, so dense [X, Y, Z] storage is read fully coalesced.
-// fill_gfs_delp is a pair of non-linear vertical recurrences; it runs one
-// thread per column.
+// fill_gfs_delp is a pair of non-linear vertical recurrences; it stages
+// blocks of whole columns in LDS and runs one thread per column there.
 #include <math.h>
 
 #include "mp_common.h"
@@ -17,6 +17,9 @@ using namespace shield;
 namespace {
 
 constexpr int kBlock = 256;
+// LDS of fill_gfs_delp's stage (q and delp of the block's columns)
+constexpr int64_t kFillStageBytes = 40960;
+static_assert(kFillStageBytes / (16 * SHIELD_FILL_MAX_NK) >= 1, "one column must fit the LDS stage");
 
 struct Idx {
   int64_t ni, nj, nk, si, sj;
@@ -63,22 +66,64 @@ __global__ __launch_bounds__(kBlock) void forward_euler_kernel(Idx ix, Ten in, T
   for (int n = 0; n < 10; ++n) out.p[n][o] = in.p[n][o] + tend.p[n][o] * dt;
 }
 
-__global__ __launch_bounds__(kBlock) void fill_gfs_delp_kernel(Idx ix, const double* delp, double* q, double q_min) {
-  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (col >= ix.ni * ix.nj) return;
-  const int64_t i = col / ix.nj, j = col - i * ix.nj;
-  const int64_t b = i * ix.si + j * ix.sj;
+// fill_gfs_delp (update_atmos_state.py:21-37) over a block of `ncb`
+// consecutive columns: q and delp are staged in LDS with consecutive threads
+// on consecutive levels, one thread per column runs the vertical passes there
+// (every element gets the reference's operations in the reference's order),
+// and q is written back the same way.
+__global__ __launch_bounds__(kBlock) void fill_gfs_delp_kernel(Idx ix, const double* delp, double* q, double q_min,
+                                                               int ncb) {
+  extern __shared__ double stage[];  // [ncb][nk] of q, then [ncb][nk] of delp
   const int nk = (int)ix.nk;
-  double* qc = q + b;
-  const double* dp = delp + b;
-  for (int k = nk - 2; k >= 0; --k)
-    if (qc[k + 1] < q_min) qc[k] = qc[k] + (qc[k + 1] - q_min) * dp[k + 1] / dp[k];
-  for (int k = 1; k < nk; ++k)
-    if (qc[k] < q_min) qc[k] = q_min;
-  for (int k = 1; k < nk; ++k)
-    if (qc[k - 1] < 0.0) qc[k] = qc[k] + qc[k - 1] * (dp[k - 1]) / (dp[k]);
-  for (int k = 0; k < nk; ++k)
-    if (qc[k] < 0.0) qc[k] = 0.0;
+  const int64_t ncol = ix.ni * ix.nj;
+  const int64_t c0 = (int64_t)blockIdx.x * ncb;
+  const int nc = (int)(ncol - c0 < ncb ? ncol - c0 : ncb);
+  double* qs = stage;
+  double* ds = stage + ncb * nk;
+  auto base = [&](int cl) {
+    const int64_t col = c0 + cl, i = col / ix.nj, j = col - i * ix.nj;
+    return i * ix.si + j * ix.sj;
+  };
+  for (int e = threadIdx.x; e < nc * nk; e += kBlock) {
+    const int cl = e / nk, k = e - cl * nk;
+    const int64_t o = base(cl) + k;
+    qs[e] = q[o];
+    ds[e] = delp[o];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nc) {
+    double* qc = qs + threadIdx.x * nk;
+    const double* dp = ds + threadIdx.x * nk;
+    // upward: fill from the layer below (its value as just updated)
+    double qn = qc[nk - 1], dn = dp[nk - 1];
+    for (int k = nk - 2; k >= 0; --k) {
+      double qk = qc[k];
+      const double dk = dp[k];
+      if (qn < q_min) qk = qk + (qn - q_min) * dn / dk;
+      qc[k] = qk;
+      qn = qk;
+      dn = dk;
+    }
+    // downward, the reference's three passes in one: the floor at q_min
+    // (k >= 1), the borrow from a negative layer above (its value after its
+    // own borrow, before the final clamp), the clamp at zero
+    double qp = qc[0], dpv = dp[0];
+    qc[0] = qp < 0.0 ? 0.0 : qp;
+    for (int k = 1; k < nk; ++k) {
+      double qk = qc[k];
+      if (qk < q_min) qk = q_min;
+      const double dk = dp[k];
+      if (qp < 0.0) qk = qk + qp * (dpv) / (dk);
+      qc[k] = qk < 0.0 ? 0.0 : qk;
+      qp = qk;
+      dpv = dk;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nc * nk; e += kBlock) {
+    const int cl = e / nk, k = e - cl * nk;
+    q[base(cl) + k] = qs[e];
+  }
 }
 
 struct TracerArgs {
@@ -165,13 +210,16 @@ int shield_update_physics_state_with_tendencies(const shield_mp_shape* shape, co
 }
 
 int shield_fill_gfs_delp(const shield_mp_shape* shape, const double* delp, double* q, double q_min, void* stream) {
-  if (!shape_ok(shape)) return SHIELD_ERR_SHAPE;
+  if (!shape_ok(shape) || shape->nk > SHIELD_FILL_MAX_NK) return SHIELD_ERR_SHAPE;
   if (!delp || !q) return SHIELD_ERR_NULL;
   const int64_t ncol = shape->ni * shape->nj;
   if (ncol == 0) return SHIELD_OK;
   const Idx ix{shape->ni, shape->nj, shape->nk, shape->si, shape->sj, shape->si2, shape->sj2};
-  hipLaunchKernelGGL(fill_gfs_delp_kernel, dim3(grid_for(ncol)), dim3(kBlock), 0, (hipStream_t)stream, ix, delp, q,
-                     q_min);
+  // columns per block: up to 64 within kFillStageBytes of LDS
+  const int64_t per_col = 16 * shape->nk;
+  const int ncb = (int)(kFillStageBytes / per_col < 64 ? kFillStageBytes / per_col : 64);
+  hipLaunchKernelGGL(fill_gfs_delp_kernel, dim3((unsigned)((ncol + ncb - 1) / ncb)), dim3(kBlock),
+                     (size_t)ncb * per_col, (hipStream_t)stream, ix, delp, q, q_min, ncb);
   return finish();
 }
 

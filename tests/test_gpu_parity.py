@@ -101,7 +101,7 @@ def _mixed_vs_oracle(dt, seed):
 
 
 @pytest.mark.parametrize("dt,seed", [(150.0, 1000), (225.0, 2001)])
-def test_mixed_within_north_star_tolerance(dt, seed):
+def test_mixed_within_stated_tolerance(dt, seed):
     nm, no = _mixed_vs_oracle(dt, seed)
     terr = float(np.max(np.abs(nm["pt"] - no["pt"])))
     print("T err", terr, flush=True)

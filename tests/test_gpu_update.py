@@ -64,16 +64,19 @@ def test_update_physics_state_with_tendencies():
         np.testing.assert_allclose(o.cpu().numpy(), r, rtol=1e-15, atol=1e-16)
 
 
-def test_fill_gfs_delp():
+# the default shape; ragged blocks of 12 columns (200 levels); halo'd storage
+@pytest.mark.parametrize("shape,strided", [((NI, NJ, NK), False), ((9, 11, 200), False), ((NI, NJ, NK), True)])
+def test_fill_gfs_delp(shape, strided):
     import torch
     from pyshield_amd.update import fill_gfs_delp
     rng = np.random.default_rng(3)
-    q = rng.uniform(-2e-9, 5e-9, (NI, NJ, NK))
+    q = rng.uniform(-2e-9, 5e-9, shape)
     q[0, 0, :] = -1e-8          # a column that stays negative down to the surface
     q[1, 1, 10:20] = 1e-3
-    delp = rng.uniform(100, 2000, (NI, NJ, NK))
-    tq = _t(q)
-    fill_gfs_delp(_t(delp), tq, 1.0e-9)
+    delp = rng.uniform(100, 2000, shape)
+    mk = _halo_view if strided else _t
+    tq = mk(q)
+    fill_gfs_delp(mk(delp), tq, 1.0e-9)
     torch.cuda.synchronize()
     ref = U.fill_gfs_delp(delp, q, 1.0e-9)
     np.testing.assert_allclose(tq.cpu().numpy(), ref, rtol=1e-15, atol=1e-30)
