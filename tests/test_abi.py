@@ -290,3 +290,23 @@ def test_dycore_struct_and_rejects():
             setattr(f, n, None)                 # the doubly periodic form needs no grid
         f.npx = 100
         assert call(ctypes.byref(shape), ctypes.byref(f), None) == -8
+
+
+def test_level_limits_of_the_staged_kernels():
+    # the dycore update, the prologue and fill_gfs_delp stage whole columns in
+    # LDS and reject columns taller than their stage
+    L = _lib.lib()
+    big = _lib.ShieldMpShape(4, 4, 4097, 4097 * 4, 4097, 4, 1)
+    assert L.shield_apply_physics_to_dycore(ctypes.byref(big), ctypes.byref(_fake_dycore()), None) == -2
+    f = _lib.ShieldPrologueFields()
+    for i, n in enumerate(n for n, _ in f._fields_ if n not in ("q", "tend", "si_i", "sj_i", "ptop", "omga",
+                                                                "dz", "wmp")):
+        setattr(f, n, 0x1000 * (i + 1))
+    for i in range(7):
+        f.q[i] = 0x100000 + 0x1000 * i
+    f.si_i, f.sj_i, f.ptop = 1026 * 4, 1026, 300.0
+    tall = _lib.ShieldMpShape(3, 4, 1025, 1025 * 4, 1025, 0, 0)
+    assert L.shield_physics_prologue(ctypes.byref(tall), ctypes.byref(f), None) == -2
+    tall = _lib.ShieldMpShape(3, 4, 2049, 2049 * 4, 2049, 0, 0)
+    assert L.shield_fill_gfs_delp(ctypes.byref(tall), ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000),
+                                  ctypes.c_double(1e-9), None) == -2
