@@ -4,8 +4,9 @@
   code (oracle/mp_numpy.py under TorchOps): the vector-Jacobian product for
   random output cotangents, for the twelve input fields, qaz and the six
   process parameters, at one and two sub-steps;
-* central finite differences of the FP64 GPU forward along random directions
-  (the tangent-linear / adjoint dot-product identity <J v, w> = <v, J^T w>);
+* the tangent-linear / adjoint dot-product identity <J v, w> = <v, J^T w>,
+  with J v from forward-mode AD of the same FP64 transcription (to 1e-10) and
+  from central finite differences of the FP64 GPU forward;
 * the torch.autograd.Function wrapper.
 """
 
@@ -58,6 +59,36 @@ def _oracle_vjp(st, qaz, cfg, dt, cot):
     g["qaz"] = grad(tq)
     g["params"] = np.array([0.0 if p.grad is None else float(p.grad) for p in [fac_rc] + [pn[k] for k in PARAMS[1:]]])
     return g
+
+
+def oracle_jvp_dot(st, qaz, cfg, dt, v, vq, vp, cot):
+    """<J v, w>: the tangent-linear of the oracle's torch transcription by
+    forward-mode AD along (v, vq, vp) -- the twelve input fields, qaz and the
+    six process parameters -- dotted with the output cotangents w = cot."""
+    import torch
+    import torch.autograd.forward_ad as fwAD
+    flat = {k: (x.reshape(-1, x.shape[-1]) if x.ndim == 3 else x.reshape(-1)) for k, x in st.items()}
+    nml = cfg.mp_namelist()
+    t64 = lambda x: torch.tensor(x, dtype=torch.float64)  # noqa: E731
+    with fwAD.dual_level():
+        ts = {k: fwAD.make_dual(t64(x), t64(v[k].reshape(x.shape))) if k in STATE else t64(x) for k, x in flat.items()}
+        shp = flat["pt"].shape
+        tq = fwAD.make_dual(t64(qaz.reshape(shp)), t64(vq.reshape(shp)))
+        pn = {k: fwAD.make_dual(t64(float(nml[k])), t64(vp[i + 1])) for i, k in enumerate(PARAMS[1:])}
+        fac_rc = fwAD.make_dual(t64(M.derived(nml, dt)["fac_rc"]), t64(vp[0]))
+        out = M.run(dict(nml, **pn), dt, ts, qaz=tq, xp=M.TorchOps(), derived_overrides={"fac_rc": fac_rc})
+        total = 0.0
+        for n in OUTS:
+            tan = fwAD.unpack_dual(out[n]).tangent
+            if tan is not None:
+                total += float((tan.numpy() * cot[n].reshape(tan.shape)).sum())
+    return total
+
+
+def _directions(st, seed):
+    rng = np.random.default_rng(seed)
+    v = {n: rng.normal(size=st[n].shape) * (np.abs(st[n]) + 1e-12) for n in STATE}
+    return v, rng.normal(size=st["pt"].shape) * 1e-3, rng.normal(size=6) * 1e-2
 
 
 def _gpu_fields(st, qaz):
@@ -184,6 +215,31 @@ def test_adjoint_dot_product_with_finite_differences():
     # central differences at h = 1e-7 carry ~1e-6 relative truncation and
     # round-off; a process threshold crossed inside +-h shows as a larger gap
     assert sum(abs(fd - ad) / abs(ad) <= 1e-4 for fd, ad in results) >= 2, results
+
+
+# the tangent-linear / adjoint identity in FP64: forward-mode AD of the oracle
+# against the hand-written adjoint on the GPU, bar 1e-10 relative
+DOT_TOL = 1e-10
+
+
+@pytest.mark.parametrize("dt,seed", [(150.0, 77), (225.0, 78)])
+def test_adjoint_dot_product_with_tangent_linear(dt, seed):
+    """<J v, w> from forward-mode AD of the FP64 oracle equals <v, J^T w>
+    from the adjoint, for random directions v over the input fields, qaz and
+    the six parameters, and random output cotangents w."""
+    st = small_state(2, 3, 79, seed=seed)
+    qaz = np.random.default_rng(seed).uniform(0.0, 1.0, st["pt"].shape)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79)
+    cot = _cotangents(st, seed + 1)
+    got = _gpu_vjp(st, qaz, cfg, dt, cot)
+    for trial in range(2):
+        v, vq, vp = _directions(st, 100 * seed + trial)
+        jv_w = oracle_jvp_dot(st, qaz, cfg, dt, v, vq, vp, cot)
+        v_jtw = (sum(float((got[n] * v[n]).sum()) for n in STATE) + float((got["qaz"] * vq).sum())
+                 + float((got["params"] * vp).sum()))
+        rel = abs(jv_w - v_jtw) / abs(jv_w)
+        print(f"direction {trial}: <Jv,w> {jv_w:.15e} <v,J^T w> {v_jtw:.15e} rel {rel:.1e}", flush=True)
+        assert rel <= DOT_TOL, (jv_w, v_jtw)
 
 
 def test_autograd_function_matches_adjoint():
