@@ -10,7 +10,8 @@ physics.py:133-143); the zeroing is fused into the kernel.
 Default workload: C384 (6 x 384 x 384 columns x 79 levels, dt 150 s),
 mixed precision.  The JSON line also carries the plain FP64 kernel's rate on
 the same state, the dominant kernel's HBM roofline fraction (algorithmic
-bytes / HIP-event kernel time / 8 TB/s), the FP64 adjoint's time, the
+bytes / HIP-event kernel time / 8 TB/s), the FP64 adjoint's time, the fused
+microphysics + state update against its separate kernels, the
 ApplyPhysicsToDycore kernels' and the physics prologue's times on one tile
 (384 x 384 x 79 at C384) and the CPU baseline (the C oracle on the host cores, on a bounded sample of the
 same workload).
@@ -253,6 +254,60 @@ def time_prologue(n, nk, steps):
     return ms
 
 
+def time_fused(f, params, mode, dt, steps):
+    """The microphysics call followed by the Physics / UpdateAtmosphereState
+    state update (forward Euler, fill_gfs_delp, the tracer conversion): the
+    fused kernel (shield_mp_forward_update) and the four separate launches,
+    on the bench state (synthetic update inputs); returns (fused ms,
+    separate ms), HIP events around each sequence."""
+    import torch
+    from pyshield_amd import _lib
+    from pyshield_amd.microphysics import forward, forward_update
+    from pyshield_amd.update import (fill_gfs_delp, prepare_tendencies_and_update_tracers,
+                                     update_physics_state_with_tendencies)
+    ref = f["pt"]
+    ni, nj, nk = ref.shape
+    z = lambda: torch.zeros_like(ref)  # noqa: E731
+    prsi = torch.empty((ni, nj, nk + 1), dtype=torch.float64, device=ref.device)
+    prsi[..., 0] = 300.0
+    prsi[..., 1:] = 300.0 + torch.cumsum(f["delp"], dim=2)
+    upd = {"updated": {n: z() for n in _lib.UPDATED_NAMES}, "u_dt": z(), "v_dt": z(), "pt_dt": z(),
+           "tracers": {n: f[src].clone() for n, src in zip(_lib.TRACER_NAMES, ("qv", "ql", "qr", "qs", "qi", "qg"))},
+           "delp": f["delp"].clone(), "prsi": prsi}
+    src = ("qv", "ql", "qr", "qi", "qs", "qg", "qa", "pt", "ua", "va")
+    tend = ("qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa_dt", "pt_dt", "udt", "vdt")
+
+    def fused():
+        forward_update(f, upd, params, dt=dt, rdt=1.0 / dt, mode=mode)
+
+    def separate():
+        forward(f, params, mode)
+        u = upd["updated"]
+        update_physics_state_with_tendencies(*[f[n] for n in src], *[f[n] for n in tend],
+                                             *[u[n] for n in _lib.UPDATED_NAMES], dt)
+        fill_gfs_delp(upd["delp"], u["qvapor"], 1.0e-9)
+        tr = upd["tracers"]
+        prepare_tendencies_and_update_tracers(
+            upd["u_dt"], upd["v_dt"], upd["pt_dt"], u["ua"], u["va"], u["pt"], u["qvapor"], u["qliquid"],
+            u["qrain"], u["qsnow"], u["qice"], u["qgraupel"], f["ua"], f["va"], f["pt"],
+            *[tr[n] for n in _lib.TRACER_NAMES], upd["prsi"], upd["delp"], 1.0 / dt)
+
+    out = []
+    for run in (fused, separate):
+        run()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            run()
+            b.record()
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        out.append(float(np.mean(ms)))
+    return out
+
+
 def time_fill(n, nk, steps):
     """fill_gfs_delp on one n x n tile of nk levels (vapour near q_min so the
     fills run; restored before each call); returns the times in ms."""
@@ -339,6 +394,10 @@ def main():
         print(f"# apply_physics_to_dycore ({w.ni}x{w.nj}x{w.nz} tile): {dms:.3f} ms", file=sys.stderr, flush=True)
 
     if not args.no_dycore and not args.only_kernel:
+        fu_ms, sep_ms = time_fused(f, params, args.mode, w.timestep, 3)
+        extra.update({"fused_update_ms": fu_ms, "separate_update_ms": sep_ms})
+        print(f"# microphysics + state update: fused {fu_ms:.3f} ms, separate kernels {sep_ms:.3f} ms",
+              file=sys.stderr, flush=True)
         fms = float(np.mean(time_fill(w.ni, w.nz, 5)))
         extra.update({"fill_gfs_delp_tile_ms": fms})
         print(f"# fill_gfs_delp ({w.ni}x{w.nj}x{w.nz} tile): {fms:.3f} ms", file=sys.stderr, flush=True)

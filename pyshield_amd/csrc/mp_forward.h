@@ -566,11 +566,12 @@ __device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>&
 // ------------------------------------------------------------------ kernel
 // Waves per SIMD the kernel is compiled for.  The single-sub-step
 // mixed-precision kernel with at most 3 slots fits three waves (168 VGPRs)
-// with a few bytes of spill; every other instance is left to the register
-// allocator.
+// with a few bytes of spill, and so do the single-sub-step fused instances,
+// whose state-update epilogue would otherwise push the whole kernel to two
+// waves; every other instance is left to the register allocator.
 template <int S, class R, bool MULTI, bool FUSED>
 constexpr int min_waves() {
-  return (S <= 3 && sizeof(R) == 4 && !MULTI && !FUSED) ? 3 : 1;
+  return (S <= 3 && !MULTI && (sizeof(R) == 4 || FUSED)) ? 3 : 1;
 }
 
 // MULTI: more than one sub-step.  With a single sub-step the body is not
