@@ -277,7 +277,9 @@ size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shi
  * (accumulate = 0).  The forward of each column is recomputed, so f's inputs
  * must be those of the forward call.  The workspace (ws, ws_bytes) bounds
  * the number of concurrent wavefronts; nk <= 128.  Returns
- * SHIELD_ERR_ADJ_SWITCH for do_sedi_w, z_slope_ice, fix_negative or de_ice.
+ * SHIELD_ERR_ADJ_SWITCH for do_sedi_w, z_slope_ice, fix_negative or de_ice,
+ * and SHIELD_ERR_ALIAS when a written cotangent, the parameter gradient or
+ * the workspace shares memory with another of them or with a field read.
  * No reference counterpart: the reference has no derivative code; this is
  * the torch.autograd backward of the microphysics call.                     */
 int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,

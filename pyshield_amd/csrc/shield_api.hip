@@ -276,6 +276,22 @@ int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, c
                         a->ua,    a->va,    a->delp,  a->delz,  a->params};
   for (const void* q : need)
     if (!q) return SHIELD_ERR_NULL;
+  // the written cotangents, the parameter gradient and the workspace are
+  // distinct from each other and from everything the kernel reads
+  const void* wr[] = {a->pt, a->qv, a->ql, a->qr, a->qi, a->qs, a->qg, a->qa, a->ua, a->va, a->delp, a->delz,
+                      a->qaz, a->params, ws};
+  const void* rd[] = {f->pt,    f->qv,    f->ql,    f->qr,    f->qi,    f->qs,    f->qg,    f->qa,
+                      f->ua,    f->va,    f->delp,  f->delz,  f->land,  f->area,  f->qaz,   a->pt_dt,
+                      a->qv_dt, a->ql_dt, a->qr_dt, a->qi_dt, a->qs_dt, a->qg_dt, a->qa_dt, a->udt,
+                      a->vdt,   a->rain,  a->snow,  a->ice,   a->graupel};
+  const int nwr = (int)(sizeof(wr) / sizeof(wr[0]));
+  for (int i = 0; i < nwr; ++i) {
+    if (!wr[i]) continue;
+    for (int j = i + 1; j < nwr; ++j)
+      if (wr[i] == wr[j]) return SHIELD_ERR_ALIAS;
+    for (const void* q : rd)
+      if (q && wr[i] == q) return SHIELD_ERR_ALIAS;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SHIELD_ERR_NOGPU;
   if (ws_bytes < shield_mp_adjoint_workspace_bytes(shape, p, 1)) return SHIELD_ERR_WORKSPACE;
