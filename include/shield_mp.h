@@ -285,6 +285,22 @@ size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shi
 int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
                       const shield_mp_adjoint_fields* a, void* ws, size_t ws_bytes, void* stream);
 
+/* shield_mp_adjoint without a workspace argument and without any device
+ * allocation: the tapes and parameter partial sums go to a device array the
+ * library holds (SHIELD_MP_ADJOINT_SCRATCH_BYTES per GPU, for the life of the
+ * process), which bounds the number of concurrent wavefronts exactly as a
+ * workspace of that size does.  Same inputs, outputs, checks and results as
+ * shield_mp_adjoint.  Returns SHIELD_ERR_WORKSPACE when one wavefront's tape
+ * (ntimes x 77 x 64 x slots doubles) does not fit the array.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller.
+ * No counterpart in the reference.  It is for callers that cannot allocate
+ * device memory: code under HIP graph capture, serving loops and harnesses
+ * that forbid hipMalloc.                                                      */
+#define SHIELD_MP_ADJOINT_SCRATCH_BYTES ((size_t)128 << 20)
+int shield_mp_adjoint_noalloc(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                              const shield_mp_adjoint_fields* a, void* stream);
+
 /* The column-local part of Physics.__call__ before the microphysics
  * (physics.py:278-330): atmos_phys_driver_statein (physics.py:26-102),
  * get_prs_fv3 (get_prs_fv3.py:7-20), get_phi_fv3 (get_phi_fv3.py:7-25) and,

@@ -24,6 +24,7 @@ EXPORTS = (
     "shield_fill_gfs_delp", "shield_prepare_tendencies_and_update_tracers",
     "shield_strerror", "shield_device_count", "shield_mp_layout", "shield_physics_prologue",
     "shield_mp_adjoint", "shield_mp_adjoint_workspace_bytes", "shield_apply_physics_to_dycore",
+    "shield_mp_adjoint_noalloc",
 )
 
 _D = ctypes.c_double
@@ -171,6 +172,9 @@ def lib():
         L.shield_mp_adjoint.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldMpParams),
                                         ctypes.POINTER(ShieldMpFields), ctypes.POINTER(ShieldMpAdjointFields),
                                         _VP, ctypes.c_size_t, _VP]
+        L.shield_mp_adjoint_noalloc.argtypes = [ctypes.POINTER(ShieldMpShape), ctypes.POINTER(ShieldMpParams),
+                                                ctypes.POINTER(ShieldMpFields), ctypes.POINTER(ShieldMpAdjointFields),
+                                                _VP]
         L.shield_mp_adjoint_workspace_bytes.argtypes = [ctypes.POINTER(ShieldMpShape),
                                                         ctypes.POINTER(ShieldMpParams), _I64]
         for fn in EXPORTS:

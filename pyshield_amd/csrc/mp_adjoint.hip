@@ -1,8 +1,17 @@
 // mp_adjoint.hip -- launch of the adjoint kernel (mp_adjoint.h) and the
 // reduction of its per-wavefront parameter-gradient partial sums.
+#include <mutex>
+
 #include "mp_adjoint.h"
 
 namespace shield {
+
+// Device scratch of shield_mp_adjoint_noalloc: the tapes and parameter
+// partial sums of as many wavefronts as fit (SHIELD_MP_ADJOINT_SCRATCH_BYTES,
+// each GPU holds its own copy for the life of the process).  Host code never
+// uses the array's name as an address: the per-device address comes from
+// hipGetSymbolAddress (scratch_address).
+static __device__ __align__(256) double scratch[SHIELD_MP_ADJOINT_SCRATCH_BYTES / sizeof(double)];
 
 struct AdjLayout {
   int S, P;  // slots, columns per wavefront
@@ -65,6 +74,30 @@ hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, 
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(adj_param_reduce, dim3(1), dim3(64), 0, stream, part, nw, a.params);
   return hipGetLastError();
+}
+
+// The address of this device's copy of `scratch`, looked up once per device
+// with that device current; nullptr on failure.
+static void* scratch_address() {
+  constexpr int kMaxDevices = 64;
+  static void* addr[kMaxDevices] = {};
+  static std::mutex mu;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
+
+hipError_t launch_adjoint_noalloc(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
+                                  const shield_mp_adjoint_fields& a, hipStream_t stream) {
+  void* ws = scratch_address();
+  if (!ws) return hipErrorInvalidSymbol;
+  return launch_adjoint(sh, p, f, a, ws, SHIELD_MP_ADJOINT_SCRATCH_BYTES, stream);
 }
 
 }  // namespace shield

@@ -14,6 +14,8 @@ struct LayoutChoice {
 LayoutChoice choose_layout(int64_t nz, int mode, int ntimes, bool fused);
 int64_t adjoint_tape_doubles(int64_t nk, int ntimes);
 int adjoint_cols_per_wave(int64_t nk);
+hipError_t launch_adjoint_noalloc(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
+                                  const shield_mp_adjoint_fields& a, hipStream_t stream);
 hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
                           const shield_mp_adjoint_fields& a, void* ws, size_t ws_bytes, hipStream_t stream);
 }  // namespace shield
@@ -260,17 +262,19 @@ size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shi
   return (size_t)waves * (size_t)(adjoint_tape_doubles(shape->nk, p->ntimes) + SHIELD_MP_NPARAMS) * sizeof(double);
 }
 
-int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
-                      const shield_mp_adjoint_fields* a, void* ws, size_t ws_bytes, void* stream) {
-  if (!shape || !p || !f || !a || !ws) return SHIELD_ERR_NULL;
-  int empty = 0;
-  const int rc = check_inputs(shape, p, f, &empty);
+// Checks of shield_mp_adjoint and shield_mp_adjoint_noalloc (ws may be NULL
+// for the latter); *empty = 1 when there is nothing to launch.
+static int check_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                         const shield_mp_adjoint_fields* a, const void* ws, int* empty) {
+  *empty = 0;
+  if (!shape || !p || !f || !a) return SHIELD_ERR_NULL;
+  const int rc = check_inputs(shape, p, f, empty);
   if (rc != SHIELD_OK) return rc;
   if (!f->qaz && !p->nml.reset_qaz) return SHIELD_ERR_NULL;
   if (shape->nk > 128) return SHIELD_ERR_SHAPE;
   if (p->do_sedi_w || p->nml.z_slope_ice || p->nml.fix_negative || (p->nml.de_ice && p->accumulate))
     return SHIELD_ERR_ADJ_SWITCH;
-  if (empty) return SHIELD_OK;
+  if (*empty) return SHIELD_OK;
   const void* need[] = {a->pt_dt, a->qv_dt, a->ql_dt, a->qr_dt, a->qi_dt, a->qs_dt, a->qg_dt, a->qa_dt, a->udt,
                         a->vdt,   a->pt,    a->qv,    a->ql,    a->qr,    a->qi,    a->qs,    a->qg,    a->qa,
                         a->ua,    a->va,    a->delp,  a->delz,  a->params};
@@ -294,8 +298,27 @@ int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, c
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SHIELD_ERR_NOGPU;
+  return SHIELD_OK;
+}
+
+int shield_mp_adjoint(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                      const shield_mp_adjoint_fields* a, void* ws, size_t ws_bytes, void* stream) {
+  if (!ws) return SHIELD_ERR_NULL;
+  int empty = 0;
+  const int rc = check_adjoint(shape, p, f, a, ws, &empty);
+  if (rc != SHIELD_OK || empty) return rc;
   if (ws_bytes < shield_mp_adjoint_workspace_bytes(shape, p, 1)) return SHIELD_ERR_WORKSPACE;
   const hipError_t e = launch_adjoint(*shape, *p, *f, *a, ws, ws_bytes, (hipStream_t)stream);
+  return e == hipSuccess ? SHIELD_OK : SHIELD_ERR_LAUNCH;
+}
+
+int shield_mp_adjoint_noalloc(const shield_mp_shape* shape, const shield_mp_params* p, const shield_mp_fields* f,
+                              const shield_mp_adjoint_fields* a, void* stream) {
+  int empty = 0;
+  const int rc = check_adjoint(shape, p, f, a, nullptr, &empty);
+  if (rc != SHIELD_OK || empty) return rc;
+  if (shield_mp_adjoint_workspace_bytes(shape, p, 1) > SHIELD_MP_ADJOINT_SCRATCH_BYTES) return SHIELD_ERR_WORKSPACE;
+  const hipError_t e = launch_adjoint_noalloc(*shape, *p, *f, *a, (hipStream_t)stream);
   return e == hipSuccess ? SHIELD_OK : SHIELD_ERR_LAUNCH;
 }
 

@@ -175,6 +175,20 @@ def check_adjoint_rejects_before_launch():
         assert call(a=a2) == -5, written
     # every check passes: without a GPU the call reports it
     assert call() == -8
+    # the variant without a workspace argument makes the same checks
+    noalloc = L.shield_mp_adjoint_noalloc
+    assert noalloc(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), None, None) == -1
+    aliased = _fake_adjoint()
+    aliased.g_qv = f.qv
+    assert noalloc(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(aliased), None) == -5
+    a3 = _fake_adjoint()
+    a3.d_qv_dt = None
+    assert noalloc(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(a3), None) == -1
+    assert noalloc(ctypes.byref(_lib.ShieldMpShape(2, 2, 200, 400, 200, 2, 1)), ctypes.byref(p), ctypes.byref(f),
+                   ctypes.byref(a), None) == -2
+    assert noalloc(ctypes.byref(_lib.ShieldMpShape(0, 4, 79, 0, 79, 0, 1)), ctypes.byref(p), ctypes.byref(f),
+                   ctypes.byref(a), None) == 0
+    assert noalloc(ctypes.byref(shape), ctypes.byref(p), ctypes.byref(f), ctypes.byref(a), None) == -8
     # workspace: one tape of ntimes x 77 recorded fields x 192 cells (2
     # columns of 79 levels in 3 slots) per wavefront, plus six partial sums;
     # bounded by the number of column pairs
