@@ -68,7 +68,7 @@ DERIVED_NAMES = (
        "fac_i2s", "fac_g2v", "fac_v2g", "fac_imlt", "fac_l2v", "ntimes"]
 )
 
-_lib = None
+_libs = {}
 
 
 def build(force=False):
@@ -80,12 +80,21 @@ def build(force=False):
     return LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def build_native(outdir):
+    """Compile the C restatement for the CPU baseline: -O3 -march=native on
+    the host that will run it (make -C oracle native).  Returns its path."""
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={outdir}"], check=True)
+    return os.path.join(outdir, "libmp_oracle_native.so")
+
+
+def lib(path=None):
+    """The ctypes library of the checker build, or of another build of the
+    same source (``path``, e.g. build_native's)."""
+    path = path or LIB_PATH
+    if path not in _libs:
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         L.oracle_mp_run.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(OracleNml),
                                     ctypes.c_double, ctypes.POINTER(OracleFields), ctypes.c_int]
         L.oracle_mp_run.restype = ctypes.c_int
@@ -99,8 +108,8 @@ def lib():
             getattr(L, fn).restype = ctypes.c_double
         L.oracle_fall_column.argtypes = [ctypes.c_int, ctypes.c_double, _DP, _DP, _DP, _DP, _DP]
         L.oracle_fall_column.restype = ctypes.c_double
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def make_nml(nml: dict) -> OracleNml:
@@ -154,13 +163,14 @@ TEND_NAMES = ("pt_dt", "qv_dt", "ql_dt", "qr_dt", "qi_dt", "qs_dt", "qg_dt", "qa
 PRECIP_NAMES = ("rain", "snow", "ice", "graupel")
 
 
-def run(nml: dict, timestep: float, state: dict, *, qaz=None, tend=None, nthreads=None):
+def run(nml: dict, timestep: float, state: dict, *, qaz=None, tend=None, nthreads=None, library=None):
     """One Microphysics call on the CPU.
 
     ``state`` holds float64 arrays [..., nz] for pt qv ql qr qi qs qg qa ua va
     delp delz (w optional) and [...] for land, area.  Returns a dict with the
     ten tendencies, ``qaz``, ``w`` (when given), the updated ``qi``/``qs`` (which
     the reference rewrites when de_ice) and the 2D precipitation in mm/day.
+    ``library`` selects another build of the restatement (the CPU baseline's).
     """
     nz = state["pt"].shape[-1]
     ncol = state["pt"].size // nz
@@ -200,7 +210,7 @@ def run(nml: dict, timestep: float, state: dict, *, qaz=None, tend=None, nthread
         a = np.zeros(ncol)
         out[n] = a
         setattr(f, n, a.ctypes.data_as(_DP))
-    rc = lib().oracle_mp_run(ncol, nz, ctypes.byref(make_nml(nml)), float(timestep),
+    rc = lib(library).oracle_mp_run(ncol, nz, ctypes.byref(make_nml(nml)), float(timestep),
                              ctypes.byref(f), int(nthreads))
     if rc != 0:
         raise RuntimeError(f"oracle_mp_run failed with code {rc}")
