@@ -246,6 +246,7 @@ struct Coef {
   // heat-capacity coefficients (latent heats and cvm are FP64 in both modes)
   double t_sub_d, t_wfr_d, qc_crt_d;
   double c_air_d, c_vap_d, d0_vap_d, lv00_d;
+  double cgfr1_d;  // the Bigg coefficient A' of the rain freezing, FP64
 };
 
 // The mixed-precision kernel keeps its copy of Coef::tab in LDS (filled at
@@ -293,6 +294,7 @@ inline Coef<R> make_coef(const shield_mp_params& p) {
   c.use_ccn = p.use_ccn; c.pad_ = 0;
   c.t_sub_d = n.t_sub; c.t_wfr_d = p.t_wfr; c.qc_crt_d = n.qc_crt;
   c.c_air_d = p.c_air; c.c_vap_d = p.c_vap; c.d0_vap_d = p.d0_vap; c.lv00_d = p.lv00;
+  c.cgfr1_d = p.cgfr[1];
   return c;
 }
 

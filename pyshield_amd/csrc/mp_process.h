@@ -58,6 +58,38 @@ SH_FN wide_t<R> cap(wide_t<R> x, wide_t<R> q) {
   return x;
 }
 
+// The part f / (1 + f) q of q that a backward-Euler collection step takes
+// (f the step's rate factor).  FP64 keeps the reference's expression.  In
+// mixed precision the residue q / (1 + f) is formed in FP64 from the FP32 f
+// and the transfer is q minus it: the residue then carries the relative
+// error of 1 + f, at most that of f, where f / (1 + f) q rounded in FP32
+// would leave it that error times (1 + f) when f is large.
+template <class R>
+SH_FN wide_t<R> implicit_take(R f, wide_t<R> q) {
+  if constexpr (is_f32<R>::value)
+    return q - fdiv<R>(q, 1.0 + double(f));
+  else
+    return f / (R(1.0) + f) * q;
+}
+
+// exp(a t) for an FP64 temperature difference t in a rate law of type R (a_d
+// the FP64 value of the coefficient a).  FP64 and the dual numbers keep
+// exp(a * R(t)).  FP32 rate laws reduce the
+// argument exactly in FP64 (x = k ln2 + r, |r| <= ln2 / 2) and take the FP32
+// exponential of r only: rounding x to FP32 first would cost |x| 2^-24
+// relative, 1.7e-6 (14 ulp) for the Bigg freezing's exp(0.66 tc) at tc = 43 K.
+template <class R>
+SH_FN R exp_lin(R a, double a_d, wide_t<R> t) {
+  if constexpr (is_f32<R>::value) {
+    const double x = a_d * t;
+    const double k = __builtin_rint(x * 1.4426950408889634);
+    const float r = float(fma(-k, 0.6931471805599453, x));
+    return ldexpf(mexp(r), int(k));
+  } else {
+    return mexp(a * R(t));
+  }
+}
+
 // moist heat capacity (FP64 in both modes)
 #define SH_CVM(c, s, q_liq, q_sol) ((c).c_air_d + (s).qv * (c).c_vap_d + (q_liq) * C_LIQ + (q_sol) * C_ICE)
 
@@ -221,7 +253,7 @@ SH_FN void revap_racc(const Coef<R>& c, double t_wfr, R dt, R h_var, CellR<R>& s
     }
     if ((s.qr > QRMIN) && (s.ql > 1.0e-6) && (qsat < q_minus)) {
       R sink = dt * denfac * c.cracw * pw_el<R>(R(s.qr) * den, 0.95);
-      W sk = cap<R>(W(sink / (R(1.0) + sink) * R(s.ql)), s.ql);
+      W sk = implicit_take<R>(sink, s.ql);
       s.ql = s.ql - sk;
       s.qr = s.qr + sk;
     }
@@ -326,7 +358,8 @@ SH_FN void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var,
   W lcpk = fdiv<R>(lhl, cvm);
   W icpk = fdiv<R>(lhi, cvm);
   W tcpk = lcpk + icpk;
-  W tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
+  // the share of the heat of fusion in the adjustment, FP64 in both modes
+  W tcp3 = lcpk + icpk * mmin(W(1.0), fdiv<R>(dim(tice_d, s.tz), tice_d - c.t_wfr_d));
   if (!(p1 >= P_MIN)) return;
   if (s.tz < T_MIN) {
     W sink = dim(1.0e-7, s.qv);
@@ -343,7 +376,7 @@ SH_FN void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var,
   lcpk = fdiv<R>(lhl, cvm);
   icpk = fdiv<R>(lhi, cvm);
   tcpk = lcpk + icpk;
-  tcp3 = lcpk + icpk * mmin(R(1.0), R(dim(tice_d, s.tz)) / (tice - t_wfr));
+  tcp3 = lcpk + icpk * mmin(W(1.0), fdiv<R>(dim(tice_d, s.tz), tice_d - c.t_wfr_d));
 
   // instant evaporation / sublimation of all clouds when rh < rh_adj
   const W qpz = s.qv + s.ql + s.qi;
@@ -370,8 +403,10 @@ SH_FN void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var,
   const W dq0 = qsw - s.qv;
   W evap;
   if (dq0 > 0.0) {
-    R factor = mmin(R(1.0), c.fac_l2v * R(fdiv<R>(10.0 * dq0, qsw)));
-    evap = mmin(s.ql, fdiv<R>(W(factor) * dq0, 1.0 + tcp3 * dwsdt));
+    // FP64: when this drains nearly all cloud water the residue inherits the
+    // relative error of the factor times (cloud water / residue)
+    const W factor = mmin(W(1.0), W(c.fac_l2v) * fdiv<R>(10.0 * dq0, qsw));
+    evap = mmin(s.ql, fdiv<R>(factor * dq0, 1.0 + tcp3 * dwsdt));
   } else {
     evap = fdiv<R>(dq0, 1.0 + tcp3 * dwsdt);
   }
@@ -406,7 +441,7 @@ SH_FN void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var,
     const W tcd = tice_d - s.tz;
     const R tc = R(tcd);
     if ((s.ql > QRMIN) && (tcd > 0.0)) {
-      R sink = R(3.3333e-10) * dts * (mexp(R(0.66) * tc) - R(1.0)) * den * R(s.ql) * R(s.ql);
+      R sink = R(3.3333e-10) * dts * (exp_lin<R>(R(0.66), 0.66, tcd) - R(1.0)) * den * R(s.ql) * R(s.ql);
       W sk = mmin(s.ql, mmin(fdiv<R>(tcd, icpk), W(sink)));
       s.ql = s.ql - sk;
       s.qi = s.qi + sk;
@@ -699,7 +734,7 @@ SH_FN void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>
       }
     } else {
       if (s.qi > 3.0e-7) {
-        R psaci;
+        W psaci;
         if (s.qs > 1.0e-7) {
           R e;
           if constexpr (is_f32<R>::value)
@@ -707,9 +742,9 @@ SH_FN void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>
           else
             e = mexp(R(0.05) * tc + R(0.8125) * mlog(R(s.qs) * den));
           R factor = dts * denfac * c.csaci * e;
-          psaci = factor / (R(1.0) + factor) * R(s.qi);
+          psaci = implicit_take<R>(factor, s.qi);
         } else {
-          psaci = R(0.0);
+          psaci = W(0.0);
         }
         R qim = c.qi0_crt / den;
         R tmp = c.const_vi ? c.fac_i2s : c.fac_i2s * mexp(R(0.025) * tc);
@@ -727,12 +762,12 @@ SH_FN void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>
         } else {
           psaut = R(0.0);
         }
-        W sink = mmin(0.75 * s.qi, W(psaci + psaut));
+        W sink = mmin(0.75 * s.qi, psaci + W(psaut));
         s.qi = s.qi - sink;
         s.qs = s.qs + sink;
         if (s.qg > 1.0e-6) {
           R factor = dts * c.cgaci * msqrt(den) * R(s.qg);
-          W pgaci = cap<R>(W(factor / (R(1.0) + factor) * R(s.qi)), s.qi);
+          W pgaci = implicit_take<R>(factor, s.qi);
           s.qi = s.qi - pgaci;
           s.qg = s.qg + pgaci;
         }
@@ -744,7 +779,8 @@ SH_FN void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>
           psacr = dts * acr3d(vtsz, vtrz, R(s.qr), R(s.qs), c.csacr, ck(c, T_ACCO + 1), ck(c, T_ACCO + 5), ck(c, T_ACCO + 9), den);
         else
           psacr = R(0.0);
-        R pgfr = dts * ck(c, T_CGFR + 0) / den * (mexp(-ck(c, T_CGFR + 1) * tc) - R(1.0)) * pw_el<R>(R(s.qr) * den, 1.75);
+        R pgfr = dts * ck(c, T_CGFR + 0) / den * (exp_lin<R>(-ck(c, T_CGFR + 1), -c.cgfr1_d, s.tz - tice_d) - R(1.0)) *
+                 pw_el<R>(R(s.qr) * den, 1.75);
         // the limiter that can drain all rain is applied in FP64, so a fully
         // frozen layer keeps no FP32 residue
         const W sink = W(psacr) + W(pgfr);
@@ -777,7 +813,7 @@ SH_FN void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>
           sink = R(0.0);
         R qsm = c.qs0_crt / den;
         if (s.qs > W(qsm)) {
-          R factor = dts * R(1.0e-3) * mexp(R(0.09) * R(s.tz - tice_d));
+          R factor = dts * R(1.0e-3) * exp_lin<R>(R(0.09), 0.09, s.tz - tice_d);
           sink = sink + factor / (R(1.0) + factor) * R(s.qs - W(qsm));
         }
         W sk = mmin(s.qs, W(sink));
@@ -785,21 +821,25 @@ SH_FN void icloud_main(const Coef<R>& c, double tice_d, double tice0_d, CellR<R>
         s.qg = s.qg + sk;
       }
       if ((s.qg > 1.0e-7) && (s.tz < tice0_d)) {
+        // both collections are bounded by the FP64 species they take from,
+        // so a layer whose rain the graupel collects entirely keeps none
+        W gw, gr;
         if (s.ql > 1.0e-6) {
           R qden = R(s.qg) * den;
           R factor = dts * c.cgacw * qden / msqrt(den * msqrt(msqrt(qden)));
-          pgacw = factor / (R(1.0) + factor) * R(s.ql);
+          gw = implicit_take<R>(factor, s.ql);
         } else {
-          pgacw = R(0.0);
+          gw = W(0.0);
         }
         if (s.qr > 1.0e-6)
-          pgacr = mmin(dts * acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, ck(c, T_ACCO + 2), ck(c, T_ACCO + 6), ck(c, T_ACCO + 10), den),
-                       R(s.qr));
+          gr = mmin(W(dts * acr3d(vtgz, vtrz, R(s.qr), R(s.qg), c.cgacr, ck(c, T_ACCO + 2), ck(c, T_ACCO + 6),
+                                  ck(c, T_ACCO + 10), den)),
+                    s.qr);
         else
-          pgacr = R(0.0);
-        const W sink = W(pgacr) + W(pgacw);
+          gr = W(0.0);
+        const W sink = gr + gw;
         const W factor = fdiv<R>(mmin(sink, fdiv<R>(dim(tice_d, s.tz), icpk)), mmax(sink, QRMIN));
-        const W pr = cap<R>(factor * W(pgacr), s.qr), pl = cap<R>(factor * W(pgacw), s.ql);
+        const W pr = cap<R>(factor * gr, s.qr), pl = cap<R>(factor * gw, s.ql);
         W sk = pr + pl;
         s.qg = s.qg + sk;
         s.qr = s.qr - pr;

@@ -4,10 +4,11 @@ two sub-steps) and the deep tile (768 x 768 columns x 127 levels, dt 90 s).
 
 Columns are independent, so the whole-domain GPU run is checked against the
 oracle on a random sample of its columns (FP64 to round-off, mixed within the
-field-wise 1e-5), and through size-independent properties over every column:
-total-water conservation in both precisions and agreement of the mixed run
-with the FP64 run (1e-3 K at every cell, the vapour change within 1e-5 of its
-scale)."""
+field-wise 1e-5, cell by cell within 1e-5 or the oracle's FP32 envelope), and
+through size-independent properties over every column: total-water
+conservation in both precisions and agreement of the mixed run with the FP64
+run (1e-3 K at every cell and the vapour change within 1e-5 of its scale,
+except cells the oracle's own FP32 envelope shows to sit on a threshold)."""
 
 import numpy as np
 import pytest
@@ -15,6 +16,7 @@ import pytest
 from pyshield_amd._config import PhysicsConfig
 from pyshield_amd.synthetic import WORKLOAD_IDS, WORKLOADS, make_state
 
+import fp32_envelope as FE
 from helpers import PRECIP, TEND, WATER, norm_err, run_oracle
 
 pytestmark = pytest.mark.gpu
@@ -23,9 +25,9 @@ NSAMPLE = 384
 MIX_T_TOL = 1e-3        # K, every cell
 
 
-def _load(name):
-    """The workload's state on cuda:0, tile by tile, and the host copy of the
-    sampled columns (flat column indices `cols`)."""
+def _load(name, nsample=NSAMPLE, seed=7):
+    """The workload's state on cuda:0, tile by tile, and the host copy of
+    `nsample` sampled columns (flat column indices `cols`)."""
     import torch
     w = WORKLOADS[name]
     dev = torch.device("cuda:0")
@@ -33,9 +35,9 @@ def _load(name):
     names = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz", "w", "land", "area")
     f = {n: torch.empty((ni, nj, nz) if n not in ("land", "area") else (ni, nj), dtype=torch.float64,
                         device=dev) for n in names}
-    rng = np.random.default_rng(7)
-    cols = np.sort(rng.choice(ni * nj, NSAMPLE, replace=False))
-    sample = {n: np.empty((NSAMPLE, 1, nz) if n not in ("land", "area") else (NSAMPLE, 1)) for n in names}
+    rng = np.random.default_rng(seed)
+    cols = np.sort(rng.choice(ni * nj, nsample, replace=False))
+    sample = {n: np.empty((nsample, 1, nz) if n not in ("land", "area") else (nsample, 1)) for n in names}
     for t in range(w.ntiles):
         off = (-10.0 + 4.0 * t) if w.tile_offsets else 0.0
         st = make_state(w.ni, w.nj, nz, 1000 * WORKLOAD_IDS[name] + t, ptop=w.ptop, t_offset=off,
@@ -52,6 +54,29 @@ def _load(name):
     for n in PRECIP:
         f[n] = torch.zeros((ni, nj), dtype=torch.float64, device=dev)
     return w, f, cols, sample
+
+
+def explain_cells(f, qaz, beyond, tend_mixed, tend_fp64, dt, nml):
+    """The cells of a whole-domain mixed call beyond the bound (`beyond`, a
+    boolean [ni, nj, nz] device mask) against the oracle's FP32 envelope of
+    their columns: every such cell must differ from the FP64 call by at most
+    FP32_ENV x the envelope.  Returns (columns checked, unexplained cells)."""
+    import torch
+    nz = f["pt"].shape[-1]
+    cols = beyond.reshape(-1, nz).any(dim=1).nonzero().flatten()
+    if cols.numel() == 0:
+        return 0, []
+    names = ("pt", "qv", "ql", "qr", "qi", "qs", "qg", "qa", "ua", "va", "delp", "delz", "w", "land", "area")
+    st = {n: f[n].reshape((-1,) + tuple(f[n].shape[2:]))[cols].cpu().numpy() for n in names}
+    q0 = qaz.reshape(-1, nz)[cols].cpu().numpy()
+    _, env = FE.envelope(nml, dt, st, qaz=q0)
+    bad = []
+    mask = beyond.reshape(-1, nz)[cols].cpu().numpy()
+    for n in ("pt", "qv"):
+        d = ((tend_mixed[n].reshape(-1, nz)[cols] - tend_fp64[n].reshape(-1, nz)[cols]) * dt).abs().cpu().numpy()
+        for c, k in np.argwhere(mask & (d > FE.ENV_FACTOR * env[n])):
+            bad.append((n, int(cols[c]), int(k), float(d[c, k]), float(env[n][c, k])))
+    return int(cols.numel()), bad
 
 
 def _column_water_imbalance(f, dt):
@@ -90,18 +115,28 @@ def test_full_configuration(name):
         outs[mode]["all_qv_dt"] = f["qv_dt"].clone()
     # mixed against FP64 over every cell: the temperature and vapour changes of
     # the call
-    dT = ((outs["mixed"]["all_pt_dt"] - outs["fp64"]["all_pt_dt"]) * w.timestep).abs().flatten()
+    dT3 = ((outs["mixed"]["all_pt_dt"] - outs["fp64"]["all_pt_dt"]) * w.timestep).abs()
+    dT = dT3.flatten()
     n = dT.numel()
     big = int((dT > MIX_T_TOL).sum().item())
     q = torch.quantile(dT[torch.randperm(n, device=dT.device)[:2 ** 24]], torch.tensor(
         [0.5, 0.999, 0.99999], dtype=dT.dtype, device=dT.device)).tolist()
     print(name, f"mixed vs fp64 |dT| over {n} cells: max {dT.max().item():.3e} K, median {q[0]:.1e}, "
           f"99.9% {q[1]:.1e}, 99.999% {q[2]:.1e}; {big} cells above {MIX_T_TOL} K", flush=True)
-    dq = (outs["mixed"]["all_qv_dt"] - outs["fp64"]["all_qv_dt"]).abs().max() * w.timestep
+    dq3 = (outs["mixed"]["all_qv_dt"] - outs["fp64"]["all_qv_dt"]).abs() * w.timestep
     qscale = outs["fp64"]["all_qv_dt"].abs().max() * w.timestep
-    print(name, f"qv change: max difference {(dq / qscale).item():.1e} of the field scale", flush=True)
-    assert big == 0
-    assert dq <= 1e-5 * qscale
+    print(name, f"qv change: max difference {(dq3.max() / qscale).item():.1e} of the field scale", flush=True)
+    # cells beyond 1e-3 K or 1e-5 of the vapour scale must be threshold or
+    # drained cells, where the oracle's FP32 envelope is as large
+    beyond = (dT3 > MIX_T_TOL) | (dq3 > 1e-5 * qscale)
+    ncols, bad = explain_cells(f, torch.zeros_like(f["pt"]), beyond,
+                               {"pt": outs["mixed"]["all_pt_dt"], "qv": outs["mixed"]["all_qv_dt"]},
+                               {"pt": outs["fp64"]["all_pt_dt"], "qv": outs["fp64"]["all_qv_dt"]},
+                               w.timestep, cfg.mp_namelist())
+    print(name, f"{int(beyond.sum().item())} cells beyond the bound in {ncols} columns; "
+          f"unexplained by the FP32 envelope: {len(bad)}", flush=True)
+    assert not bad, bad[:10]
+    assert ncols <= max(1, 1e-5 * n / w.nz)
     ora = run_oracle(sample, cfg, w.timestep)
     errs = {n: norm_err(outs["fp64"][n], ora[n], floor=1e-30) for n in TEND}
     print(name, "fp64 vs oracle on", NSAMPLE, "columns", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
@@ -111,3 +146,8 @@ def test_full_configuration(name):
     errs = {n: norm_err(outs["mixed"][n], ora[n], floor=1e-30) for n in TEND if n != "qa_dt"}
     print(name, "mixed vs oracle", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
     assert all(v <= 1e-5 for v in errs.values()), errs
+    # cell by cell on the sampled columns: within the bound or the envelope
+    rep = FE.check_cells(cfg.mp_namelist(), w.timestep, sample, {n: outs["mixed"][n] for n in TEND})
+    print(name, "mixed cell by cell on", NSAMPLE, "columns:", {k: v["beyond_bound"] for k, v in rep["fields"].items()},
+          "cells beyond the bound,", rep["columns_checked_by_envelope"], "columns checked by the envelope", flush=True)
+    assert not rep["unexplained"], rep["unexplained"][:10]
