@@ -350,6 +350,17 @@ def adjoint(fields: dict, cotangents: dict, grads: dict, params, workspace=None,
     return workspace
 
 
+def _strided_like(ref):
+    """An uninitialised float64 tensor with ref's shape and strides (halo'd
+    Quantity views keep their layout, which forward() and adjoint() require
+    of every field)."""
+    return torch.empty_strided(tuple(ref.shape), tuple(ref.stride()), dtype=torch.float64, device=ref.device)
+
+
+# the adjoint's workspace, kept between backward calls (per device)
+_AD_WORKSPACE = {}
+
+
 class MicrophysicsFunction(torch.autograd.Function):
     """One microphysics call as a differentiable function of the twelve state
     inputs (AD_STATE_NAMES), the incoming qaz and the six process parameters
@@ -358,20 +369,20 @@ class MicrophysicsFunction(torch.autograd.Function):
     HIP forward in ``mode``; the backward runs the HIP adjoint, which
     recomputes the FP64 forward of every column and follows the branch it
     takes at each point.  Tendencies are those of this call (overwritten,
-    not accumulated)."""
+    not accumulated).  Halo'd views are accepted: outputs and gradients take
+    the state's strides.  The adjoint's workspace is kept for the next
+    backward on the same device."""
 
     @staticmethod
     def forward(ctx, params, mode, adp, land, area, qaz, *state):
         p = with_ad_params(params, adp.detach().cpu().tolist())
         p.accumulate = 0
         ref = state[0]
-        ni, nj, nk = ref.shape
         f = {n: t.detach() for n, t in zip(AD_STATE_NAMES, state)}
         f.update(land=land.detach(), area=area.detach())
-        f["qaz"] = qaz.detach().clone()
-        f["w"] = torch.zeros_like(ref)
-        out = {n: torch.empty((ni, nj, nk) if n not in FIELDS_2D else (ni, nj), dtype=torch.float64,
-                              device=ref.device) for n in AD_OUT_NAMES}
+        f["qaz"] = _strided_like(ref).copy_(qaz.detach())
+        f["w"] = _strided_like(ref).zero_()
+        out = {n: _strided_like(land if n in FIELDS_2D else ref) for n in AD_OUT_NAMES}
         f.update(out)
         forward(f, p, mode)
         ctx.p = p
@@ -386,11 +397,12 @@ class MicrophysicsFunction(torch.autograd.Function):
         f.update(land=land, area=area, qaz=qaz)
         cot = {}
         for n, g in zip(AD_OUT_NAMES, gout):
-            shp = tuple(land.shape) if n in FIELDS_2D else tuple(ref.shape)
-            cot[n] = (g if g is not None else torch.zeros(shp, dtype=torch.float64, device=ref.device)).contiguous()
-        grads = {n: torch.empty_like(ref) for n in AD_STATE_NAMES + ("qaz",)}
+            like = land if n in FIELDS_2D else ref
+            cot[n] = _strided_like(like).zero_() if g is None else _strided_like(like).copy_(g)
+        grads = {n: _strided_like(ref) for n in AD_STATE_NAMES + ("qaz",)}
         grads["params"] = torch.zeros(6, dtype=torch.float64, device=ref.device)
-        adjoint(f, cot, grads, ctx.p)
+        key = ref.device
+        _AD_WORKSPACE[key] = adjoint(f, cot, grads, ctx.p, workspace=_AD_WORKSPACE.get(key))
         gadp = grads["params"].cpu()
         return (None, None, gadp, None, None, grads["qaz"]) + tuple(grads[n] for n in AD_STATE_NAMES)
 

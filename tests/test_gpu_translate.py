@@ -37,7 +37,14 @@ def test_translate_microph_against_oracle_savepoint(dt):
     # standard.yaml, gt:gpu) and udt, vdt at 1e-8 (baroclinic.yaml,
     # dace:gpu).  pt_dt gets one by the same count, 1e-7 K/s (1.5e-5 K per
     # call).
-    near_zero = dict(T.MICROPH_NEAR_ZERO, mph_udt=1e-8, mph_vdt=1e-8, mph_pt_dt=1e-7)
-    rep = T.compare(out, expected, near_zero=near_zero)
-    print({k: f"{v[1]:.1e}" for k, v in rep.items()}, flush=True)
+    # The cited gt:gpu set (max_error 2.2e-8; ql, qr, qg near zero) holds
+    # for every variable but these three, which get the allowances above:
+    # udt, vdt at 1e-8 (the reference's own baroclinic.yaml dace:gpu value)
+    # and pt_dt at 1e-7 K/s (the same ulp count for a temperature).
+    extra = {"mph_udt": 1e-8, "mph_vdt": 1e-8, "mph_pt_dt": 1e-7}
+    strict = T.compare(out, expected)
+    print("gt:gpu set:", {k: f"{v[1]:.1e} ({v[2]} cells)" for k, v in strict.items()}, flush=True)
+    assert all(v[0] for k, v in strict.items() if k not in extra), strict
+    rep = T.compare(out, expected, near_zero=dict(T.MICROPH_NEAR_ZERO, **extra))
+    print("with the three allowances:", {k: f"{v[1]:.1e}" for k, v in rep.items()}, flush=True)
     assert all(v[0] for v in rep.values()), rep
