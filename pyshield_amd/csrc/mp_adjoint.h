@@ -999,6 +999,8 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
   double gp_acc[AP_N];
 #pragma unroll
   for (int p = 0; p < AP_N; ++p) gp_acc[p] = 0.0;
+  fill_coef_tab(c, lane);
+  __syncthreads();
 
   const int64_t stride = (int64_t)gridDim.x * cols_per_wave;
   for (int64_t c0 = (int64_t)blockIdx.x * cols_per_wave; c0 < ncol; c0 += stride) {

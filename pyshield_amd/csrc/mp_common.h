@@ -249,20 +249,40 @@ struct Coef {
   double cgfr1_d;  // the Bigg coefficient A' of the rain freezing, FP64
 };
 
-// The mixed-precision kernel keeps its copy of Coef::tab in LDS (filled at
-// its start): with the whole block in scalar registers the compiler spills
-// them to VGPR lanes and reloads them with v_readlane inside the process
-// code.  Each translation unit gets its own (per-kernel) copy.
+// The kernels keep their copy of Coef::tab in LDS (filled at their start by
+// fill_coef_tab): with the whole block in scalar registers the compiler
+// spills them to VGPR lanes (or, for the FP64 block, to scratch memory) and
+// reloads them inside the process code.  Each translation unit gets its own
+// (per-kernel) copy; a kernel allocates only the one its precision reads.
 static __shared__ float coef_tab_lds[T_N];
+static __shared__ double coef_tab_lds64[T_N];
 
-// Coefficient-array entry i (a constant after inlining).  In mixed precision
-// it is read from the LDS copy, in FP64 from the kernel argument.
+// Coefficient-array entry i (a constant after inlining): in device code from
+// the LDS copy, in host builds of the process code (the point-wise test
+// harness) from the block itself.
 template <class R>
 SH_FN R ck(const Coef<R>& c, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (sizeof(R) == 4)
     return coef_tab_lds[i];
+  else if constexpr (sizeof(R) == 8)
+    return coef_tab_lds64[i];
   else
     return c.tab[i];
+#else
+  return c.tab[i];
+#endif
+}
+// Copies Coef::tab into the LDS table of R; the caller synchronises the
+// workgroup before the first ck().
+template <class R>
+__device__ __forceinline__ void fill_coef_tab(const Coef<R>& c, int lane) {
+  if (lane < T_N) {
+    if constexpr (sizeof(R) == 4)
+      coef_tab_lds[lane] = c.tab[lane];
+    else if constexpr (sizeof(R) == 8)
+      coef_tab_lds64[lane] = c.tab[lane];
+  }
 }
 
 template <class R>

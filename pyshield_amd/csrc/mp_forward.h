@@ -51,6 +51,48 @@ __device__ __forceinline__ double launder(double v) {
   return v;
 }
 
+// ------------------------------------------------------ kernel arguments
+// The forward kernel takes one argument block.  Its field pointers are read
+// where they are used, each with a scalar load from the kernarg segment.
+// Argument values are otherwise all loaded at kernel entry and kept for the
+// whole kernel; the ~30 pointers and the coefficients outgrow the scalar
+// register file, and the compiler spills the excess to VGPR lanes and reloads
+// each use with v_readlane, a VALU instruction.  The empty asm hides the
+// offset's value, so no two reads are merged and none is hoisted.
+template <class R>
+struct FwdArgs {
+  shield_mp_shape sh3;
+  shield_mp_params P;
+  Coef<R> c;
+  shield_mp_fields f;
+  shield_update_fields u;
+  int cols_per_wave;
+};
+
+using gdouble = __attribute__((address_space(1))) double;
+template <uint32_t OFF>
+struct KPtr {
+  __device__ __forceinline__ gdouble* get() const {
+    uint32_t o = OFF;
+    asm volatile("" : "+s"(o));
+    const __attribute__((address_space(4))) char* base =
+        (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+    return *(gdouble* const __attribute__((address_space(4)))*)(base + o);
+  }
+  __device__ __forceinline__ gdouble& operator[](int64_t i) const { return get()[i]; }
+  __device__ __forceinline__ explicit operator bool() const { return get() != nullptr; }
+};
+// shield_mp_fields at kernarg offset B, member by member
+template <uint32_t B>
+struct KFields {
+#define SH_KF(n) KPtr<B + (uint32_t)offsetof(shield_mp_fields, n)> n;
+  SH_KF(pt) SH_KF(qv) SH_KF(ql) SH_KF(qr) SH_KF(qi) SH_KF(qs) SH_KF(qg) SH_KF(qa) SH_KF(ua) SH_KF(va)
+  SH_KF(delp) SH_KF(delz) SH_KF(w) SH_KF(land) SH_KF(area) SH_KF(pt_dt) SH_KF(qv_dt) SH_KF(ql_dt)
+  SH_KF(qr_dt) SH_KF(qi_dt) SH_KF(qs_dt) SH_KF(qg_dt) SH_KF(qa_dt) SH_KF(udt) SH_KF(vdt) SH_KF(qaz)
+  SH_KF(rain) SH_KF(snow) SH_KF(ice) SH_KF(graupel)
+#undef SH_KF
+};
+
 // The point-wise stages are unrolled over the S slots with static indices, so
 // every slot's state stays in registers.
 template <int S, class R>
@@ -99,9 +141,9 @@ __device__ __forceinline__ void layer_dz(const Lay<S>& L, Shared<S>& sh, double 
 // mass flux through each layer bottom (0 in non-falling columns) and surf
 // the surface flux of the cell's column.  When do_sedi_w the vertical
 // velocity, which stays in HBM, is transported with the falling mass.
-template <int S, class R>
+template <int S, class R, class FS>
 __device__ __forceinline__ void implicit_fall(const Lay<S>& L, const St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
-                                              const shield_mp_fields& f, const double (&dz)[S], double (&q)[S],
+                                              const FS& f, const double (&dz)[S], double (&q)[S],
                                               const R (&vt)[S], double dt, const bool (&fall)[S], bool do_sedi_w,
                                               double (&flux)[S], double (&surf)[S]) {
   double dd[S], qq[S], a[S], b[S], qm[S], ddp[S];
@@ -231,9 +273,9 @@ __device__ __forceinline__ void slope_limiter(const Lay<S>& L, Shared<S>& sh, co
 }
 
 // ------------------------------------------------------ warm_rain (:279-653)
-template <int S, class R>
+template <int S, class R, class FS>
 __device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
-                                          const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c,
+                                          const FS& f, const shield_mp_params& P, const Coef<R>& c,
                                           bool is_first) {
   const int lane = lane_id();
   const double t_wfr = P.t_wfr;
@@ -331,9 +373,9 @@ __device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S
 }
 
 // ---------------------------------------------------- sedimentation (:656-1217)
-template <int S, class R>
+template <int S, class R, class FS>
 __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
-                                              const shield_mp_fields& f, const shield_mp_params& P,
+                                              const FS& f, const shield_mp_params& P,
                                               const Coef<R>& c) {
   const int lane = lane_id();
   const double tice_d = P.nml.tice;
@@ -415,9 +457,9 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
 }
 
 // ------------------------------------------------------------ icloud (:1220-1449)
-template <int S, class R>
+template <int S, class R, class FS>
 __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
-                                       const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c) {
+                                       const FS& f, const shield_mp_params& P, const Coef<R>& c) {
   const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr, ql_mlt = P.nml.ql_mlt;
   const bool slope = P.nml.z_slope_ice != 0;
   const int lane = lane_id();
@@ -442,8 +484,8 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
     }
     slope_limiter(L, sh, X.qi, dl);
   }
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
+  for_slots<S>([&](auto si) __attribute__((always_inline)) {
+    constexpr int s = decltype(si)::value;
     const int cl = L.cl[s];
     const double hv = sh.colv[CV_HVAR][cl];
     const double den = sh.at(CL_DEN, s);
@@ -469,7 +511,7 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
                    R(sh.at(CL_DENFAC, s)), X.vtg[s], X.vtr[s], X.vts[s], p1, R(di), q_liq, q_sol,
                    cvm);
     set_cell(X, s, cs);
-  }
+  });
 }
 
 // The state update that follows the microphysics for this wavefront's
@@ -479,9 +521,9 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
 // (:40-92).  The tendencies were just written by this wavefront and are read
 // back from the cache, so every expression is the one the separate kernels
 // evaluate.
-template <int S>
+template <int S, class FS>
 __device__ __forceinline__ void state_update(const Lay<S>& L, const Glob<S>& G, Shared<S>& sh, const shield_mp_shape& sh3,
-                                             int64_t c0, const shield_mp_fields& f, const shield_update_fields& u) {
+                                             int64_t c0, const FS& f, const shield_update_fields& u) {
   const int lane = lane_id();
   const int nz = L.nz;
   const double* in[10] = {f.qv, f.ql, f.qr, f.qi, f.qs, f.qg, f.qa, f.pt, f.ua, f.va};
@@ -554,9 +596,9 @@ __device__ __forceinline__ void state_update(const Lay<S>& L, const Glob<S>& G, 
 }
 
 // One sub-step (microphysics.py:2045-2220).
-template <int S, class R>
+template <int S, class R, class FS>
 __device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
-                                        const shield_mp_fields& f, const shield_mp_params& P, const Coef<R>& c) {
+                                        const FS& f, const shield_mp_params& P, const Coef<R>& c) {
   warm_rain(L, X, sh, G, f, P, c, true);
   sedimentation(L, X, sh, G, f, P, c);
   warm_rain(L, X, sh, G, f, P, c, false);
@@ -580,9 +622,13 @@ constexpr int min_waves() {
 // FUSED: the state update of shield_mp_forward_update follows (a separate
 // instance, so the plain kernel keeps its registers).
 template <int S, class R, bool MULTI, bool FUSED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S, R, MULTI, FUSED>()))) void mp_forward_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<R> c,
-                                                        shield_mp_fields f, shield_update_fields u,
-                                                        int cols_per_wave) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S, R, MULTI, FUSED>(), 3))) void mp_forward_kernel(const FwdArgs<R> a) {
+  const shield_mp_shape& sh3 = a.sh3;
+  const shield_mp_params& P = a.P;
+  const Coef<R>& c = a.c;
+  const KFields<(uint32_t)offsetof(FwdArgs<R>, f)> f;
+  const shield_update_fields& u = a.u;
+  const int cols_per_wave = a.cols_per_wave;
   __shared__ Shared<S> sh;
   const int lane = lane_id();
   const int nz = (int)sh3.nk;
@@ -631,9 +677,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
     sh.colv[CV_ICE][lane] = 0.0;
     sh.colv[CV_GRAUPEL][lane] = 0.0;
   }
-  if constexpr (sizeof(R) == 4) {
-    if (lane < T_N) coef_tab_lds[lane] = c.tab[lane];
-  }
+  fill_coef_tab(c, lane);
   __syncthreads();
 
   // ---------------------------------------------------- fields_init (:101-276)
@@ -876,7 +920,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
     if (f.ice) f.ice[o2] = sh.colv[CV_ICE][lane] * convt;
     if (f.graupel) f.graupel[o2] = sh.colv[CV_GRAUPEL][lane] * convt;
   }
-  if constexpr (FUSED) state_update(L, G, sh, sh3, c0, f, u);
+  if constexpr (FUSED) state_update(L, G, sh, sh3, c0, a.f, u);
 }
 
 // Launch the kernel for one slot count: a grid of ceil(ncol / P) one-wave
@@ -886,16 +930,16 @@ hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, co
                         const shield_update_fields& u, int P, hipStream_t stream) {
   const int64_t ncol = sh.ni * sh.nj;
   const dim3 grid((unsigned)((ncol + P - 1) / P)), block(64);
-  const Coef<R> c = make_coef<R>(p);
+  const FwdArgs<R> a{sh, p, make_coef<R>(p), f, u, P};
   const bool multi = p.ntimes > 1, fused = u.delp != nullptr;
   if (multi && fused)
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, true>), grid, block, 0, stream, sh, p, c, f, u, P);
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, true>), grid, block, 0, stream, a);
   else if (multi)
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, false>), grid, block, 0, stream, sh, p, c, f, u, P);
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, false>), grid, block, 0, stream, a);
   else if (fused)
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, true>), grid, block, 0, stream, sh, p, c, f, u, P);
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, true>), grid, block, 0, stream, a);
   else
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, false>), grid, block, 0, stream, sh, p, c, f, u, P);
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, false>), grid, block, 0, stream, a);
   return hipGetLastError();
 }
 

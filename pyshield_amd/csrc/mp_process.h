@@ -165,6 +165,14 @@ SH_FN double iqs_fast(double ta, double den, double& dqdt) {
   const double t = ice ? (ta < T_SAT_MIN ? T_SAT_MIN : ta) : (ta > TICE + 102.0 ? TICE + 102.0 : ta);
   return qs_fast(t, ice ? D2ICE : DC_VAP, ice ? LI2 : LV0, den, dqdt);
 }
+// qs_fast(ta, DC_VAP, LV0, ...) where `water`, else iqs_fast(ta, ...), as one
+// evaluation with the constants and the clamp chosen per cell
+SH_FN double qs_pick(double ta, double den, bool water) {
+  const bool ice = !water && ta < TICE;
+  const double t = water ? ta : ice ? (ta < T_SAT_MIN ? T_SAT_MIN : ta) : (ta > TICE + 102.0 ? TICE + 102.0 : ta);
+  double g;
+  return qs_fast(t, ice ? D2ICE : DC_VAP, ice ? LI2 : LV0, den, g);
+}
 // the reference's forms in FP64 mode (and for the adjoint's dual numbers),
 // the folded ones in mixed mode
 template <class R>
@@ -576,7 +584,25 @@ SH_FN void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var,
     const W qpz2 = s.qv + q_cond;
     const W tin2 = s.tz - (lcpk * q_cond + icpk * q_sol2);
     W qstar;
-    if (tin2 <= c.t_wfr_d) {
+    if constexpr (is_f32<R>::value) {
+      // The same three cases with one saturation evaluation for every cell
+      // (over ice below tice, over water above) and a second, over water, for
+      // the cells between t_wfr and tice: the wavefront runs two evaluations
+      // instead of one per case.
+      const bool cold = tin2 <= c.t_wfr_d, warm = !cold && tin2 >= tice_d;
+      const W qa = qs_pick(tin2, den_d, warm);
+      qstar = qa;
+      if (!cold && !warm) {
+        W g;
+        const W qsw3 = qs_fast(tin2, DC_VAP, LV0, den_d, g);
+        W rqi;
+        if (q_cond > 3.0e-6)
+          rqi = fdiv<R>(q_sol2, q_cond);
+        else
+          rqi = fdiv<R>(tice_d - tin2, tice_d - c.t_wfr_d);
+        qstar = rqi * qa + (1.0 - rqi) * qsw3;
+      }
+    } else if (tin2 <= c.t_wfr_d) {
       qstar = qs_i1<R>(tin2, den_d);
     } else if (tin2 >= tice_d) {
       qstar = qs_w1<R>(tin2, den_d);

@@ -16,6 +16,8 @@
 // synchronisation the LDS round trips need.
 #pragma once
 
+#include <utility>
+
 #include "mp_common.h"
 
 namespace shield {
@@ -39,6 +41,19 @@ __device__ __forceinline__ uint64_t range_mask(int lo, int hi) {
   const int n = hi - lo;
   const uint64_t m = (n == 64) ? ~0ull : ((1ull << n) - 1ull);
   return m << lo;
+}
+
+// Calls f(std::integral_constant<int, s>) for s = 0 .. S-1: a loop over the
+// slots that is unrolled by construction, for bodies too large for the
+// unroller's size limit (a rolled slot loop would index the per-slot state
+// arrays dynamically and move them to scratch memory).
+template <class F, int... I>
+__device__ __forceinline__ void for_slots_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int S, class F>
+__device__ __forceinline__ void for_slots(F&& f) {
+  for_slots_impl(f, std::make_integer_sequence<int, S>{});
 }
 
 __device__ __forceinline__ double readlane(double v, int l) {
