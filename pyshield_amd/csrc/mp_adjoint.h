@@ -1096,7 +1096,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
       sed_rec(L, X, sh, Gl, f, P, c, T, it);
       warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B);
       tape_save(T, it, RI, X, sh);
-      icloud(L, X, sh, Gl, f, P, c);
+      icloud<S, double, shield_mp_fields, false>(L, X, sh, Gl, f, P, c);
     }
     __syncthreads();
     // ----------------------------------------------------- reverse walk

@@ -45,9 +45,10 @@ struct Shared {
 
 // launder() hides a value's origin from the optimiser (an empty asm that
 // claims to rewrite it in place), so the value is materialised exactly as
-// rounded and cannot be fused into a neighbouring multiply-add.
+// rounded and cannot be fused into a neighbouring multiply-add.  The asm is
+// not volatile: it orders nothing, so loads around it are scheduled freely.
 __device__ __forceinline__ double launder(double v) {
-  asm volatile("" : "+v"(v));
+  asm("" : "+v"(v));
   return v;
 }
 
@@ -82,15 +83,31 @@ struct KPtr {
   __device__ __forceinline__ gdouble& operator[](int64_t i) const { return get()[i]; }
   __device__ __forceinline__ explicit operator bool() const { return get() != nullptr; }
 };
+#define SH_FIELDS(X)                                                                                 \
+  X(pt) X(qv) X(ql) X(qr) X(qi) X(qs) X(qg) X(qa) X(ua) X(va) X(delp) X(delz) X(w) X(land) X(area) X(pt_dt) \
+  X(qv_dt) X(ql_dt) X(qr_dt) X(qi_dt) X(qs_dt) X(qg_dt) X(qa_dt) X(udt) X(vdt) X(qaz) X(rain) X(snow) X(ice)    \
+  X(graupel)
+// every field pointer, read once: the prologue and fields_update take them
+// all at their start, so the empty asm of each read (which orders memory
+// operations around it) precedes all their loads and stores
+struct GFields {
+#define SH_GF(n) gdouble* n;
+  SH_FIELDS(SH_GF)
+#undef SH_GF
+};
 // shield_mp_fields at kernarg offset B, member by member
 template <uint32_t B>
 struct KFields {
 #define SH_KF(n) KPtr<B + (uint32_t)offsetof(shield_mp_fields, n)> n;
-  SH_KF(pt) SH_KF(qv) SH_KF(ql) SH_KF(qr) SH_KF(qi) SH_KF(qs) SH_KF(qg) SH_KF(qa) SH_KF(ua) SH_KF(va)
-  SH_KF(delp) SH_KF(delz) SH_KF(w) SH_KF(land) SH_KF(area) SH_KF(pt_dt) SH_KF(qv_dt) SH_KF(ql_dt)
-  SH_KF(qr_dt) SH_KF(qi_dt) SH_KF(qs_dt) SH_KF(qg_dt) SH_KF(qa_dt) SH_KF(udt) SH_KF(vdt) SH_KF(qaz)
-  SH_KF(rain) SH_KF(snow) SH_KF(ice) SH_KF(graupel)
+  SH_FIELDS(SH_KF)
 #undef SH_KF
+  __device__ __forceinline__ GFields all() const {
+    GFields g;
+#define SH_KA(n) g.n = n.get();
+    SH_FIELDS(SH_KA)
+#undef SH_KA
+    return g;
+  }
 };
 
 // The point-wise stages are unrolled over the S slots with static indices, so
@@ -457,7 +474,18 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
 }
 
 // ------------------------------------------------------------ icloud (:1220-1449)
-template <int S, class R, class FS>
+// The point-wise processes of icloud run on the wavefront's cells in a sorted
+// order: cells holding more than QRMIN of some condensate first, then those
+// holding more than QCMIN, then the rest (and the lanes past the last
+// column).  A branch costs the wavefront a pass per slot in which any lane
+// takes it; most branches need condensate, and a column's cloud layers are a
+// fifth of its levels spread over every slot, so sorted they run in about one
+// slot instead of two or three.  The state is moved through LDS cells that are
+// free during icloud and moved back after; every cell's arithmetic is the
+// same as in cell order.  With the ice slope limiter the cells keep their
+// order (the limiter's results are per slot).  SORT = false keeps cell order
+// everywhere.
+template <int S, class R, class FS, bool SORT = true>
 __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
                                        const FS& f, const shield_mp_params& P, const Coef<R>& c) {
   const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr, ql_mlt = P.nml.ql_mlt;
@@ -484,34 +512,92 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
     }
     slope_limiter(L, sh, X.qi, dl);
   }
+  // p1 = den0 rdgas t0 from the call's input pt, delz (:149-151), in cell order
+  double p1[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int32_t o = G.off[s];
+    const bool v = L.v[s];
+    const double t0 = v ? f.pt[o] : 250.0;
+    const double dz0 = v ? f.delz[o] : -100.0;
+    p1[s] = dry_density<R>(X.dp1[s], dz0) * RDGAS * t0;
+  }
+  int pos[S], src[S];
+  const bool sorted = SORT && !slope;
+  if (sorted) {
+    int cls[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double qm = mmax(mmax(mmax(X.ql[s], X.qr[s]), mmax(X.qi[s], X.qs[s])), X.qg[s]);
+      cls[s] = !L.v[s] ? 2 : qm > QRMIN ? 0 : qm > QCMIN ? 1 : 2;
+    }
+    sort_cells3(cls, pos, src, reinterpret_cast<int*>(sh.buf));
+    double* b1 = sh.cell[CL_M1SOL];
+    double* b2 = sh.cell[CL_DZ1];
+    double* b3 = sh.cell[CL_TMP];
+    put_cells(X.tz, sh.buf); put_cells(X.qv, b1); put_cells(X.ql, b2); put_cells(X.qr, b3);
+    __syncthreads();
+    get_cells(X.tz, sh.buf, src); get_cells(X.qv, b1, src); get_cells(X.ql, b2, src); get_cells(X.qr, b3, src);
+    __syncthreads();
+    put_cells(X.qi, sh.buf); put_cells(X.qs, b1); put_cells(X.qg, b2); put_cells(X.qaz, b3);
+    __syncthreads();
+    get_cells(X.qi, sh.buf, src); get_cells(X.qs, b1, src); get_cells(X.qg, b2, src); get_cells(X.qaz, b3, src);
+    __syncthreads();
+    // the fall speeds are not used again before sedimentation recomputes them
+    put_cells(p1, sh.buf);
+    put_cells(X.vtg, reinterpret_cast<R*>(b1));
+    put_cells(X.vtr, reinterpret_cast<R*>(b2));
+    put_cells(X.vts, reinterpret_cast<R*>(b3));
+    __syncthreads();
+    get_cells(p1, sh.buf, src);
+    get_cells(X.vtg, reinterpret_cast<const R*>(b1), src);
+    get_cells(X.vtr, reinterpret_cast<const R*>(b2), src);
+    get_cells(X.vts, reinterpret_cast<const R*>(b3), src);
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) src[s] = pos[s] = 64 * s + lane;
+  }
+  const int ncell = L.ncols * L.nz;
+  // column of cell id: floor((id + 1/2) / nz) in FP32 is exact here, the
+  // quotient stays >= 1/(2 nz) away from an integer
+  const float rnz = 1.0f / float(L.nz);
   for_slots<S>([&](auto si) __attribute__((always_inline)) {
     constexpr int s = decltype(si)::value;
-    const int cl = L.cl[s];
+    const int id = src[s];
+    const int cl = id < ncell ? int((float(id) + 0.5f) * rnz) : 0;
     const double hv = sh.colv[CV_HVAR][cl];
-    const double den = sh.at(CL_DEN, s);
+    const double den = sh.cell[CL_DEN][id];
     Cell cs = cell_of(X, s);
     double q_liq, q_sol, cvm;
     double di;
     if (slope) {
-      q_liq = q_liq_l[64 * s + lane];
-      q_sol = q_sol_l[64 * s + lane];
-      cvm = cvm_l[64 * s + lane];
+      q_liq = q_liq_l[id];
+      q_sol = q_sol_l[id];
+      cvm = cvm_l[id];
       di = mmax(dl[s], mmax(QVMIN, hv * cs.qi));
     } else {
       icloud_melt_freeze<R>(c, tice_d, t_wfr_d, ql_mlt, cs, den, q_liq, q_sol, cvm);
       di = mmax(QVMIN, hv * cs.qi);
     }
-    // p1 = den0 rdgas t0 from the call's input pt, delz (:149-151)
-    const int32_t o = G.off[s];
-    const bool v = L.v[s];
-    const double t0 = v ? f.pt[o] : 250.0;
-    const double dz0 = v ? f.delz[o] : -100.0;
-    const double p1 = dry_density<R>(X.dp1[s], dz0) * RDGAS * t0;
     icloud_main<R>(c, tice_d, P.tice0, cs, R(hv), sh.colv[CV_RHADJ][cl], sh.colv[CV_RHRAIN][cl], den,
-                   R(sh.at(CL_DENFAC, s)), X.vtg[s], X.vtr[s], X.vts[s], p1, R(di), q_liq, q_sol,
+                   R(sh.cell[CL_DENFAC][id]), X.vtg[s], X.vtr[s], X.vts[s], p1[s], R(di), q_liq, q_sol,
                    cvm);
     set_cell(X, s, cs);
   });
+  if (sorted) {
+    double* b1 = sh.cell[CL_M1SOL];
+    double* b2 = sh.cell[CL_DZ1];
+    double* b3 = sh.cell[CL_TMP];
+    put_cells(X.tz, sh.buf); put_cells(X.qv, b1); put_cells(X.ql, b2); put_cells(X.qr, b3);
+    __syncthreads();
+    get_cells(X.tz, sh.buf, pos); get_cells(X.qv, b1, pos); get_cells(X.ql, b2, pos); get_cells(X.qr, b3, pos);
+    __syncthreads();
+    put_cells(X.qi, sh.buf); put_cells(X.qs, b1); put_cells(X.qg, b2); put_cells(X.qaz, b3);
+    __syncthreads();
+    get_cells(X.qi, sh.buf, pos); get_cells(X.qs, b1, pos); get_cells(X.qg, b2, pos); get_cells(X.qaz, b3, pos);
+    __syncthreads();
+  }
 }
 
 // The state update that follows the microphysics for this wavefront's
@@ -613,7 +699,7 @@ __device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>&
 // waves; every other instance is left to the register allocator.
 template <int S, class R, bool MULTI, bool FUSED>
 constexpr int min_waves() {
-  return (S <= 3 && !MULTI && (sizeof(R) == 4 || FUSED)) ? 3 : 1;
+  return (S <= 3 && !MULTI) ? 3 : 1;
 }
 
 // MULTI: more than one sub-step.  With a single sub-step the body is not
@@ -687,45 +773,69 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
   // reference does; fields_update then reads them back.
   const bool de_ice = n.de_ice && P.accumulate;
   St<S, R> X;
+  {
+    // every input of every slot is loaded before any is used, so the loads
+    // of the wavefront are in flight together
+    const GFields F = f.all();
+    double r_pt[S], r_dp[S], r_qv[S], r_ql[S], r_qr[S], r_qi[S], r_qs[S], r_qg[S], r_qa[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (L.v[s]) {
+    for (int s = 0; s < S; ++s) {
       const int32_t o = G.off[s];
-      double qi = f.qi[o], qs = f.qs[o];
-      if (de_ice) {
-        const double qi_dt = f.qi_dt[o];
-        const double qio = qi - P.timestep * qi_dt;
+      if (L.v[s]) {
+        r_pt[s] = F.pt[o];
+        r_dp[s] = F.delp[o];
+        r_qv[s] = F.qv[o];
+        r_ql[s] = F.ql[o];
+        r_qr[s] = F.qr[o];
+        r_qi[s] = F.qi[o];
+        r_qs[s] = F.qs[o];
+        r_qg[s] = F.qg[o];
+        r_qa[s] = n.reset_qaz ? 0.0 : F.qaz[o];
+      }
+    }
+    if (de_ice) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        if (!L.v[s]) continue;
+        const int32_t o = G.off[s];
+        const double qi_dt = F.qi_dt[o];
+        const double qio = r_qi[s] - P.timestep * qi_dt;
         const double qin = mmax(qio, n.qi0_max);
-        if (qi > qin) {
-          qs = qs + qi - qin;
-          qi = qin;
+        if (r_qi[s] > qin) {
+          r_qs[s] = r_qs[s] + r_qi[s] - qin;
+          r_qi[s] = qin;
           const double dqi = (qin - qio) * P.rdt;
-          f.qs_dt[o] = f.qs_dt[o] + qi_dt - dqi;
-          f.qi_dt[o] = dqi;
-          f.qs[o] = qs;
-          f.qi[o] = qi;
+          F.qs_dt[o] = F.qs_dt[o] + qi_dt - dqi;
+          F.qi_dt[o] = dqi;
+          F.qs[o] = r_qs[s];
+          F.qi[o] = r_qi[s];
         }
       }
-      X.tz[s] = f.pt[o];
-      const double dp0 = f.delp[o];
-      const double qv = f.qv[o];
-      X.dp1[s] = dp0 * (1.0 - qv);
-      const double omq = fdiv<R>(dp0, X.dp1[s]);
-      // the products are materialised exactly (launder) because
-      // fields_update forms the same products again and subtracts
-      X.qv[s] = launder(qv * omq);
-      X.ql[s] = launder(f.ql[o] * omq);
-      X.qr[s] = launder(f.qr[o] * omq);
-      X.qi[s] = launder(qi * omq);
-      X.qs[s] = launder(qs * omq);
-      X.qg[s] = launder(f.qg[o] * omq);
-      X.qaz[s] = n.reset_qaz ? 0.0 : f.qaz[o];
-    } else {
-      X.tz[s] = 250.0;
-      X.dp1[s] = 100.0;
-      X.qv[s] = X.ql[s] = X.qr[s] = X.qi[s] = X.qs[s] = X.qg[s] = X.qaz[s] = 0.0;
     }
-    X.vtr[s] = X.vtg[s] = X.vts[s] = R(VF_MIN);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (L.v[s]) {
+        X.tz[s] = r_pt[s];
+        const double dp0 = r_dp[s];
+        const double qv = r_qv[s];
+        X.dp1[s] = dp0 * (1.0 - qv);
+        const double omq = fdiv<R>(dp0, X.dp1[s]);
+        // the products are materialised exactly (launder) because
+        // fields_update forms the same products again and subtracts
+        X.qv[s] = launder(qv * omq);
+        X.ql[s] = launder(r_ql[s] * omq);
+        X.qr[s] = launder(r_qr[s] * omq);
+        X.qi[s] = launder(r_qi[s] * omq);
+        X.qs[s] = launder(r_qs[s] * omq);
+        X.qg[s] = launder(r_qg[s] * omq);
+        X.qaz[s] = r_qa[s];
+      } else {
+        X.tz[s] = 250.0;
+        X.dp1[s] = 100.0;
+        X.qv[s] = X.ql[s] = X.qr[s] = X.qi[s] = X.qs[s] = X.qg[s] = X.qaz[s] = 0.0;
+      }
+      X.vtr[s] = X.vtg[s] = X.vts[s] = R(VF_MIN);
+    }
   }
   // ccn scaled by the surface-layer density, then c_praut (:174-186)
 #pragma unroll
@@ -834,6 +944,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
   // again from the inputs and recomputed with the operations fields_init used.
   const double rdt = P.rdt;
   const bool acc = P.accumulate != 0;
+  // every input of every slot is loaded before the first store, so the loads
+  // of the wavefront are in flight together (a store may alias a later load
+  // as far as the compiler knows, which would otherwise serialise them)
+  const GFields F = f.all();
+  double e_dp[S], e_u[S], e_v[S], e_pt[S], e_qa[S], e_q[6][S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int32_t o = G.off[s];
+    if (L.v[s]) {
+      e_dp[s] = F.delp[o];
+      e_u[s] = F.ua[o];
+      e_v[s] = F.va[o];
+      e_pt[s] = F.pt[o];
+      e_qa[s] = F.qa[o];
+      e_q[0][s] = F.qv[o];
+      e_q[1][s] = F.ql[o];
+      e_q[2][s] = F.qr[o];
+      e_q[3][s] = F.qi[o];
+      e_q[4][s] = F.qs[o];
+      e_q[5][s] = F.qg[o];
+    } else {
+      e_dp[s] = 100.0;
+      e_u[s] = e_v[s] = e_pt[s] = e_qa[s] = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) e_q[q][s] = 0.0;
+    }
+  }
   if (n.sedi_transport) {
     double m1[S], mp[S], a[S], bu[S], bv[S], u1[S], v1[S];
 #pragma unroll
@@ -841,15 +978,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
     col_prev(sh.buf, m1, mp);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const int32_t o = G.off[s];
-      const double u0 = L.v[s] ? f.ua[o] : 0.0;
-      const double v0 = L.v[s] ? f.va[o] : 0.0;
+      const double u0 = e_u[s];
+      const double v0 = e_v[s];
       if (L.k[s] == 0) {
         a[s] = 0.0;
         bu[s] = u0;
         bv[s] = v0;
       } else {
-        const double dp0 = L.v[s] ? f.delp[o] : 100.0;
+        const double dp0 = e_dp[s];
         const double r = recip<R>(dp0 + mp[s]);
         a[s] = mp[s] * r;
         if constexpr (sizeof(R) == 8) {
@@ -869,56 +1005,55 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
       if (!L.v[s]) continue;
       const int32_t o = G.off[s];
       if (L.k[s] > 0) {
-        const double u0 = f.ua[o], v0 = f.va[o];
-        f.udt[o] = (acc ? f.udt[o] : 0.0) + (u1[s] - u0) * rdt;
-        f.vdt[o] = (acc ? f.vdt[o] : 0.0) + (v1[s] - v0) * rdt;
+        F.udt[o] = (acc ? F.udt[o] : 0.0) + (u1[s] - e_u[s]) * rdt;
+        F.vdt[o] = (acc ? F.vdt[o] : 0.0) + (v1[s] - e_v[s]) * rdt;
       } else if (!acc) {
-        f.udt[o] = 0.0;
-        f.vdt[o] = 0.0;
+        F.udt[o] = 0.0;
+        F.vdt[o] = 0.0;
       }
     }
   } else if (!acc) {
 #pragma unroll
     for (int s = 0; s < S; ++s)
       if (L.v[s]) {
-        f.udt[G.off[s]] = 0.0;
-        f.vdt[G.off[s]] = 0.0;
+        F.udt[G.off[s]] = 0.0;
+        F.vdt[G.off[s]] = 0.0;
       }
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!L.v[s]) continue;
     const int32_t o = G.off[s];
-    const double dp0 = f.delp[o];
+    const double dp0 = e_dp[s];
     const double omq0 = fdiv<R>(dp0, X.dp1[s]);
     const double omq = fdiv<R>(X.dp1[s], dp0);
     // q0 is rounded exactly as fields_init rounded it (launder keeps the
     // product out of a fused multiply-add with the difference), so an
     // untouched species has a zero tendency
-    f.qv_dt[o] = (acc ? f.qv_dt[o] : 0.0) + rdt * (X.qv[s] - launder(f.qv[o] * omq0)) * omq;
-    f.ql_dt[o] = (acc ? f.ql_dt[o] : 0.0) + rdt * (X.ql[s] - launder(f.ql[o] * omq0)) * omq;
-    f.qr_dt[o] = (acc ? f.qr_dt[o] : 0.0) + rdt * (X.qr[s] - launder(f.qr[o] * omq0)) * omq;
-    f.qi_dt[o] = (acc ? f.qi_dt[o] : 0.0) + rdt * (X.qi[s] - launder(f.qi[o] * omq0)) * omq;
-    f.qs_dt[o] = (acc ? f.qs_dt[o] : 0.0) + rdt * (X.qs[s] - launder(f.qs[o] * omq0)) * omq;
-    f.qg_dt[o] = (acc ? f.qg_dt[o] : 0.0) + rdt * (X.qg[s] - launder(f.qg[o] * omq0)) * omq;
+    F.qv_dt[o] = (acc ? F.qv_dt[o] : 0.0) + rdt * (X.qv[s] - launder(e_q[0][s] * omq0)) * omq;
+    F.ql_dt[o] = (acc ? F.ql_dt[o] : 0.0) + rdt * (X.ql[s] - launder(e_q[1][s] * omq0)) * omq;
+    F.qr_dt[o] = (acc ? F.qr_dt[o] : 0.0) + rdt * (X.qr[s] - launder(e_q[2][s] * omq0)) * omq;
+    F.qi_dt[o] = (acc ? F.qi_dt[o] : 0.0) + rdt * (X.qi[s] - launder(e_q[3][s] * omq0)) * omq;
+    F.qs_dt[o] = (acc ? F.qs_dt[o] : 0.0) + rdt * (X.qs[s] - launder(e_q[4][s] * omq0)) * omq;
+    F.qg_dt[o] = (acc ? F.qg_dt[o] : 0.0) + rdt * (X.qg[s] - launder(e_q[5][s] * omq0)) * omq;
     const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
                        (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
-    f.pt_dt[o] = (acc ? f.pt_dt[o] : 0.0) + fdiv<R>(rdt * (X.tz[s] - f.pt[o]) * cvm, CP_AIR);
+    F.pt_dt[o] = (acc ? F.pt_dt[o] : 0.0) + fdiv<R>(rdt * (X.tz[s] - e_pt[s]) * cvm, CP_AIR);
     if (n.do_qa)
-      f.qa_dt[o] = 0.0;
+      F.qa_dt[o] = 0.0;
     else
-      f.qa_dt[o] = (acc ? f.qa_dt[o] : 0.0) + rdt * (fdiv<R>(X.qaz[s], P.ntimes) - f.qa[o]);
-    f.qaz[o] = X.qaz[s];
+      F.qa_dt[o] = (acc ? F.qa_dt[o] : 0.0) + rdt * (fdiv<R>(X.qaz[s], P.ntimes) - e_qa[s]);
+    F.qaz[o] = X.qaz[s];
   }
   if (lane < ncols) {
     const int64_t col = c0 + lane;
     const int64_t i = col / sh3.nj, j = col - i * sh3.nj;
     const int64_t o2 = i * sh3.si2 + j * sh3.sj2;
     const double convt = 86400.0 * rdt * RGRAV;
-    if (f.rain) f.rain[o2] = sh.colv[CV_RAIN][lane] * convt;
-    if (f.snow) f.snow[o2] = sh.colv[CV_SNOW][lane] * convt;
-    if (f.ice) f.ice[o2] = sh.colv[CV_ICE][lane] * convt;
-    if (f.graupel) f.graupel[o2] = sh.colv[CV_GRAUPEL][lane] * convt;
+    if (F.rain) F.rain[o2] = sh.colv[CV_RAIN][lane] * convt;
+    if (F.snow) F.snow[o2] = sh.colv[CV_SNOW][lane] * convt;
+    if (F.ice) F.ice[o2] = sh.colv[CV_ICE][lane] * convt;
+    if (F.graupel) F.graupel[o2] = sh.colv[CV_GRAUPEL][lane] * convt;
   }
   if constexpr (FUSED) state_update(L, G, sh, sh3, c0, a.f, u);
 }

@@ -56,6 +56,56 @@ __device__ __forceinline__ void for_slots(F&& f) {
   for_slots_impl(f, std::make_integer_sequence<int, S>{});
 }
 
+// number of set bits of m below this lane
+__device__ __forceinline__ int bits_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// A stable counting sort of the wavefront's 64 S cells by a class 0, 1 or 2:
+// pos[s] is the sorted position of this lane's cell in slot s, src[s] the
+// cell (64 s' + lane') that sorted position 64 s + lane holds.  `ib` is LDS
+// scratch of 64 S ints; the workgroup is synchronised on return.
+template <int S>
+__device__ __forceinline__ void sort_cells3(const int (&cls)[S], int (&pos)[S], int (&src)[S], int* ib) {
+  const int lane = lane_id();
+  uint64_t b0[S], b1[S];
+  int p0[S], p1[S], n0 = 0, n1 = 0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    b0[s] = __ballot(cls[s] == 0);
+    b1[s] = __ballot(cls[s] == 1);
+    p0[s] = n0;
+    p1[s] = n1;
+    n0 += __popcll(b0[s]);
+    n1 += __popcll(b1[s]);
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int idx = 64 * s + lane;
+    const int r0 = p0[s] + bits_below(b0[s]), r1 = p1[s] + bits_below(b1[s]);
+    pos[s] = cls[s] == 0 ? r0 : cls[s] == 1 ? n0 + r1 : n0 + n1 + (idx - r0 - r1);
+    ib[pos[s]] = idx;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S; ++s) src[s] = ib[64 * s + lane];
+  __syncthreads();
+}
+
+// Per-slot values through LDS: put_cells writes them in cell order, get_cells
+// reads cell from[s] into slot s (after a workgroup synchronisation).
+template <int S, class T>
+__device__ __forceinline__ void put_cells(const T (&v)[S], T* buf) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) buf[64 * s + lane] = v[s];
+}
+template <int S, class T>
+__device__ __forceinline__ void get_cells(T (&v)[S], const T* buf, const int (&from)[S]) {
+#pragma unroll
+  for (int s = 0; s < S; ++s) v[s] = buf[from[s]];
+}
+
 __device__ __forceinline__ double readlane(double v, int l) {
   int lo = __double2loint(v), hi = __double2hiint(v);
   lo = __builtin_amdgcn_readlane(lo, l);
