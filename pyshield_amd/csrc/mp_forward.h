@@ -289,30 +289,52 @@ __device__ __forceinline__ void slope_limiter(const Lay<S>& L, Shared<S>& sh, co
   }
 }
 
+// Air density from hydrostatic balance (:331-341) with t0 and dz0 the call's
+// input pt and delz: the layer thickness dz1, den and denfac of the sub-step,
+// and icloud's p1 = den0 rdgas t0 (:149-151), into their LDS cells (p1 into
+// CL_TMP, which icloud reads before it reuses the cell).
+template <int S, class R>
+__device__ __forceinline__ void air_density(const Lay<S>& L, const St<S, R>& X, Shared<S>& sh,
+                                            const double (&t0)[S], const double (&dz0)[S]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double dz1 = fdiv<R>(dz0[s] * X.tz[s], t0[s]);
+    sh.cell[CL_DZ1][64 * s + lane] = dz1;
+    const double den0 = dry_density<R>(X.dp1[s], dz0[s]);
+    const double den = fdiv<R>(den0 * dz0[s], dz1);
+    sh.at(CL_DEN, s) = den;
+    // denfac only enters the rate laws, so mixed mode takes it in FP32
+    if constexpr (sizeof(R) == 4)
+      sh.at(CL_DENFAC, s) = double(msqrt(R(SFCRHO) / R(den)));
+    else
+      sh.at(CL_DENFAC, s) = sqrt(SFCRHO / den);
+    sh.cell[CL_TMP][64 * s + lane] = den0 * RDGAS * t0[s];
+  }
+}
+
 // ------------------------------------------------------ warm_rain (:279-653)
+// `density`: re-read t0 and dz0 and recompute the air density (sub-steps
+// after the first; the kernel computes it for the first from the inputs it
+// has loaded).
 template <int S, class R, class FS>
 __device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
                                           const FS& f, const shield_mp_params& P, const Coef<R>& c,
-                                          bool is_first) {
+                                          bool is_first, bool density) {
   const int lane = lane_id();
   const double t_wfr = P.t_wfr;
-  if (is_first) {
-    // air density from hydrostatic balance (:331-341); t0 and dz0 are the
-    // call's input pt and delz
+  if (is_first && density) {
+    double t0[S], dz0[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const double t0 = L.v[s] ? f.pt[G.off[s]] : 250.0;
-      const double dz0 = L.v[s] ? f.delz[G.off[s]] : -100.0;
-      const double dz1 = fdiv<R>(dz0 * X.tz[s], t0);
-      sh.cell[CL_DZ1][64 * s + lane] = dz1;
-      const double den = fdiv<R>(dry_density<R>(X.dp1[s], dz0) * dz0, dz1);
-      sh.at(CL_DEN, s) = den;
-      // denfac only enters the rate laws, so mixed mode takes it in FP32
-      if constexpr (sizeof(R) == 4)
-        sh.at(CL_DENFAC, s) = double(msqrt(R(SFCRHO) / R(den)));
-      else
-        sh.at(CL_DENFAC, s) = sqrt(SFCRHO / den);
+      t0[s] = 250.0;
+      dz0[s] = -100.0;
+      if (L.v[s]) {
+        t0[s] = f.pt[G.off[s]];
+        dz0[s] = f.delz[G.off[s]];
+      }
     }
+    air_density(L, X, sh, t0, dz0);
   }
   bool pred[S], fall[S];
 #pragma unroll
@@ -484,13 +506,30 @@ __device__ __forceinline__ void sedimentation(const Lay<S>& L, St<S, R>& X, Shar
 // free during icloud and moved back after; every cell's arithmetic is the
 // same as in cell order.  With the ice slope limiter the cells keep their
 // order (the limiter's results are per slot).  SORT = false keeps cell order
-// everywhere.
+// everywhere and reads p1's inputs from HBM (the adjoint's replay, whose
+// warm_rain does not leave p1 in LDS).
 template <int S, class R, class FS, bool SORT = true>
 __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
                                        const FS& f, const shield_mp_params& P, const Coef<R>& c) {
   const double tice_d = P.nml.tice, t_wfr_d = P.t_wfr, ql_mlt = P.nml.ql_mlt;
   const bool slope = P.nml.z_slope_ice != 0;
   const int lane = lane_id();
+  // p1 = den0 rdgas t0 from the call's input pt, delz (:149-151), in cell
+  // order: in the forward kernel from the LDS cell air_density left it in
+  // (before the slope limiter's or the sort's use of that cell)
+  double p1[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if constexpr (SORT) {
+      p1[s] = sh.cell[CL_TMP][64 * s + lane];
+    } else {
+      const int32_t o = G.off[s];
+      const bool v = L.v[s];
+      const double t0 = v ? f.pt[o] : 250.0;
+      const double dz0 = v ? f.delz[o] : -100.0;
+      p1[s] = dry_density<R>(X.dp1[s], dz0) * RDGAS * t0;
+    }
+  }
   // With the ice slope limiter the whole column's melt / freeze must be done
   // before the limiter sees cloud ice; the heat-capacity terms it leaves are
   // parked in LDS cells that are free during icloud (m1_sol is consumed by
@@ -511,16 +550,6 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
       cvm_l[64 * s + lane] = cvm;
     }
     slope_limiter(L, sh, X.qi, dl);
-  }
-  // p1 = den0 rdgas t0 from the call's input pt, delz (:149-151), in cell order
-  double p1[S];
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int32_t o = G.off[s];
-    const bool v = L.v[s];
-    const double t0 = v ? f.pt[o] : 250.0;
-    const double dz0 = v ? f.delz[o] : -100.0;
-    p1[s] = dry_density<R>(X.dp1[s], dz0) * RDGAS * t0;
   }
   int pos[S], src[S];
   const bool sorted = SORT && !slope;
@@ -684,10 +713,10 @@ __device__ __forceinline__ void state_update(const Lay<S>& L, const Glob<S>& G, 
 // One sub-step (microphysics.py:2045-2220).
 template <int S, class R, class FS>
 __device__ __forceinline__ void substep(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Glob<S>& G,
-                                        const FS& f, const shield_mp_params& P, const Coef<R>& c) {
-  warm_rain(L, X, sh, G, f, P, c, true);
+                                        const FS& f, const shield_mp_params& P, const Coef<R>& c, bool density) {
+  warm_rain(L, X, sh, G, f, P, c, true, density);
   sedimentation(L, X, sh, G, f, P, c);
-  warm_rain(L, X, sh, G, f, P, c, false);
+  warm_rain(L, X, sh, G, f, P, c, false, false);
   icloud(L, X, sh, G, f, P, c);
 }
 
@@ -773,6 +802,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
   // reference does; fields_update then reads them back.
   const bool de_ice = n.de_ice && P.accumulate;
   St<S, R> X;
+  // the call's input pt and delz, for the first sub-step's air density
+  double t0_in[S], dz0_in[S];
   {
     // every input of every slot is loaded before any is used, so the loads
     // of the wavefront are in flight together
@@ -781,8 +812,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int32_t o = G.off[s];
+      t0_in[s] = 250.0;
+      dz0_in[s] = -100.0;
       if (L.v[s]) {
-        r_pt[s] = F.pt[o];
+        t0_in[s] = F.pt[o];
+        dz0_in[s] = F.delz[o];
+        r_pt[s] = t0_in[s];
         r_dp[s] = F.delp[o];
         r_qv[s] = F.qv[o];
         r_ql[s] = F.ql[o];
@@ -929,14 +964,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
   for (int s = 0; s < S; ++s) {
     sh.cell[CL_M1][64 * s + lane] = 0.0;
     sh.cell[CL_M1SOL][64 * s + lane] = 0.0;
-    sh.cell[CL_DZ1][64 * s + lane] = 1.0;
   }
+  air_density(L, X, sh, t0_in, dz0_in);
 
   // --------------------------------------------------------- sub-steps (:2045)
   if constexpr (MULTI) {
-    for (int it = 0; it < P.ntimes; ++it) substep(L, X, sh, G, f, P, c);
+    for (int it = 0; it < P.ntimes; ++it) substep(L, X, sh, G, f, P, c, it > 0);
   } else {
-    substep(L, X, sh, G, f, P, c);
+    substep(L, X, sh, G, f, P, c, false);
   }
 
   // ---------------------------------------------------- fields_update (:1452-1553)
