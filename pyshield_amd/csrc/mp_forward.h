@@ -313,6 +313,48 @@ __device__ __forceinline__ void air_density(const Lay<S>& L, const St<S, R>& X, 
   }
 }
 
+// revap_racc on the cells of the columns with falling rain.  It changes only
+// cells warmer than t_wfr with rain (its first branch), so those are packed
+// into one slot when there are at most 64 of them, and it runs once there;
+// qi, qs and qg enter only through their sum, which is packed in qi's place
+// (the sum plus two zeros is the sum).  Otherwise it runs per slot.
+template <int S, class R>
+__device__ __forceinline__ void evaporation(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const Coef<R>& c,
+                                            double t_wfr, R dt5, const bool (&fall)[S]) {
+  bool pred[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) pred[s] = fall[s] && (X.tz[s] > t_wfr) && (X.qr[s] > QRMIN);
+  const Packed<S> pk = pack_cells(pred);
+  if (pk.n <= 64) {
+    if (pk.n == 0) return;
+    double q_sol[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) q_sol[s] = X.qi[s] + X.qs[s] + X.qg[s];
+    Cell cs{gather_packed(X.qv, pk.src), gather_packed(X.ql, pk.src), gather_packed(X.qr, pk.src),
+            gather_packed(q_sol, pk.src), 0.0, 0.0, gather_packed(X.tz, pk.src), 0.0};
+    const int lane = lane_id();
+    if (lane < pk.n) {
+      const int id = pk.src;
+      // column of cell id: floor((id + 1/2) / nz) in FP32 is exact here
+      const int cl = int((float(id) + 0.5f) * (1.0f / float(L.nz)));
+      revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][cl]), cs, sh.cell[CL_DEN][id], R(sh.cell[CL_DENFAC][id]));
+    }
+    scatter_packed(X.qv, cs.qv, pk, pred);
+    scatter_packed(X.ql, cs.ql, pk, pred);
+    scatter_packed(X.qr, cs.qr, pk, pred);
+    scatter_packed(X.tz, cs.tz, pk, pred);
+    return;
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (fall[s]) {
+      Cell cs = cell_of(X, s);
+      revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][L.cl[s]]), cs, sh.at(CL_DEN, s), R(sh.at(CL_DENFAC, s)));
+      set_cell(X, s, cs);
+    }
+  }
+}
+
 // ------------------------------------------------------ warm_rain (:279-653)
 // `density`: re-read t0 and dz0 and recompute the air density (sub-steps
 // after the first; the kernel computes it for the first from the inputs it
@@ -350,28 +392,12 @@ __device__ __forceinline__ void warm_rain(const Lay<S>& L, St<S, R>& X, Shared<S
   }
   const R dt5 = R(0.5) * c.dt_rain;
   if (wave_any(fall)) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (fall[s]) {
-        Cell cs = cell_of(X, s);
-        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][L.cl[s]]), cs, sh.at(CL_DEN, s),
-                      R(sh.at(CL_DENFAC, s)));
-        set_cell(X, s, cs);
-      }
-    }
+    evaporation(L, X, sh, c, t_wfr, dt5, fall);
     double dz[S];
     layer_dz(L, sh, dz);
     implicit_fall(L, X, sh, G, f, dz, X.qr, X.vtr, P.dt_rain, fall, P.do_sedi_w != 0, flux, surf);
     if (P.nml.do_sedi_heat) sedi_heat(L, X, sh, flux, C_LIQ, fall);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (fall[s]) {
-        Cell cs = cell_of(X, s);
-        revap_racc<R>(c, t_wfr, dt5, R(sh.colv[CV_HVAR][L.cl[s]]), cs, sh.at(CL_DEN, s),
-                      R(sh.at(CL_DENFAC, s)));
-        set_cell(X, s, cs);
-      }
-    }
+    evaporation(L, X, sh, c, t_wfr, dt5, fall);
   }
   // auto-conversion (:587-645)
   if (P.nml.irain_f != 0) {

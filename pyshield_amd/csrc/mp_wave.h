@@ -92,6 +92,76 @@ __device__ __forceinline__ void sort_cells3(const int (&cls)[S], int (&pos)[S], 
   __syncthreads();
 }
 
+// The value of v at lane `from` of the wavefront (two ds_bpermute_b32, which
+// move data through the LDS crossbar without using LDS memory).  Every lane
+// must be active.
+__device__ __forceinline__ double pull(double v, int from) {
+  const int a = from << 2;
+  const int lo = __builtin_amdgcn_ds_bpermute(a, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(a, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// The cells of the wavefront where pred holds, packed into the lanes 0 .. n-1
+// of one slot (when n <= 64), in cell order.  A branch that only those cells
+// take then runs once instead of once per slot that holds one of them.
+template <int S>
+struct Packed {
+  int n;       // cells packed (the same in every lane)
+  int src;     // packed lane j < n: its cell, 64 s + l
+  int pos[S];  // this lane's cell of slot s: its packed lane (where pred holds)
+};
+template <int S>
+__device__ __forceinline__ Packed<S> pack_cells(const bool (&pred)[S]) {
+  Packed<S> P;
+  const int lane = lane_id();
+  uint64_t m[S];
+  int pre[S], cnt[S], n = 0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    m[s] = __ballot(pred[s]);
+    pre[s] = n;
+    cnt[s] = __popcll(m[s]);
+    n += cnt[s];
+  }
+  P.n = n;
+  P.src = 0;
+  if (n > 64) return P;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    // slot s sends its cells to lanes pre .. pre+cnt-1 and, to make the
+    // forward permute a bijection, the other lanes to the remaining ones
+    const int below = bits_below(m[s]);
+    const int p = pred[s] ? pre[s] + below : (pre[s] + cnt[s] + (lane - below)) & 63;
+    P.pos[s] = p;
+    const int t = __builtin_amdgcn_ds_permute(p << 2, 64 * s + lane);
+    if (lane >= pre[s] && lane < pre[s] + cnt[s]) P.src = t;
+  }
+  return P;
+}
+// the value of the per-slot v at cell `src` (every lane active)
+template <int S>
+__device__ __forceinline__ double gather_packed(const double (&v)[S], int src) {
+  const int sl = src >> 6, l = src & 63;
+  double r = pull(v[0], l);
+#pragma unroll
+  for (int s = 1; s < S; ++s) {
+    const double t = pull(v[s], l);
+    r = sl == s ? t : r;
+  }
+  return r;
+}
+// v[s] = the packed value of this lane's cell, where pred holds
+template <int S>
+__device__ __forceinline__ void scatter_packed(double (&v)[S], double pv, const Packed<S>& P,
+                                               const bool (&pred)[S]) {
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double t = pull(pv, P.pos[s]);
+    if (pred[s]) v[s] = t;
+  }
+}
+
 // Per-slot values through LDS: put_cells writes them in cell order, get_cells
 // reads cell from[s] into slot s (after a workgroup synchronisation).
 template <int S, class T>
