@@ -76,6 +76,31 @@ __host__ __device__ constexpr int rec_offset(int rec) {
 constexpr int REC_FIELDS = rec_offset(REC_N);
 static_assert(REC_FIELDS == 77, "fields per sub-step");
 
+// The adjoint kernel's argument block.  As in the forward kernel, the field
+// pointers of f and a are read from the kernarg segment where they are used
+// (KPtr), not held in scalar registers for the whole kernel.
+struct AdjArgs {
+  shield_mp_shape sh3;
+  shield_mp_params P;
+  Coef<double> c;
+  shield_mp_fields f;
+  shield_mp_adjoint_fields a;
+  double* tape;
+  int64_t tape_stride;
+  double* part;
+  int cols_per_wave;
+};
+#define SH_AFIELDS(X)                                                                                            \
+  X(pt_dt) X(qv_dt) X(ql_dt) X(qr_dt) X(qi_dt) X(qs_dt) X(qg_dt) X(qa_dt) X(udt) X(vdt) X(rain) X(snow) X(ice) \
+  X(graupel) X(pt) X(qv) X(ql) X(qr) X(qi) X(qs) X(qg) X(qa) X(ua) X(va) X(delp) X(delz) X(qaz)
+// shield_mp_adjoint_fields at kernarg offset B, member by member
+template <uint32_t B>
+struct KAdjFields {
+#define SH_KAF(n) KPtr<B + (uint32_t)offsetof(shield_mp_adjoint_fields, n)> n;
+  SH_AFIELDS(SH_KAF)
+#undef SH_KAF
+};
+
 // per-cell cotangents of the auxiliary fields, kept in LDS
 enum { AB_DEN, AB_DENFAC, AB_DZ1, AB_VTR, AB_VTG, AB_VTS, AB_M1, AB_M1SOL, AB_DP1, AB_T0, AB_DZ0, AB_N };
 
@@ -393,9 +418,9 @@ __device__ __forceinline__ void fall_fwd(const Lay<S>& L, Shared<S>& sh, const d
 
 
 // ------------------------------------------------------- taped forward
-template <int S>
+template <int S, class FS>
 __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X, Shared<S>& sh, const Glob<S>& G,
-                                              const shield_mp_fields& f, const shield_mp_params& P,
+                                              const FS& f, const shield_mp_params& P,
                                               const Coef<double>& c, bool is_first, const TapeView<S>& T, int it,
                                               int rb) {
   const int lane = lane_id();
@@ -482,9 +507,9 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
   __syncthreads();
 }
 
-template <int S>
+template <int S, class FS>
 __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Shared<S>& sh, const Glob<S>& G,
-                                        const shield_mp_fields& f, const shield_mp_params& P, const Coef<double>& c,
+                                        const FS& f, const shield_mp_params& P, const Coef<double>& c,
                                         const TapeView<S>& T, int it) {
   const int lane = lane_id();
   const double tice_d = P.nml.tice;
@@ -690,9 +715,9 @@ __device__ __forceinline__ void autoconv_adj(const Lay<S>& L, AdjShared<S>& A, A
 }
 
 // The reverse of one warm_rain call (records rb .. rb+3).
-template <int S>
+template <int S, class FS>
 __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
-                                              const shield_mp_fields& f, const shield_mp_params& P,
+                                              const FS& f, const shield_mp_params& P,
                                               const Coef<double>& c, bool is_first,
                                               const TapeView<S>& T, int it, int rb, const double (&dp1)[S]) {
   Shared<S>& sh = A.sh;
@@ -929,9 +954,9 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
 // The reverse of icloud (record RI): melt / freeze, then icloud_main with
 // subgrid_z_proc.  Inputs: the cell (0-6), den (8), denfac (9), vtg (10),
 // vtr (11), vts (12), p1 (13), qi_gen (14).
-template <int S>
+template <int S, class FS>
 __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
-                                           const shield_mp_fields& f, const shield_mp_params& P,
+                                           const FS& f, const shield_mp_params& P,
                                            const Coef<double>& c, const TapeView<S>& T, int it,
                                            const double (&dp1)[S]) {
   Shared<S>& sh = A.sh;
@@ -948,9 +973,9 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
       r.z[Z_VTR][s] = T.at(it, RW2_C, Z_VTR)[64 * s + lane];
     }
   }
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (!L.v[s]) continue;
+  for_slots<S>([&](auto si) __attribute__((always_inline)) {
+    constexpr int s = decltype(si)::value;
+    if (!L.v[s]) return;
     const int cl = L.cl[s];
     const double hv = sh.colv[CV_HVAR][cl], rh_adj = sh.colv[CV_RHADJ][cl], rh_rain = sh.colv[CV_RHRAIN][cl];
     const int32_t o = Gl.off[s];
@@ -978,16 +1003,22 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
     abar(A, AB_DP1, s) += ib.p1 * p1 / dp1[s];
     abar(A, AB_DZ0, s) -= ib.p1 * p1 / dz0;
     abar(A, AB_T0, s) += ib.p1 * p1 / t0;
-  }
+  });
 }
 
 
 // ------------------------------------------------------------------ kernel
 template <int S>
-__global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shield_mp_params P, Coef<double> c,
-                                                        shield_mp_fields f, shield_mp_adjoint_fields a,
-                                                        double* tape, int64_t tape_stride, double* part,
-                                                        int cols_per_wave) {
+__global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
+  const shield_mp_shape& sh3 = args.sh3;
+  const shield_mp_params& P = args.P;
+  const Coef<double>& c = args.c;
+  const KFields<(uint32_t)offsetof(AdjArgs, f)> f;
+  const KAdjFields<(uint32_t)offsetof(AdjArgs, a)> a;
+  double* const tape = args.tape;
+  const int64_t tape_stride = args.tape_stride;
+  double* const part = args.part;
+  const int cols_per_wave = args.cols_per_wave;
   __shared__ AdjShared<S> A;
   Shared<S>& sh = A.sh;
   const int lane = lane_id();
@@ -1096,7 +1127,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(shield_mp_shape sh3, shi
       sed_rec(L, X, sh, Gl, f, P, c, T, it);
       warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B);
       tape_save(T, it, RI, X, sh);
-      icloud<S, double, shield_mp_fields, false>(L, X, sh, Gl, f, P, c);
+      icloud<S, double, decltype(f), false>(L, X, sh, Gl, f, P, c);
     }
     __syncthreads();
     // ----------------------------------------------------- reverse walk

@@ -62,14 +62,14 @@ hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, 
   if (nw < 1) return hipErrorInvalidValue;
   double* base = static_cast<double*>(ws);
   double* part = base + nw * tape;
-  const Coef<double> c = make_coef<double>(p);
+  const AdjArgs args{sh, p, make_coef<double>(p), f, a, base, tape, part, lay.P};
   const dim3 grid((unsigned)nw), block(64);
   if (lay.S == 1)
-    hipLaunchKernelGGL(mp_adjoint_kernel<1>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part, lay.P);
+    hipLaunchKernelGGL(mp_adjoint_kernel<1>, grid, block, 0, stream, args);
   else if (lay.S == 2)
-    hipLaunchKernelGGL(mp_adjoint_kernel<2>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part, lay.P);
+    hipLaunchKernelGGL(mp_adjoint_kernel<2>, grid, block, 0, stream, args);
   else
-    hipLaunchKernelGGL(mp_adjoint_kernel<3>, grid, block, 0, stream, sh, p, c, f, a, base, tape, part, lay.P);
+    hipLaunchKernelGGL(mp_adjoint_kernel<3>, grid, block, 0, stream, args);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(adj_param_reduce, dim3(1), dim3(64), 0, stream, part, nw, a.params);
