@@ -426,16 +426,18 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
   const int lane = lane_id();
   const double t_wfr = P.t_wfr;
   if (is_first) {
+    // the forward kernel's density update (dz1, den, denfac, and icloud's p1)
+    double t0[S], dz0[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const double t0 = L.v[s] ? f.pt[G.off[s]] : 250.0;
-      const double dz0 = L.v[s] ? f.delz[G.off[s]] : -100.0;
-      const double dz1 = dz0 * X.tz[s] / t0;
-      sh.cell[CL_DZ1][64 * s + lane] = dz1;
-      const double den = dry_density<double>(X.dp1[s], dz0) * dz0 / dz1;
-      sh.at(CL_DEN, s) = den;
-      sh.at(CL_DENFAC, s) = sqrt(SFCRHO / den);
+      t0[s] = 250.0;
+      dz0[s] = -100.0;
+      if (L.v[s]) {
+        t0[s] = f.pt[G.off[s]];
+        dz0[s] = f.delz[G.off[s]];
+      }
     }
+    air_density(L, X, sh, t0, dz0);
     tape_save(T, it, RAUX, X, sh);
   }
   tape_save(T, it, rb, X, sh);
@@ -450,28 +452,14 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
   for (int s = 0; s < S; ++s) flux[s] = surf[s] = 0.0;
   const double dt5 = 0.5 * c.dt_rain;
   if (wave_any(fall)) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (fall[s]) {
-        Cell cs = cell_of(X, s);
-        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][L.cl[s]], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
-        set_cell(X, s, cs);
-      }
-    }
+    evaporation(L, X, sh, c, t_wfr, dt5, fall);
     tape_save(T, it, rb + 1, X, sh);
     double dz[S];
     layer_dz(L, sh, dz);
     implicit_fall(L, X, sh, G, f, dz, X.qr, X.vtr, P.dt_rain, fall, false, flux, surf);
     if (P.nml.do_sedi_heat) sedi_heat(L, X, sh, flux, C_LIQ, fall);
     tape_save(T, it, rb + 2, X, sh);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (fall[s]) {
-        Cell cs = cell_of(X, s);
-        revap_racc<double>(c, t_wfr, dt5, sh.colv[CV_HVAR][L.cl[s]], cs, sh.at(CL_DEN, s), sh.at(CL_DENFAC, s));
-        set_cell(X, s, cs);
-      }
-    }
+    evaporation(L, X, sh, c, t_wfr, dt5, fall);
   } else {
     // no rain falls: the reverse reads only the fall speed (icloud's)
     tape_save(T, it, rb + 1, X, sh, zbit(Z_VTR));
@@ -1127,7 +1115,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
       sed_rec(L, X, sh, Gl, f, P, c, T, it);
       warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B);
       tape_save(T, it, RI, X, sh);
-      icloud<S, double, decltype(f), false>(L, X, sh, Gl, f, P, c);
+      icloud(L, X, sh, Gl, f, P, c);
     }
     __syncthreads();
     // ----------------------------------------------------- reverse walk
