@@ -940,58 +940,105 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
 }
 
 // The reverse of icloud (record RI): melt / freeze, then icloud_main with
-// subgrid_z_proc.  Inputs: the cell (0-6), den (8), denfac (9), vtg (10),
-// vtr (11), vts (12), p1 (13), qi_gen (14).
+// subgrid_z_proc, on the cells sorted by condensate as the forward's icloud
+// sorts them (some condensate above QRMIN, above QCMIN, none), so a slot of
+// cells without condensate skips the processes that need it.  Each cell's
+// arithmetic is that of cell order; only the order of the qi_gen partial sum
+// differs.  The cell cotangents move to sorted order and back through the LDS
+// cells the reverse walk leaves free (buf, m1, m1_sol, tmp); the inputs of a
+// sorted cell are read at its cell index: the record and fall speeds from the
+// tape, den and denfac from LDS, t0, dz0 and dp1 = delp (1 - qv) from the
+// call's inputs.
 template <int S, class FS>
 __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
                                            const FS& f, const shield_mp_params& P,
-                                           const Coef<double>& c, const TapeView<S>& T, int it,
-                                           const double (&dp1)[S]) {
+                                           const Coef<double>& c, const TapeView<S>& T, int it) {
   Shared<S>& sh = A.sh;
-  Rec<S> r;
-  tape_load(T, it, RI, r, sh);
+  const int lane = lane_id();
+  int pos[S], src[S], osrc[S];
   {
-    // the fall speeds icloud used: the solid ones of sedimentation, the rain
-    // one of the second warm_rain
-    const int lane = lane_id();
+    int cls[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      r.z[Z_VTG][s] = T.at(it, RS_G, Z_VTG)[64 * s + lane];
-      r.z[Z_VTS][s] = T.at(it, RS_G, Z_VTS)[64 * s + lane];
-      r.z[Z_VTR][s] = T.at(it, RW2_C, Z_VTR)[64 * s + lane];
+      const int idx = 64 * s + lane;
+      const double qm = mmax(mmax(mmax(T.at(it, RI, Z_QL)[idx], T.at(it, RI, Z_QR)[idx]),
+                                  mmax(T.at(it, RI, Z_QI)[idx], T.at(it, RI, Z_QS)[idx])),
+                             T.at(it, RI, Z_QG)[idx]);
+      cls[s] = !L.v[s] ? 2 : qm > QRMIN ? 0 : qm > QCMIN ? 1 : 2;
     }
+    // the cells' HBM offsets ride in the upper half of buf, which the sort
+    // (64 S ints) leaves free
+    int* const ob = reinterpret_cast<int*>(sh.buf) + 64 * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) ob[64 * s + lane] = Gl.off[s];
+    sort_cells3(cls, pos, src, reinterpret_cast<int*>(sh.buf));
+#pragma unroll
+    for (int s = 0; s < S; ++s) osrc[s] = ob[src[s]];
+    __syncthreads();
   }
+  double* const b1 = sh.cell[CL_M1];
+  double* const b2 = sh.cell[CL_M1SOL];
+  double* const b3 = sh.cell[CL_TMP];
+  // cotangents of the cell after icloud in sorted order (qa's is read by the
+  // cloud fraction and passes through icloud unchanged)
+  double xs[8][S];
+  put_cells(G.xb.v[0], sh.buf); put_cells(G.xb.v[1], b1); put_cells(G.xb.v[2], b2); put_cells(G.xb.v[3], b3);
+  __syncthreads();
+  get_cells(xs[0], sh.buf, src); get_cells(xs[1], b1, src); get_cells(xs[2], b2, src); get_cells(xs[3], b3, src);
+  __syncthreads();
+  put_cells(G.xb.v[4], sh.buf); put_cells(G.xb.v[5], b1); put_cells(G.xb.v[6], b2); put_cells(G.xb.v[7], b3);
+  __syncthreads();
+  get_cells(xs[4], sh.buf, src); get_cells(xs[5], b1, src); get_cells(xs[6], b2, src); get_cells(xs[7], b3, src);
+  __syncthreads();
+  const int ncell = L.ncols * L.nz;
+  // column of cell id: floor((id + 1/2) / nz) in FP32 is exact here, the
+  // quotient stays >= 1/(2 nz) away from an integer
+  const float rnz = 1.0f / float(L.nz);
+  const double* const rq[7] = {T.at(it, RI, Z_QV), T.at(it, RI, Z_QL), T.at(it, RI, Z_QR), T.at(it, RI, Z_QI),
+                               T.at(it, RI, Z_QS), T.at(it, RI, Z_QG), T.at(it, RI, Z_TZ)};
+  const double* const rvtg = T.at(it, RS_G, Z_VTG);
+  const double* const rvts = T.at(it, RS_G, Z_VTS);
+  const double* const rvtr = T.at(it, RW2_C, Z_VTR);
   for_slots<S>([&](auto si) __attribute__((always_inline)) {
     constexpr int s = decltype(si)::value;
-    if (!L.v[s]) return;
-    const int cl = L.cl[s];
+    const int id = src[s];
+    if (id >= ncell) return;
+    const int cl = int((float(id) + 0.5f) * rnz);
     const double hv = sh.colv[CV_HVAR][cl], rh_adj = sh.colv[CV_RHADJ][cl], rh_rain = sh.colv[CV_RHRAIN][cl];
-    const int32_t o = Gl.off[s];
+    const int32_t o = osrc[s];
     const double t0 = f.pt[o], dz0 = f.delz[o];
-    const double p1 = dry_density<double>(dp1[s], dz0) * RDGAS * t0;
+    const double dp1 = f.delp[o] * (1.0 - f.qv[o]);  // fields_init's dp1, the same expression
+    const double p1 = dry_density<double>(dp1, dz0) * RDGAS * t0;
     const IcK k{c, P.nml.tice, P.t_wfr, P.nml.ql_mlt, P.tice0, hv, rh_adj, rh_rain,
-                {r.z[Z_DEN][s], r.z[Z_DENFAC][s], r.z[Z_VTG][s], r.z[Z_VTR][s], r.z[Z_VTS][s], p1}};
+                {sh.cell[CL_DEN][id], sh.cell[CL_DENFAC][id], rvtg[id], rvtr[id], rvts[id], p1}};
     double yb[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) yb[q] = G.xb.v[q][s];
+    for (int q = 0; q < 8; ++q) yb[q] = xs[q][s];
     IcInB ib{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    icloud_vjp(k,
-               Cell{r.z[Z_QV][s], r.z[Z_QL][s], r.z[Z_QR][s], r.z[Z_QI][s], r.z[Z_QS][s], r.z[Z_QG][s],
-                    r.z[Z_TZ][s], 0.0},
-               yb, ib);
+    icloud_vjp(k, Cell{rq[0][id], rq[1][id], rq[2][id], rq[3][id], rq[4][id], rq[5][id], rq[6][id], 0.0}, yb, ib);
 #pragma unroll
-    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = yb[q];
-    abar(A, AB_DEN, s) += ib.den;
-    abar(A, AB_DENFAC, s) += ib.denfac;
-    abar(A, AB_VTG, s) += ib.vtg;
-    abar(A, AB_VTR, s) += ib.vtr;
-    abar(A, AB_VTS, s) += ib.vts;
+    for (int q = 0; q < 7; ++q) xs[q][s] = yb[q];
+    A.bar[AB_DEN][id] += ib.den;
+    A.bar[AB_DENFAC][id] += ib.denfac;
+    A.bar[AB_VTG][id] += ib.vtg;
+    A.bar[AB_VTR][id] += ib.vtr;
+    A.bar[AB_VTS][id] += ib.vts;
     G.gp[AP_QI_GEN] += ib.qi_gen;
     // p1 = -dp1 / (g dz0) rdgas t0
-    abar(A, AB_DP1, s) += ib.p1 * p1 / dp1[s];
-    abar(A, AB_DZ0, s) -= ib.p1 * p1 / dz0;
-    abar(A, AB_T0, s) += ib.p1 * p1 / t0;
+    A.bar[AB_DP1][id] += ib.p1 * p1 / dp1;
+    A.bar[AB_DZ0][id] -= ib.p1 * p1 / dz0;
+    A.bar[AB_T0][id] += ib.p1 * p1 / t0;
   });
+  __syncthreads();
+  put_cells(xs[0], sh.buf); put_cells(xs[1], b1); put_cells(xs[2], b2); put_cells(xs[3], b3);
+  __syncthreads();
+  get_cells(G.xb.v[0], sh.buf, pos); get_cells(G.xb.v[1], b1, pos); get_cells(G.xb.v[2], b2, pos);
+  get_cells(G.xb.v[3], b3, pos);
+  __syncthreads();
+  put_cells(xs[4], sh.buf); put_cells(xs[5], b1); put_cells(xs[6], b2);
+  __syncthreads();
+  get_cells(G.xb.v[4], sh.buf, pos); get_cells(G.xb.v[5], b1, pos); get_cells(G.xb.v[6], b2, pos);
+  __syncthreads();
 }
 
 
@@ -1241,7 +1288,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     // ------------------------------------------------ sub-steps, reversed
     for (int it = P.ntimes - 1; it >= 0; --it) {
       aux_load(T, it, sh);
-      icloud_adj(L, A, G, Gl, f, P, c, T, it, X.dp1);
+      icloud_adj(L, A, G, Gl, f, P, c, T, it);
       __syncthreads();
       warm_rain_adj(L, A, G, Gl, f, P, c, false, T, it, RW2_B, X.dp1);
       __syncthreads();
