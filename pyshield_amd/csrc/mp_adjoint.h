@@ -939,6 +939,57 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
   }
 }
 
+// Cotangents of the call's inputs that are held in registers, cell order,
+// until fields_init's reverse stores them.
+template <int S>
+struct InBar {
+  double pt[S], q[6][S], dp0[S], ua[S], va[S], qa[S];
+};
+
+// fields_update's outputs at one cell, read from the call's inputs and
+// output cotangents
+struct FuIn {
+  double dp0, dp1, pt0, gt, ga, qin[6], gq[6];
+};
+// cotangents fields_update's reverse gives the call's inputs at one cell
+struct FuB {
+  double pt, dp0, dp1, qa, q[6];
+};
+// Transpose of fields_update (microphysics.py:1452-1553) at one cell: the
+// tendencies of the final cell x, (x q - q_in dp0 / dp1) dp1 / dp0 / dt for
+// the species, (x tz - pt) cvm(x) / cp_air / dt for temperature, and the
+// cloud-fraction accumulator's tendency.  Adds to yb (the cotangents of x)
+// and to b.
+__device__ __forceinline__ void fields_update_adj(const shield_mp_params& P, double rdt, const Cell& x, const FuIn& in,
+                                                  double (&yb)[8], FuB& b) {
+  const double omq0 = in.dp0 / in.dp1, omq = in.dp1 / in.dp0;
+  const double xq[6] = {x.qv, x.ql, x.qr, x.qi, x.qs, x.qg};
+  double omqb = 0.0, omq0b = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    yb[q] += in.gq[q] * rdt * omq;
+    b.q[q] -= in.gq[q] * rdt * omq0 * omq;
+    omqb += in.gq[q] * rdt * (xq[q] - in.qin[q] * omq0);
+    omq0b -= in.gq[q] * rdt * in.qin[q] * omq;
+  }
+  b.dp1 += omqb / in.dp0 - omq0b * omq0 / in.dp1;
+  b.dp0 += omq0b / in.dp1 - omqb * omq / in.dp0;
+  const double cvm = P.c_air + x.qv * P.c_vap + (x.qr + x.ql) * C_LIQ + (x.qi + x.qs + x.qg) * C_ICE;
+  yb[6] += in.gt * rdt * cvm / CP_AIR;
+  b.pt -= in.gt * rdt * cvm / CP_AIR;
+  const double cvmb = in.gt * rdt * (x.tz - in.pt0) / CP_AIR;
+  yb[0] += cvmb * P.c_vap;
+  yb[1] += cvmb * C_LIQ;
+  yb[2] += cvmb * C_LIQ;
+  yb[3] += cvmb * C_ICE;
+  yb[4] += cvmb * C_ICE;
+  yb[5] += cvmb * C_ICE;
+  if (!P.nml.do_qa) {
+    yb[7] += in.ga * rdt / P.ntimes;
+    b.qa -= in.ga * rdt;
+  }
+}
+
 // The reverse of icloud (record RI): melt / freeze, then icloud_main with
 // subgrid_z_proc, on the cells sorted by condensate as the forward's icloud
 // sorts them (some condensate above QRMIN, above QCMIN, none), so a slot of
@@ -949,10 +1000,16 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
 // sorted cell are read at its cell index: the record and fall speeds from the
 // tape, den and denfac from LDS, t0, dz0 and dp1 = delp (1 - qv) from the
 // call's inputs.
-template <int S, class FS>
+//
+// In the last sub-step (`last`) the replay has not run icloud: its forward
+// sweep here forms the call's final cell, and fields_update's reverse runs on
+// it at once (fields_update_adj), giving the cotangents of icloud's outputs
+// and adding the input cotangents of fields_update to nb.
+template <int S, class FS, class FA>
 __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
-                                           const FS& f, const shield_mp_params& P,
-                                           const Coef<double>& c, const TapeView<S>& T, int it) {
+                                           const FS& f, const FA& a, const shield_mp_params& P,
+                                           const Coef<double>& c, const TapeView<S>& T, int it, bool last,
+                                           InBar<S>& nb) {
   Shared<S>& sh = A.sh;
   const int lane = lane_id();
   int pos[S], src[S], osrc[S];
@@ -981,15 +1038,32 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
   double* const b3 = sh.cell[CL_TMP];
   // cotangents of the cell after icloud in sorted order (qa's is read by the
   // cloud fraction and passes through icloud unchanged)
+  // (zero in the last sub-step, where fields_update's reverse forms them)
   double xs[8][S];
-  put_cells(G.xb.v[0], sh.buf); put_cells(G.xb.v[1], b1); put_cells(G.xb.v[2], b2); put_cells(G.xb.v[3], b3);
-  __syncthreads();
-  get_cells(xs[0], sh.buf, src); get_cells(xs[1], b1, src); get_cells(xs[2], b2, src); get_cells(xs[3], b3, src);
-  __syncthreads();
-  put_cells(G.xb.v[4], sh.buf); put_cells(G.xb.v[5], b1); put_cells(G.xb.v[6], b2); put_cells(G.xb.v[7], b3);
-  __syncthreads();
-  get_cells(xs[4], sh.buf, src); get_cells(xs[5], b1, src); get_cells(xs[6], b2, src); get_cells(xs[7], b3, src);
-  __syncthreads();
+  // fields_update's input cotangents (last sub-step), sorted order
+  double fpt[S], fdp0[S], fqa[S], fq[6][S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    fpt[s] = fdp0[s] = fqa[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) fq[q][s] = 0.0;
+  }
+  if (last) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) xs[q][s] = 0.0;
+  } else {
+    put_cells(G.xb.v[0], sh.buf); put_cells(G.xb.v[1], b1); put_cells(G.xb.v[2], b2); put_cells(G.xb.v[3], b3);
+    __syncthreads();
+    get_cells(xs[0], sh.buf, src); get_cells(xs[1], b1, src); get_cells(xs[2], b2, src); get_cells(xs[3], b3, src);
+    __syncthreads();
+    put_cells(G.xb.v[4], sh.buf); put_cells(G.xb.v[5], b1); put_cells(G.xb.v[6], b2); put_cells(G.xb.v[7], b3);
+    __syncthreads();
+    get_cells(xs[4], sh.buf, src); get_cells(xs[5], b1, src); get_cells(xs[6], b2, src); get_cells(xs[7], b3, src);
+    __syncthreads();
+  }
+  const double rdt = P.rdt;
   const int ncell = L.ncols * L.nz;
   // column of cell id: floor((id + 1/2) / nz) in FP32 is exact here, the
   // quotient stays >= 1/(2 nz) away from an integer
@@ -1015,7 +1089,26 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
 #pragma unroll
     for (int q = 0; q < 8; ++q) yb[q] = xs[q][s];
     IcInB ib{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    icloud_vjp(k, Cell{rq[0][id], rq[1][id], rq[2][id], rq[3][id], rq[4][id], rq[5][id], rq[6][id], 0.0}, yb, ib);
+    const Cell x0{rq[0][id], rq[1][id], rq[2][id], rq[3][id], rq[4][id], rq[5][id], rq[6][id], 0.0};
+    // one instance of icloud_vjp for both cases (the sweep forms the final
+    // cell in every sub-step; only the last one uses it)
+    FuB fb{0.0, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+    icloud_vjp<true>(k, x0, yb, ib, nullptr, [&](const Cell& x1, double (&y)[8]) {
+      if (!last) return;
+      const FuIn in{f.delp[o], dp1, t0, a.pt_dt[o], a.qa_dt[o],
+                    {f.qv[o], f.ql[o], f.qr[o], f.qi[o], f.qs[o], f.qg[o]},
+                    {a.qv_dt[o], a.ql_dt[o], a.qr_dt[o], a.qi_dt[o], a.qs_dt[o], a.qg_dt[o]}};
+      fields_update_adj(P, rdt, x1, in, y, fb);
+    });
+    if (last) {
+      fpt[s] = fb.pt;
+      fdp0[s] = fb.dp0;
+      fqa[s] = fb.qa;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) fq[q][s] = fb.q[q];
+      A.bar[AB_DP1][id] += fb.dp1;
+      xs[7][s] = yb[7];
+    }
 #pragma unroll
     for (int q = 0; q < 7; ++q) xs[q][s] = yb[q];
     A.bar[AB_DEN][id] += ib.den;
@@ -1039,6 +1132,25 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
   __syncthreads();
   get_cells(G.xb.v[4], sh.buf, pos); get_cells(G.xb.v[5], b1, pos); get_cells(G.xb.v[6], b2, pos);
   __syncthreads();
+  if (last) {
+    // back to cell order: qa's cotangent and fields_update's input cotangents
+    double t[S];
+    put_cells(xs[7], sh.buf); put_cells(fpt, b1); put_cells(fdp0, b2); put_cells(fqa, b3);
+    __syncthreads();
+    get_cells(G.xb.v[7], sh.buf, pos); get_cells(nb.pt, b1, pos); get_cells(t, b2, pos); get_cells(nb.qa, b3, pos);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; ++s) nb.dp0[s] += t[s];
+    put_cells(fq[0], sh.buf); put_cells(fq[1], b1); put_cells(fq[2], b2); put_cells(fq[3], b3);
+    __syncthreads();
+    get_cells(nb.q[0], sh.buf, pos); get_cells(nb.q[1], b1, pos); get_cells(nb.q[2], b2, pos);
+    get_cells(nb.q[3], b3, pos);
+    __syncthreads();
+    put_cells(fq[4], sh.buf); put_cells(fq[5], b1);
+    __syncthreads();
+    get_cells(nb.q[4], sh.buf, pos); get_cells(nb.q[5], b1, pos);
+    __syncthreads();
+  }
 }
 
 
@@ -1162,7 +1274,8 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
       sed_rec(L, X, sh, Gl, f, P, c, T, it);
       warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B);
       tape_save(T, it, RI, X, sh);
-      icloud(L, X, sh, Gl, f, P, c);
+      // the last sub-step's icloud runs in its reverse (icloud_adj)
+      if (it + 1 < P.ntimes) icloud(L, X, sh, Gl, f, P, c);
     }
     __syncthreads();
     // ----------------------------------------------------- reverse walk
@@ -1184,48 +1297,15 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
       A.colbar[CB_GRAUPEL][lane] = a.graupel ? a.graupel[o2] * convt : 0.0;
       A.colbar[CB_CCN][lane] = A.colbar[CB_CPRAUT][lane] = 0.0;
     }
-    double ptb[S], qinb[6][S], dp0b[S], uab[S], vab[S], qab[S];
-    // fields_update: tendencies of the final state
+    InBar<S> nb;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      ptb[s] = dp0b[s] = uab[s] = vab[s] = qab[s] = 0.0;
+      nb.pt[s] = nb.dp0[s] = nb.ua[s] = nb.va[s] = nb.qa[s] = 0.0;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) qinb[q][s] = 0.0;
-      if (!L.v[s]) continue;
-      const int32_t o = Gl.off[s];
-      const double dp0 = f.delp[o], dp1 = X.dp1[s];
-      const double omq0 = dp0 / dp1, omq = dp1 / dp0;
-      const double xq[6] = {X.qv[s], X.ql[s], X.qr[s], X.qi[s], X.qs[s], X.qg[s]};
-      const double qin[6] = {f.qv[o], f.ql[o], f.qr[o], f.qi[o], f.qs[o], f.qg[o]};
-      const double gq[6] = {a.qv_dt[o], a.ql_dt[o], a.qr_dt[o], a.qi_dt[o], a.qs_dt[o], a.qg_dt[o]};
-      double omqb = 0.0, omq0b = 0.0;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        G.xb.v[q][s] += gq[q] * rdt * omq;
-        qinb[q][s] -= gq[q] * rdt * omq0 * omq;
-        omqb += gq[q] * rdt * (xq[q] - qin[q] * omq0);
-        omq0b -= gq[q] * rdt * qin[q] * omq;
-      }
-      abar(A, AB_DP1, s) += omqb / dp0 - omq0b * omq0 / dp1;
-      dp0b[s] += omq0b / dp1 - omqb * omq / dp0;
-      const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
-                         (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
-      const double gt = a.pt_dt[o];
-      G.xb.v[6][s] += gt * rdt * cvm / CP_AIR;
-      ptb[s] -= gt * rdt * cvm / CP_AIR;
-      const double cvmb = gt * rdt * (X.tz[s] - f.pt[o]) / CP_AIR;
-      G.xb.v[0][s] += cvmb * P.c_vap;
-      G.xb.v[1][s] += cvmb * C_LIQ;
-      G.xb.v[2][s] += cvmb * C_LIQ;
-      G.xb.v[3][s] += cvmb * C_ICE;
-      G.xb.v[4][s] += cvmb * C_ICE;
-      G.xb.v[5][s] += cvmb * C_ICE;
-      if (!n.do_qa) {
-        const double ga = a.qa_dt[o];
-        G.xb.v[7][s] += ga * rdt / P.ntimes;
-        qab[s] -= ga * rdt;
-      }
+      for (int q = 0; q < 6; ++q) nb.q[q][s] = 0.0;
     }
+    // fields_update's reverse runs in the last sub-step's icloud reverse,
+    // except the momentum transport's
     if (n.sedi_transport) {
       double m1[S], mp[S], al[S], bu[S], bv[S], u1[S], v1[S], an[S], gu[S], gv[S], nu[S], nv[S];
 #pragma unroll
@@ -1248,8 +1328,8 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
         }
         gu[s] = (L.v[s] && L.k[s] > 0) ? a.udt[o] * rdt : 0.0;
         gv[s] = (L.v[s] && L.k[s] > 0) ? a.vdt[o] * rdt : 0.0;
-        uab[s] -= gu[s];
-        vab[s] -= gv[s];
+        nb.ua[s] -= gu[s];
+        nb.va[s] -= gv[s];
       }
       scan_fwd(L, al, bu, u1);
       scan_fwd(L, al, bv, v1);
@@ -1268,14 +1348,14 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
         if (!L.v[s]) continue;
         const int32_t o = Gl.off[s];
         if (L.k[s] == 0) {
-          uab[s] += nu[s];
-          vab[s] += nv[s];
+          nb.ua[s] += nu[s];
+          nb.va[s] += nv[s];
         } else {
           const double dp0 = f.delp[o], u0 = f.ua[o], v0 = f.va[o];
           const double r = 1.0 / (dp0 + mp[s]);
-          uab[s] += nu[s] * dp0 * r;
-          vab[s] += nv[s] * dp0 * r;
-          dp0b[s] += (nu[s] * (u0 - u1[s]) + nv[s] * (v0 - v1[s])) * r;
+          nb.ua[s] += nu[s] * dp0 * r;
+          nb.va[s] += nv[s] * dp0 * r;
+          nb.dp0[s] += (nu[s] * (u0 - u1[s]) + nv[s] * (v0 - v1[s])) * r;
           mb[s] = (nu[s] * (u1p[s] - u1[s]) + nv[s] * (v1p[s] - v1[s])) * r;  // lands on level k-1
         }
       }
@@ -1288,7 +1368,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     // ------------------------------------------------ sub-steps, reversed
     for (int it = P.ntimes - 1; it >= 0; --it) {
       aux_load(T, it, sh);
-      icloud_adj(L, A, G, Gl, f, P, c, T, it);
+      icloud_adj(L, A, G, Gl, f, a, P, c, T, it, it == P.ntimes - 1, nb);
       __syncthreads();
       warm_rain_adj(L, A, G, Gl, f, P, c, false, T, it, RW2_B, X.dp1);
       __syncthreads();
@@ -1311,34 +1391,34 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
       double omqb = 0.0;
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        qinb[q][s] += G.xb.v[q][s] * omq;
+        nb.q[q][s] += G.xb.v[q][s] * omq;
         omqb += G.xb.v[q][s] * qin[q];
       }
-      ptb[s] += G.xb.v[6][s] + abar(A, AB_T0, s);
+      nb.pt[s] += G.xb.v[6][s] + abar(A, AB_T0, s);
       double dp1b = abar(A, AB_DP1, s) - omqb * omq / dp1;
-      dp0b[s] += omqb / dp1;
+      nb.dp0[s] += omqb / dp1;
       double dz0b = abar(A, AB_DZ0, s);
       if (L.kb[s] == 0 && P.use_ccn) {
         // ccn = ccn0 rdgas tz / p1, p1 = -dp1 / (g dz0) rdgas t0, with tz = t0 = pt
-        ptb[s] += ccnbar * ccn / f.pt[o];
+        nb.pt[s] += ccnbar * ccn / f.pt[o];
         const double p1b = -ccnbar * ccn / p1_b[s];
         dp1b += p1b * p1_b[s] / dp1;
         dz0b -= p1b * p1_b[s] / f.delz[o];
-        ptb[s] += p1b * p1_b[s] / f.pt[o];
+        nb.pt[s] += p1b * p1_b[s] / f.pt[o];
       }
-      dp0b[s] += dp1b * (1.0 - qv);
-      qinb[0][s] -= dp1b * dp0;
-      a.pt[o] = ptb[s];
-      a.qv[o] = qinb[0][s];
-      a.ql[o] = qinb[1][s];
-      a.qr[o] = qinb[2][s];
-      a.qi[o] = qinb[3][s];
-      a.qs[o] = qinb[4][s];
-      a.qg[o] = qinb[5][s];
-      a.qa[o] = qab[s];
-      a.ua[o] = uab[s];
-      a.va[o] = vab[s];
-      a.delp[o] = dp0b[s];
+      nb.dp0[s] += dp1b * (1.0 - qv);
+      nb.q[0][s] -= dp1b * dp0;
+      a.pt[o] = nb.pt[s];
+      a.qv[o] = nb.q[0][s];
+      a.ql[o] = nb.q[1][s];
+      a.qr[o] = nb.q[2][s];
+      a.qi[o] = nb.q[3][s];
+      a.qs[o] = nb.q[4][s];
+      a.qg[o] = nb.q[5][s];
+      a.qa[o] = nb.qa[s];
+      a.ua[o] = nb.ua[s];
+      a.va[o] = nb.va[s];
+      a.delp[o] = nb.dp0[s];
       a.delz[o] = dz0b;
       if (a.qaz) a.qaz[o] = n.reset_qaz ? 0.0 : G.xb.v[7][s];
     }

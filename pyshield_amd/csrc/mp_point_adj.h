@@ -1606,8 +1606,16 @@ SH_FN double blk_cf_rev(const IcK& k, const IcX& x, double icpk7, IcX& xb, IcInB
 // Vector-Jacobian product of icloud for one cell.  s: the cell at icloud's
 // entry; yb: in, the cotangents of the cell after icloud (qv, ql, qr, qi, qs,
 // qg, tz, qa); out, those of the cell before.  ib accumulates the input
-// cotangents.  out (optional) receives the cell after icloud.
-SH_FN void icloud_vjp(const IcK& k, const Cell& s, double (&yb)[8], IcInB& ib, Cell* out = nullptr) {
+// cotangents.  out (optional) receives the cell after icloud.  With MID, once
+// the forward sweep has formed the cell after icloud, mid(cell, yb) is called
+// before the reverse sweep reads yb (so the cotangents of icloud's outputs can
+// be formed from its outputs in the same pass).
+struct IcNoMid {
+  SH_FN void operator()(const Cell&, double (&)[8]) const {}
+};
+template <bool MID = false, class Mid = IcNoMid>
+SH_FN void icloud_vjp(const IcK& k, const Cell& s, double (&yb)[8], IcInB& ib, Cell* out = nullptr,
+                      const Mid& mid = Mid()) {
   const Coef<double>& c = k.c;
   const IcX x0{s.qv, s.ql, s.qr, s.qi, s.qs, s.qg, s.tz, s.qa, 0.0, 0.0, 0.0};
   IcX a = x0;
@@ -1647,10 +1655,12 @@ SH_FN void icloud_vjp(const IcK& k, const Cell& s, double (&yb)[8], IcInB& ib, C
   if (run) blk_gdep(k, i6);
   i7 = i6;
   if (run) blk_revp(k, i7);
-  if (out) {
+  if (MID || out) {
     IcX o = run ? i7 : t;
     if (run) blk_cf(k, o, icpk7);
-    *out = Cell{o.qv, o.ql, o.qr, o.qi, o.qs, o.qg, o.tz, o.qa};
+    const Cell oc{o.qv, o.ql, o.qr, o.qi, o.qs, o.qg, o.tz, o.qa};
+    if (out) *out = oc;
+    if constexpr (MID) mid(oc, yb);
   }
   // reverse
   IcX xb{yb[0], yb[1], yb[2], yb[3], yb[4], yb[5], yb[6], yb[7], 0.0, 0.0, 0.0};
