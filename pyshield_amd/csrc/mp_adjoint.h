@@ -588,10 +588,55 @@ __device__ __forceinline__ double& abar(AdjShared<S>& A, int which, int s) {
 }
 
 // Evaporation / accretion of rain (revap_racc) at record r, in falling
-// columns (hand-written reverse, mp_point_adj.h).
+// columns (hand-written reverse, mp_point_adj.h).  It changes only cells
+// warmer than t_wfr with rain; as in the forward, those are packed into one
+// slot when there are at most 64 of them.  qi, qs and qg enter through their
+// sum, packed in qi's place, and each gains the cotangent the sum gets (the
+// reverse adds it to each once), so the result is that of cell order.
 template <int S>
 __device__ __forceinline__ void revap_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Rec<S>& r,
                                           const bool (&fall)[S], const Coef<double>& c, double t_wfr, double dt5) {
+  bool pred[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    pred[s] = fall[s] && L.v[s] && (r.z[Z_TZ][s] > t_wfr) && (r.z[Z_QR][s] > QRMIN);
+  const Packed<S> pk = pack_cells(pred);
+  if (pk.n <= 64) {
+    if (pk.n == 0) return;
+    double q_sol[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) q_sol[s] = r.z[Z_QI][s] + r.z[Z_QS][s] + r.z[Z_QG][s];
+    const Cell cs{gather_packed(r.z[Z_QV], pk.src), gather_packed(r.z[Z_QL], pk.src),
+                  gather_packed(r.z[Z_QR], pk.src), gather_packed(q_sol, pk.src), 0.0, 0.0,
+                  gather_packed(r.z[Z_TZ], pk.src), 0.0};
+    IcX xb{gather_packed(G.xb.v[0], pk.src), gather_packed(G.xb.v[1], pk.src), gather_packed(G.xb.v[2], pk.src),
+           0.0, 0.0, 0.0, gather_packed(G.xb.v[6], pk.src), 0.0, 0.0, 0.0, 0.0};
+    if (lane_id() < pk.n) {
+      const int id = pk.src;
+      // column of cell id: floor((id + 1/2) / nz) in FP32 is exact here
+      const int cl = int((float(id) + 0.5f) * (1.0f / float(L.nz)));
+      double denb = 0.0, denfacb = 0.0;
+      revap_vjp(c, t_wfr, dt5, A.sh.colv[CV_HVAR][cl], cs, A.sh.cell[CL_DEN][id], A.sh.cell[CL_DENFAC][id], xb,
+                denb, denfacb);
+      A.bar[AB_DEN][id] += denb;
+      A.bar[AB_DENFAC][id] += denfacb;
+    }
+    scatter_packed(G.xb.v[0], xb.qv, pk, pred);
+    scatter_packed(G.xb.v[1], xb.ql, pk, pred);
+    scatter_packed(G.xb.v[2], xb.qr, pk, pred);
+    scatter_packed(G.xb.v[6], xb.tz, pk, pred);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double t = pull(xb.qi, pk.pos[s]);
+      if (pred[s]) {
+        G.xb.v[3][s] += t;
+        G.xb.v[4][s] += t;
+        G.xb.v[5][s] += t;
+      }
+    }
+    __syncthreads();
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (!(fall[s] && L.v[s])) continue;
