@@ -1843,11 +1843,11 @@ SH_FN void autoconv_vjp(const Coef<double>& c, double t_wfr, bool subgrid, doubl
   const double qc0 = c.fac_rc * ccn;
   const double qc = c.use_ccn ? qc0 : qc0 / den;
   double qcb = 0.0;
-  const double pe = pw_el<double>(ql, 7.0 / 3.0);
   if (!subgrid) {
     if (!(tz > t_wfr)) return;
     const double dq = ql - qc;
     if (!(dq > 0.0)) return;
+    const double pe = pw_el<double>(ql, 7.0 / 3.0);
     const double a2 = c.dt_rain * c_praut * den * pe;
     const double sinkb = qrb - qlb;
     if (dq < a2) {
@@ -1864,6 +1864,7 @@ SH_FN void autoconv_vjp(const Coef<double>& c, double t_wfr, bool subgrid, doubl
     const double dl1 = mmin(mx, 0.5 * ql);
     const double dq = 0.5 * (ql + dl1 - qc);
     if (!(dq > 0.0)) return;
+    const double pe = pw_el<double>(ql, 7.0 / 3.0);
     const double r = dq / dl1;
     const double m = mmin(1.0, r);
     const double sinkb = qrb - qlb;
