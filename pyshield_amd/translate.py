@@ -22,10 +22,9 @@ through NDSL's translate-test framework:
   thresholds per backend).
 
 Here the same test runs on ``pyshield_amd.Microphysics`` (the HIP kernels, FP64
-by default).  The savepoint containers this image can read are loaded by
-``load_savepoint``: ``.npz`` archives and classic (version 1/2) netCDF files.
-NDSL writes netCDF-4 (HDF5) savepoints, which need ``nccopy -k classic`` (or
-any netCDF-4 -> netCDF-3 conversion) first: no HDF5 reader is available here.
+by default).  ``load_savepoint`` reads the savepoint containers: NDSL's
+netCDF-4 files (HDF5, read by ``pyshield_amd.hdf5``), classic (version 1/2)
+netCDF files and ``.npz`` archives.
 """
 
 import os
@@ -131,29 +130,49 @@ def compare(computed, reference, max_error=MICROPH_MAX_ERROR, near_zero=None):
     return report
 
 
+_SAVEPOINT_DIMS = (("savepoint", "n_savepoints"), ("rank", "n_ranks"))
+# the NAME netCDF-4 gives a dimension that has no coordinate variable
+_NC_DIM_ONLY = "This is a netCDF dimension but not a netCDF variable."
+
+
+def _select(a, dims, savepoint, rank):
+    if dims and len(dims) >= 2 and dims[0] in _SAVEPOINT_DIMS[0] and dims[1] in _SAVEPOINT_DIMS[1]:
+        return a[savepoint, rank]
+    return a
+
+
 def load_savepoint(path, savepoint=0, rank=0):
     """Variables of one savepoint as numpy arrays.
 
-    ``.npz``: the arrays as stored.  netCDF (classic format): each variable,
-    with NDSL's leading (savepoint, rank) dimensions indexed by ``savepoint``
-    and ``rank`` when present (a variable whose first two dimensions are
-    named ``savepoint`` and ``rank``, or ``n_savepoints`` / ``n_ranks``)."""
+    ``.npz``: the arrays as stored.  netCDF, netCDF-4 (HDF5) or classic
+    format: each variable, with NDSL's leading (savepoint, rank) dimensions
+    indexed by ``savepoint`` and ``rank`` when present (a variable whose first
+    two dimensions are named ``savepoint`` and ``rank``, or ``n_savepoints``
+    / ``n_ranks``); netCDF-4's dimension-only datasets are not variables."""
     ext = os.path.splitext(path)[1].lower()
     if ext == ".npz":
         with np.load(path, allow_pickle=False) as z:
             return {k: np.array(z[k]) for k in z.files}
+    from pyshield_amd import hdf5
+    if ext in (".nc", ".nc4", ".h5", ".hdf5") and hdf5.is_hdf5(path):
+        out = {}
+        for name, var in hdf5.read(path).items():
+            label = var.attrs.get("NAME")
+            if isinstance(label, (bytes, np.bytes_)):
+                label = label.decode("utf-8", "replace")
+            if var.attrs.get("CLASS") in ("DIMENSION_SCALE", b"DIMENSION_SCALE") and isinstance(label, str) \
+                    and label.startswith(_NC_DIM_ONLY):
+                continue
+            out[name] = np.array(_select(var.data, var.dims, savepoint, rank))
+        return out
     if ext in (".nc", ".nc3", ".cdf"):
         from scipy.io import netcdf_file
         out = {}
         with netcdf_file(path, "r", mmap=False) as f:
             for name, var in f.variables.items():
-                a = np.array(var[:])
-                dims = tuple(var.dimensions)
-                if len(dims) >= 2 and dims[0] in ("savepoint", "n_savepoints") and dims[1] in ("rank", "n_ranks"):
-                    a = a[savepoint, rank]
-                out[name] = a
+                out[name] = np.array(_select(np.array(var[:]), tuple(var.dimensions), savepoint, rank))
         return out
-    raise ValueError(f"unsupported savepoint container {path!r} (.npz or classic netCDF)")
+    raise ValueError(f"unsupported savepoint container {path!r} (netCDF-4/HDF5, classic netCDF or .npz)")
 
 
 class TranslateMicroph:
