@@ -48,7 +48,12 @@ def _compile(src, verbose):
 
 
 def build(verbose=False, jobs=None):
-    """Compile every HIP source and link libshield_mp.so.  Returns its path."""
+    """Compile every HIP source and link libshield_mp.so.  Returns its path.
+    The mixed kernel's saturation table (csrc/qs_table.h) is regenerated
+    first from the constants it is built from."""
+    r = subprocess.run([sys.executable, os.path.join(CSRC, "gen_qs_table.py")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gen_qs_table.py failed:\n{r.stdout}\n{r.stderr}")
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), os.cpu_count() or 1)

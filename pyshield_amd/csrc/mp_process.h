@@ -26,6 +26,7 @@
 #pragma once
 
 #include "mp_dual.h"
+#include "qs_table.h"
 
 namespace shield {
 
@@ -146,16 +147,38 @@ SH_FN R iqs2(R ta, R den, R& dqdt) {
 }
 
 // Saturation mixing ratio and gradient in the mixed-precision kernel: the
-// closed forms above, evaluated in FP64 with the divisions by constants
-// folded into products, the others replaced by reciprocals
-// ((ta - TICE) / (ta TICE) = 1/TICE - 1/ta), and the series exp and log of
-// mp_common.h (~1e-14 relative).  Below T_SAT_MIN and above
-// TICE + 102 the reference's branches are the same expression at the clamped
-// temperature.
-SH_FN double qs_fast(double t, double d, double l, double den, double& dqdt) {
-  constexpr double rtice = 1.0 / TICE, rrv = 1.0 / RVGAS;
+// closed forms above in FP64, with e_sat(T) read from the FP64-built table of
+// qs_table.h (a degree-7 polynomial on each of 128 intervals of u = 1/T,
+// < 2e-12 relative, 16 KB; four 16-byte loads and seven multiply-adds in place
+// of an exp and a log; a degree-11 table of 32 intervals, 6 KB, measured
+// slower), the divisions by constants folded into products and the
+// others replaced by reciprocals.  Below T_SAT_MIN and above TICE + 102 the
+// reference's ice branches are the same expression at the clamped
+// temperature.  Over water the table spans 100-1000 K (the reference's own
+// cells reach 460 K where its autoconversion overdraws cloud water) and a
+// temperature outside is read at the nearest end: below 100 K the water
+// saturation ratio is under 1e-15 kg/kg either way.  (A closed-form fallback
+// for the cells outside, inlined at every call site, made the kernel 4%
+// slower through its code size.)
+//
+// e_sat at u = rt = 1/t from table `ice` (0 water: d = DC_VAP, l = LV0;
+// 1 ice: d = D2ICE, l = LI2)
+SH_FN double esat_table(double rt, int ice) {
+  double x = (rt - QS_TAB_U0[ice]) * QS_TAB_RN[ice];
+  x = x < 0.0 ? 0.0 : x > double(QS_TAB_N) ? double(QS_TAB_N) : x;
+  const int i = x < double(QS_TAB_N - 1) ? int(x) : QS_TAB_N - 1;
+  const double s = 2.0 * (x - double(i)) - 1.0;
+  const double* c = QS_TAB[ice][i];
+  double p = c[QS_TAB_DEG];
+#pragma unroll
+  for (int j = QS_TAB_DEG - 1; j >= 0; --j) p = fma(p, s, c[j]);
+  return p;
+}
+SH_FN double qs_fast(double t, int ice, double den, double& dqdt) {
+  constexpr double rrv = 1.0 / RVGAS;
+  const double d = ice ? D2ICE : DC_VAP, l = ice ? LI2 : LV0;
   const double rt = rcp64(t);
-  const double e = E00 * exp_sat(d * rrv * log_sat(t * rtice) + l * rrv * (rtice - rt));
+  const double e = esat_table(rt, ice);
   const double q = e * rrv * rt * rcp64(den);
   dqdt = q * (d + l * rt) * rrv * rt;
   return q;
@@ -163,27 +186,27 @@ SH_FN double qs_fast(double t, double d, double l, double den, double& dqdt) {
 SH_FN double iqs_fast(double ta, double den, double& dqdt) {
   const bool ice = ta < TICE;
   const double t = ice ? (ta < T_SAT_MIN ? T_SAT_MIN : ta) : (ta > TICE + 102.0 ? TICE + 102.0 : ta);
-  return qs_fast(t, ice ? D2ICE : DC_VAP, ice ? LI2 : LV0, den, dqdt);
+  return qs_fast(t, ice ? 1 : 0, den, dqdt);
 }
-// qs_fast(ta, DC_VAP, LV0, ...) where `water`, else iqs_fast(ta, ...), as one
-// evaluation with the constants and the clamp chosen per cell
+// qs_fast(ta, water table, ...) where `water`, else iqs_fast(ta, ...), as one
+// evaluation with the table and the clamp chosen per cell
 SH_FN double qs_pick(double ta, double den, bool water) {
   const bool ice = !water && ta < TICE;
   const double t = water ? ta : ice ? (ta < T_SAT_MIN ? T_SAT_MIN : ta) : (ta > TICE + 102.0 ? TICE + 102.0 : ta);
   double g;
-  return qs_fast(t, ice ? D2ICE : DC_VAP, ice ? LI2 : LV0, den, g);
+  return qs_fast(t, ice ? 1 : 0, den, g);
 }
 // the reference's forms in FP64 mode (and for the adjoint's dual numbers),
 // the folded ones in mixed mode
 template <class R>
 SH_FN wide_t<R> qs_w(wide_t<R> ta, wide_t<R> den, wide_t<R>& dqdt) {
-  if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, dqdt);
+  if constexpr (is_f32<R>::value) return qs_fast(ta, 0, den, dqdt);
   else return wqs2<wide_t<R>>(ta, den, dqdt);
 }
 template <class R>
 SH_FN wide_t<R> qs_w1(wide_t<R> ta, wide_t<R> den) {
   wide_t<R> g;
-  if constexpr (is_f32<R>::value) return qs_fast(ta, DC_VAP, LV0, den, g);
+  if constexpr (is_f32<R>::value) return qs_fast(ta, 0, den, g);
   else return wqs1<wide_t<R>>(ta, den);
 }
 template <class R>
@@ -594,7 +617,7 @@ SH_FN void subgrid_z_proc(const Coef<R>& c, double tice_d, CellR<R>& s, R h_var,
       qstar = qa;
       if (!cold && !warm) {
         W g;
-        const W qsw3 = qs_fast(tin2, DC_VAP, LV0, den_d, g);
+        const W qsw3 = qs_fast(tin2, 0, den_d, g);
         W rqi;
         if (q_cond > 3.0e-6)
           rqi = fdiv<R>(q_sol2, q_cond);
