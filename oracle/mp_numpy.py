@@ -339,7 +339,10 @@ class Proc:
         q_minus = qpz - dqh
         q_plus = qpz + dqh
         ev = xp.logical_and(act, xp.logical_and(dqv > QVMIN, qsat > q_minus))
-        dqh_s = self.safe(ev, dqh)
+        # the divisor is made safe wherever its branch is not taken (it is zero
+        # where vapour and cloud water are), so autograd's gradient of the
+        # untaken branch stays finite; the values are the reference's
+        dqh_s = self.safe(xp.logical_and(ev, xp.logical_not(qsat > q_plus)), dqh)
         dq = xp.where(qsat > q_plus, qsat - qpz, 0.25 * (q_minus - qsat) ** 2 / dqh_s)
         qden = self.safe(ev, qr * den)
         t2 = tin * tin

@@ -28,8 +28,9 @@
 //     the liquid slope limiter, the momentum transport, and the
 //     precipitation and flux sums.
 // Supported switches: the default ones, plus the const_v*, irain_f,
-// z_slope_liq, do_sedi_heat, sedi_transport, do_qa and reset_qaz variants;
-// do_sedi_w, z_slope_ice, fix_negative and de_ice are rejected by the host.
+// z_slope_liq, do_sedi_heat, sedi_transport, do_qa, reset_qaz and
+// fix_negative variants; do_sedi_w, z_slope_ice and de_ice are rejected by
+// the host.
 #pragma once
 
 #include "mp_forward.h"
@@ -1199,6 +1200,161 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
 }
 
 
+// ------------------------------------------------------------ fix_negative
+// Transpose of fix_negative_cell at one cell (the borrowing chain of
+// fields_init, microphysics.py:201-240).  x0: the cell at its entry; b: in,
+// the cotangents of qv, ql, qr, qi, qs, qg, tz after it; out, before it.  A
+// species cleared to zero passes no cotangent back; a borrowing adds the
+// borrower's cotangent to the lender's; the two heatings pass theirs to the
+// borrowed amount and, through lcpk and icpk, to the entry temperature and
+// heat capacity.
+SH_FN void fix_negative_cell_vjp(const shield_mp_params& P, const Cell& x0, double (&b)[7]) {
+  const double cvm = P.c_air + x0.qv * P.c_vap + (x0.qr + x0.ql) * C_LIQ + (x0.qi + x0.qs + x0.qg) * C_ICE;
+  const double tz0 = x0.tz;
+  // the forward's branches and borrowed amounts
+  const bool c1 = x0.qi < 0.0;
+  const double qs1 = c1 ? x0.qs + x0.qi : x0.qs;
+  const bool c2 = qs1 < 0.0;
+  const double qg2 = c2 ? x0.qg + qs1 : x0.qg;
+  const bool c3 = qg2 < 0.0;
+  const double qr3 = c3 ? x0.qr + qg2 : x0.qr;
+  const bool c4 = qr3 < 0.0;
+  const double ql4 = c4 ? x0.ql + qr3 : x0.ql;
+  const bool c5 = ql4 < 0.0;
+  double cvmb = 0.0, tz0b = 0.0;
+  // cloud water from vapour, with the condensation heat
+  double qv0b = b[0], ql4b = b[1];
+  const double tz3b = b[6];
+  if (c5) {
+    const double lcpk = (P.lv00 + P.d0_vap * tz0) / cvm;
+    ql4b = b[0] - b[6] * lcpk;
+    const double lcpkb = -b[6] * ql4;
+    tz0b += lcpkb * P.d0_vap / cvm;
+    cvmb -= lcpkb * lcpk / cvm;
+  }
+  // rain from cloud water
+  const double ql0b = ql4b, qr3b = c4 ? ql4b : b[2];
+  // graupel from rain, with the fusion heat
+  const double qr0b = qr3b;
+  double qg2b = b[5];
+  if (c3) {
+    const double icpk = (LI00 + DC_ICE * tz0) / cvm;
+    qg2b = qr3b - tz3b * icpk;
+    const double icpkb = -tz3b * qg2;
+    tz0b += icpkb * DC_ICE / cvm;
+    cvmb -= icpkb * icpk / cvm;
+  }
+  tz0b += tz3b;
+  // snow from graupel, ice from snow
+  const double qg0b = qg2b, qs1b = c2 ? qg2b : b[4];
+  const double qs0b = qs1b, qi0b = c1 ? qs1b : b[3];
+  b[0] = qv0b + cvmb * P.c_vap;
+  b[1] = ql0b + cvmb * C_LIQ;
+  b[2] = qr0b + cvmb * C_LIQ;
+  b[3] = qi0b + cvmb * C_ICE;
+  b[4] = qs0b + cvmb * C_ICE;
+  b[5] = qg0b + cvmb * C_ICE;
+  b[6] = tz0b;
+}
+
+// Transpose of fix_negative_column for one column (q: in, the vapour before
+// the column pass; overwritten).  qb: in, the cotangents of the vapour after
+// it; out, before it.  dpb accumulates the cotangents of dp.
+SH_FN void fix_negative_column_vjp(double* q, const double* dp, int nz, double* qb, double* dpb) {
+  // the forward recurrence, kept: q[k] becomes the value after it
+  for (int k = 1; k < nz; ++k)
+    if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * dp[k - 1] / dp[k];
+  const int kb = nz - 1, ka = nz - 2;
+  const double qkb = q[kb], qka = q[ka] < 0.0 ? 0.0 : q[ka];
+  // the surface layer's borrowing from the layer above
+  if ((qkb < 0.0) && (qka > 0.0)) {
+    const double a = -qkb * dp[kb], c = qka * dp[ka];
+    const double dq = mmin(a, c);
+    const double dqb = -qb[ka] / dp[ka] + qb[kb] / dp[kb];
+    dpb[ka] += qb[ka] * dq / (dp[ka] * dp[ka]);
+    dpb[kb] -= qb[kb] * dq / (dp[kb] * dp[kb]);
+    if (a < c) {
+      qb[kb] -= dqb * dp[kb];
+      dpb[kb] -= dqb * qkb;
+    } else {
+      qb[ka] += dqb * dp[ka];
+      dpb[ka] += dqb * qka;
+    }
+  }
+  // negative vapour above the surface layer cleared
+  for (int k = 0; k < nz - 1; ++k)
+    if (q[k] < 0.0) qb[k] = 0.0;
+  // the recurrence, backwards
+  for (int k = nz - 1; k >= 1; --k) {
+    if (q[k - 1] < 0.0) {
+      const double r = dp[k - 1] / dp[k];
+      qb[k - 1] += qb[k] * r;
+      dpb[k - 1] += qb[k] * q[k - 1] / dp[k];
+      dpb[k] -= qb[k] * q[k - 1] * r / dp[k];
+    }
+  }
+}
+
+// The reverse of fix_negative for the wavefront: the entry cells are formed
+// again from the inputs as the replay formed them, the cell pass is redone to
+// give the column pass its input, and both passes are transposed (the column
+// pass one lane per column in LDS, which the reverse walk has left free).
+template <int S, class FS>
+__device__ __forceinline__ void fix_negative_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G,
+                                                 const St<S, double>& X, const Glob<S>& Gl, const FS& f,
+                                                 const shield_mp_params& P) {
+  Shared<S>& sh = A.sh;
+  const int lane = lane_id();
+  Cell x0[S];
+  double qv5[S];
+  bool neg = false;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    x0[s] = Cell{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 250.0, 0.0};
+    qv5[s] = 0.0;
+    if (!L.v[s]) continue;
+    const int32_t o = Gl.off[s];
+    const double omq = f.delp[o] / X.dp1[s];
+    x0[s] = Cell{launder(f.qv[o] * omq), launder(f.ql[o] * omq), launder(f.qr[o] * omq), launder(f.qi[o] * omq),
+                 launder(f.qs[o] * omq), launder(f.qg[o] * omq), f.pt[o], 0.0};
+    Cell x5 = x0[s];
+    fix_negative_cell<double>(P, x5);
+    qv5[s] = x5.qv;
+    neg = neg || x5.qv < 0.0;
+  }
+  if (__ballot(neg) != 0ull) {
+    double* const q = sh.buf;
+    double* const dp = sh.cell[CL_M1];
+    double* const qb = sh.cell[CL_M1SOL];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      q[64 * s + lane] = qv5[s];
+      dp[64 * s + lane] = X.dp1[s];
+      qb[64 * s + lane] = G.xb.v[0][s];
+    }
+    __syncthreads();
+    if (lane < L.ncols) {
+      const int b0 = lane * L.nz;
+      fix_negative_column_vjp(q + b0, dp + b0, L.nz, qb + b0, A.bar[AB_DP1] + b0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; ++s) G.xb.v[0][s] = qb[64 * s + lane];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s]) continue;
+    double b[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) b[q] = G.xb.v[q][s];
+    fix_negative_cell_vjp(P, x0[s], b);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) G.xb.v[q][s] = b[q];
+  }
+}
+
+
 // ------------------------------------------------------------------ kernel
 template <int S>
 __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
@@ -1276,12 +1432,14 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
         const double dp0 = f.delp[o];
         X.dp1[s] = dp0 * (1.0 - f.qv[o]);
         const double omq = dp0 / X.dp1[s];
-        X.qv[s] = f.qv[o] * omq;
-        X.ql[s] = f.ql[o] * omq;
-        X.qr[s] = f.qr[o] * omq;
-        X.qi[s] = f.qi[o] * omq;
-        X.qs[s] = f.qs[o] * omq;
-        X.qg[s] = f.qg[o] * omq;
+        // the products are materialised exactly (launder), so fix_negative's
+        // reverse, which forms them again, branches as the replay did
+        X.qv[s] = launder(f.qv[o] * omq);
+        X.ql[s] = launder(f.ql[o] * omq);
+        X.qr[s] = launder(f.qr[o] * omq);
+        X.qi[s] = launder(f.qi[o] * omq);
+        X.qs[s] = launder(f.qs[o] * omq);
+        X.qg[s] = launder(f.qg[o] * omq);
         X.qaz[s] = n.reset_qaz ? 0.0 : f.qaz[o];
       } else {
         X.tz[s] = 250.0;
@@ -1306,6 +1464,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     }
     __syncthreads();
     if (lane < ncols) sh.colv[CV_CPRAUT][lane] = P.cpaut * pow(sh.colv[CV_CCN][lane] * RHOR, -1.0 / 3.0);
+    if (n.fix_negative) fix_negative(L, X, sh, P);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       sh.cell[CL_M1][64 * s + lane] = 0.0;
@@ -1422,6 +1581,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
       warm_rain_adj(L, A, G, Gl, f, P, c, true, T, it, RW1_B, X.dp1);
       __syncthreads();
     }
+    if (n.fix_negative) fix_negative_adj(L, A, G, X, Gl, f, P);
     // -------------------------------------------------- fields_init, reversed
 #pragma unroll
     for (int s = 0; s < S; ++s) {

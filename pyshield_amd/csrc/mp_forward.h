@@ -655,6 +655,86 @@ __device__ __forceinline__ void icloud(const Lay<S>& L, St<S, R>& X, Shared<S>& 
   }
 }
 
+// fields_init's fix_negative (microphysics.py:201-276).  At one cell: a
+// negative species borrows from the next one (ice from snow, snow from
+// graupel, graupel from rain with the fusion heat, rain from cloud water,
+// cloud water from vapour with the condensation heat), the heats per unit
+// mass formed from the entry state.
+template <class R>
+SH_FN void fix_negative_cell(const shield_mp_params& P, Cell& x) {
+  const double cvm = P.c_air + x.qv * P.c_vap + (x.qr + x.ql) * C_LIQ + (x.qi + x.qs + x.qg) * C_ICE;
+  const double tz0 = x.tz;
+  if (x.qi < 0.0) {
+    x.qs = x.qs + x.qi;
+    x.qi = 0.0;
+  }
+  if (x.qs < 0.0) {
+    x.qg = x.qg + x.qs;
+    x.qs = 0.0;
+  }
+  if (x.qg < 0.0) {
+    const double icpk = fdiv<R>(LI00 + DC_ICE * tz0, cvm);
+    x.qr = x.qr + x.qg;
+    x.tz = x.tz - x.qg * icpk;
+    x.qg = 0.0;
+  }
+  if (x.qr < 0.0) {
+    x.ql = x.ql + x.qr;
+    x.qr = 0.0;
+  }
+  if (x.ql < 0.0) {
+    const double lcpk = fdiv<R>(P.lv00 + P.d0_vap * tz0, cvm);
+    x.qv = x.qv + x.ql;
+    x.tz = x.tz - x.ql * lcpk;
+    x.ql = 0.0;
+  }
+}
+// Then the vapour of one column (q, dp: nz levels, top first): a negative
+// layer borrows from the one below (a forward recurrence), negative vapour
+// above the surface layer is cleared, and a negative surface layer borrows
+// from the layer above.
+SH_FN void fix_negative_column(double* q, const double* dp, int nz) {
+  for (int k = 1; k < nz; ++k)
+    if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * dp[k - 1] / dp[k];
+  for (int k = 0; k < nz - 1; ++k)
+    if (q[k] < 0.0) q[k] = 0.0;
+  const int kb = nz - 1, ka = nz - 2;
+  if ((q[kb] < 0.0) && (q[ka] > 0.0)) {
+    const double dq = mmin(-q[kb] * dp[kb], q[ka] * dp[ka]);
+    q[ka] = q[ka] - dq / dp[ka];
+    q[kb] = q[kb] + dq / dp[kb];
+  }
+}
+// The wavefront's cells and columns.  The vapour borrowing is a non-linear
+// recurrence: one lane per column walks its column in LDS, dp1 staged where
+// m1 lives later.  It changes nothing unless some cell holds negative vapour.
+template <int S, class R>
+__device__ __forceinline__ void fix_negative(const Lay<S>& L, St<S, R>& X, Shared<S>& sh, const shield_mp_params& P) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    Cell cs = cell_of(X, s);
+    fix_negative_cell<R>(P, cs);
+    set_cell(X, s, cs);
+  }
+  bool neg = false;
+#pragma unroll
+  for (int s = 0; s < S; ++s) neg = neg || (L.v[s] && X.qv[s] < 0.0);
+  if (__ballot(neg) != 0ull) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      sh.buf[64 * s + lane] = X.qv[s];
+      sh.cell[CL_M1][64 * s + lane] = X.dp1[s];
+    }
+    __syncthreads();
+    if (lane < L.ncols) fix_negative_column(sh.buf + lane * L.nz, sh.cell[CL_M1] + lane * L.nz, L.nz);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; ++s) X.qv[s] = sh.buf[64 * s + lane];
+    __syncthreads();
+  }
+}
+
 // The state update that follows the microphysics for this wavefront's
 // columns: update_physics_state_with_tendencies (physics.py:146-194),
 // fill_gfs_delp on the updated vapour with the dycore delp
@@ -921,71 +1001,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
       sh.colv[CV_CPRAUT][lane] = P.cpaut * pow(x, -1.0 / 3.0);
   }
   __syncthreads();
-  if (n.fix_negative) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      // lcpk and icpk of the entry state, formed only where a branch uses them
-      const double cvm = P.c_air + X.qv[s] * P.c_vap + (X.qr[s] + X.ql[s]) * C_LIQ +
-                         (X.qi[s] + X.qs[s] + X.qg[s]) * C_ICE;
-      const double tz0 = X.tz[s];
-      if (X.qi[s] < 0.0) {
-        X.qs[s] = X.qs[s] + X.qi[s];
-        X.qi[s] = 0.0;
-      }
-      if (X.qs[s] < 0.0) {
-        X.qg[s] = X.qg[s] + X.qs[s];
-        X.qs[s] = 0.0;
-      }
-      if (X.qg[s] < 0.0) {
-        const double icpk = fdiv<R>(LI00 + DC_ICE * tz0, cvm);
-        X.qr[s] = X.qr[s] + X.qg[s];
-        X.tz[s] = X.tz[s] - X.qg[s] * icpk;
-        X.qg[s] = 0.0;
-      }
-      if (X.qr[s] < 0.0) {
-        X.ql[s] = X.ql[s] + X.qr[s];
-        X.qr[s] = 0.0;
-      }
-      if (X.ql[s] < 0.0) {
-        const double lcpk = fdiv<R>(P.lv00 + P.d0_vap * tz0, cvm);
-        X.qv[s] = X.qv[s] + X.ql[s];
-        X.tz[s] = X.tz[s] - X.ql[s] * lcpk;
-        X.ql[s] = 0.0;
-      }
-    }
-    // vapour borrowing is a non-linear recurrence: one lane per column walks
-    // its column in LDS (:244-276); dp1 is staged where m1 lives later.  It
-    // changes nothing unless some cell holds negative vapour.
-    bool neg = false;
-#pragma unroll
-    for (int s = 0; s < S; ++s) neg = neg || (L.v[s] && X.qv[s] < 0.0);
-    if (__ballot(neg) != 0ull) {
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        sh.buf[64 * s + lane] = X.qv[s];
-        sh.cell[CL_M1][64 * s + lane] = X.dp1[s];
-      }
-      __syncthreads();
-      if (lane < ncols) {
-        double* q = sh.buf + lane * nz;
-        const double* dp = sh.cell[CL_M1] + lane * nz;
-        for (int k = 1; k < nz; ++k)
-          if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * dp[k - 1] / dp[k];
-        for (int k = 0; k < nz - 1; ++k)
-          if (q[k] < 0.0) q[k] = 0.0;
-        const int kb = nz - 1, ka = nz - 2;
-        if ((q[kb] < 0.0) && (q[ka] > 0.0)) {
-          const double dq = mmin(-q[kb] * dp[kb], q[ka] * dp[ka]);
-          q[ka] = q[ka] - dq / dp[ka];
-          q[kb] = q[kb] + dq / dp[kb];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < S; ++s) X.qv[s] = sh.buf[64 * s + lane];
-      __syncthreads();
-    }
-  }
+  if (n.fix_negative) fix_negative(L, X, sh, P);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     sh.cell[CL_M1][64 * s + lane] = 0.0;

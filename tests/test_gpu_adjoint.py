@@ -173,6 +173,42 @@ def test_adjoint_switches_match_autograd_of_oracle(switches):
     assert np.all(np.abs(got["params"][~live]) == 0.0), got["params"]
 
 
+def negative_species_state(seed=8008):
+    """small_state with negative species in some cells: every link of
+    fix_negative's borrowing chain (ice from snow, snow from graupel, graupel
+    from rain, rain from cloud water, cloud water from vapour), negative
+    vapour that borrows down a column, and a negative surface layer that
+    borrows from the layer above."""
+    st = small_state(2, 3, 79, seed=seed)
+    rng = np.random.default_rng(seed)
+    shp = st["pt"].shape
+    for q, frac in (("qi", 0.08), ("qs", 0.06), ("qg", 0.06), ("qr", 0.06), ("ql", 0.06)):
+        m = rng.random(shp) < frac
+        st[q] = np.where(m, -rng.uniform(1e-7, 5e-5, shp), st[q])
+    # vapour: a negative run in column (0, 0) above level 40, negative surface layers
+    st["qv"][0, 0, 38:42] = -rng.uniform(1e-6, 1e-5, 4)
+    st["qv"][1, 2, 70] = -2e-6
+    st["qv"][0, 1, -1] = -3e-6
+    st["qv"][1, 1, -1] = -1e-3  # more than the layer above can lend
+    return st
+
+
+def test_adjoint_fix_negative_matches_autograd_of_oracle():
+    dt, seed = 150.0, 8008
+    st = negative_species_state(seed)
+    qaz = np.random.default_rng(seed).uniform(0.0, 1.0, st["pt"].shape)
+    cfg = PhysicsConfig(dt_atmos=int(dt), npz=79, fix_negative=True)
+    cot = _cotangents(st, seed + 1)
+    ref = _oracle_vjp(st, qaz, cfg, dt, cot)
+    got = _gpu_vjp(st, qaz, cfg, dt, cot)
+    errs = {n: norm_err(got[n], ref[n], floor=1e-300) for n in STATE + ("qaz",)}
+    perr = np.abs(got["params"] - ref["params"]) / np.maximum(np.abs(ref["params"]), 1e-300)
+    print({k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert all(np.isfinite(got[n]).all() for n in STATE + ("qaz",))
+    assert all(v <= VJP_TOL for v in errs.values()), errs
+    assert np.all(perr <= VJP_TOL), perr
+
+
 def test_adjoint_dot_product_with_finite_differences():
     """<J v, w> from central differences of the GPU FP64 forward equals
     <v, J^T w> from the adjoint, along smooth random directions."""
