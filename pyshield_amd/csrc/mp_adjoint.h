@@ -28,9 +28,9 @@
 //     the liquid slope limiter, the momentum transport, and the
 //     precipitation and flux sums.
 // Supported switches: the default ones, plus the const_v*, irain_f,
-// z_slope_liq, do_sedi_heat, sedi_transport, do_qa, reset_qaz and
-// fix_negative variants; do_sedi_w, z_slope_ice and de_ice are rejected by
-// the host.
+// z_slope_liq, z_slope_ice, do_sedi_heat, sedi_transport, do_qa, reset_qaz
+// and fix_negative variants; do_sedi_w and de_ice on accumulated tendencies
+// are rejected by the host.
 #pragma once
 
 #include "mp_forward.h"
@@ -1036,6 +1036,145 @@ __device__ __forceinline__ void fields_update_adj(const shield_mp_params& P, dou
   }
 }
 
+// The reverse of icloud with the ice slope limiter (z_slope_ice), in cell
+// order as the forward runs it.  The forward melts / freezes every cell of
+// the column first, then takes the limiter's dl of the post-melt cloud ice
+// for each cell's ice autoconversion width.  So the reverse runs each cell's
+// icloud reverse down to the state after melt / freeze (its dl gaining a
+// cotangent), transposes the limiter across the column (three dual passes
+// per interior level, scattered to k-1 and k+1 as for the liquid limiter),
+// and then runs each cell's melt / freeze reverse.  In the last sub-step
+// fields_update's reverse runs inside, as in icloud_adj.
+template <int S, class FS, class FA>
+__device__ __forceinline__ void icloud_adj_slope(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
+                                                 const FS& f, const FA& a, const shield_mp_params& P,
+                                                 const Coef<double>& c, const TapeView<S>& T, int it, bool last,
+                                                 InBar<S>& nb) {
+  Shared<S>& sh = A.sh;
+  const int lane = lane_id();
+  const double rdt = P.rdt;
+  Cell x0[S];
+  double qmf[S], dl[S], p1[S], dp1[S];
+  // the post-melt cloud ice of every cell, then the limiter over the column
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int idx = 64 * s + lane;
+    x0[s] = Cell{T.at(it, RI, Z_QV)[idx], T.at(it, RI, Z_QL)[idx], T.at(it, RI, Z_QR)[idx], T.at(it, RI, Z_QI)[idx],
+                 T.at(it, RI, Z_QS)[idx], T.at(it, RI, Z_QG)[idx], T.at(it, RI, Z_TZ)[idx], 0.0};
+    qmf[s] = 0.0;
+    p1[s] = dp1[s] = 1.0;
+    if (!L.v[s]) continue;
+    const int32_t o = Gl.off[s];
+    dp1[s] = f.delp[o] * (1.0 - f.qv[o]);  // fields_init's dp1, the same expression
+    p1[s] = dry_density<double>(dp1[s], f.delz[o]) * RDGAS * f.pt[o];
+    const IcK k{c, P.nml.tice, P.t_wfr, P.nml.ql_mlt, P.tice0, 0.0, 0.0, 0.0,
+                {sh.cell[CL_DEN][idx], sh.cell[CL_DENFAC][idx], 0.0, 0.0, 0.0, p1[s]}};
+    IcX y{x0[s].qv, x0[s].ql, x0[s].qr, x0[s].qi, x0[s].qs, x0[s].qg, x0[s].tz, 0.0, 0.0, 0.0, 0.0};
+    blk_melt_freeze(k, y);
+    qmf[s] = y.qi;
+  }
+  slope_limiter(L, sh, qmf, dl);
+  IcX xbm[S];
+  double dlb[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    dlb[s] = 0.0;
+    xbm[s] = IcX{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!L.v[s]) continue;
+    const int idx = 64 * s + lane;
+    const int cl = L.cl[s];
+    const int32_t o = Gl.off[s];
+    IcK k{c, P.nml.tice, P.t_wfr, P.nml.ql_mlt, P.tice0, sh.colv[CV_HVAR][cl], sh.colv[CV_RHADJ][cl],
+          sh.colv[CV_RHRAIN][cl],
+          {sh.cell[CL_DEN][idx], sh.cell[CL_DENFAC][idx], T.at(it, RS_G, Z_VTG)[idx], T.at(it, RW2_C, Z_VTR)[idx],
+           T.at(it, RS_G, Z_VTS)[idx], p1[s]}};
+    k.slope = true;
+    k.dl = dl[s];
+    double yb[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) yb[q] = last ? 0.0 : G.xb.v[q][s];
+    IcInB ib{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    FuB fb{0.0, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+    icloud_vjp<true>(
+        k, x0[s], yb, ib, nullptr,
+        [&](const Cell& x1, double (&y)[8]) {
+          if (!last) return;
+          const FuIn in{f.delp[o], dp1[s], f.pt[o], a.pt_dt[o], a.qa_dt[o],
+                        {f.qv[o], f.ql[o], f.qr[o], f.qi[o], f.qs[o], f.qg[o]},
+                        {a.qv_dt[o], a.ql_dt[o], a.qr_dt[o], a.qi_dt[o], a.qs_dt[o], a.qg_dt[o]}};
+          fields_update_adj(P, rdt, x1, in, y, fb);
+        },
+        &xbm[s]);
+    if (last) {
+      nb.pt[s] = fb.pt;
+      nb.dp0[s] += fb.dp0;
+      nb.qa[s] = fb.qa;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) nb.q[q][s] = fb.q[q];
+      abar(A, AB_DP1, s) += fb.dp1;
+      G.xb.v[7][s] = yb[7];
+    }
+    dlb[s] = ib.dl;
+    abar(A, AB_DEN, s) += ib.den;
+    abar(A, AB_DENFAC, s) += ib.denfac;
+    abar(A, AB_VTG, s) += ib.vtg;
+    abar(A, AB_VTR, s) += ib.vtr;
+    abar(A, AB_VTS, s) += ib.vts;
+    G.gp[AP_QI_GEN] += ib.qi_gen;
+    // p1 = -dp1 / (g dz0) rdgas t0
+    abar(A, AB_DP1, s) += ib.p1 * p1[s] / dp1[s];
+    abar(A, AB_DZ0, s) -= ib.p1 * p1[s] / f.delz[o];
+    abar(A, AB_T0, s) += ib.p1 * p1[s] / f.pt[o];
+  }
+  // the limiter's transpose: dl[k] = slope_point(q[k-1], q[k], q[k+1]) at
+  // interior levels
+  {
+    double qp[S], qn[S], gm[S], gp[S], gmn[S], gpp[S];
+    col_prev(sh.buf, qmf, qp);
+    col_next(sh.buf, qmf, qn);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      gm[s] = gp[s] = 0.0;
+      if (!L.v[s] || L.k[s] == 0 || L.kb[s] == 0 || dlb[s] == 0.0) continue;
+      const Dual dm = slope_point<Dual>(Dual(qp[s], 1.0), Dual(qmf[s]), Dual(qn[s]));
+      const Dual d0 = slope_point<Dual>(Dual(qp[s]), Dual(qmf[s], 1.0), Dual(qn[s]));
+      const Dual dp = slope_point<Dual>(Dual(qp[s]), Dual(qmf[s]), Dual(qn[s], 1.0));
+      xbm[s].qi += dlb[s] * d0.d;
+      gm[s] = dlb[s] * dm.d;  // lands on level k-1
+      gp[s] = dlb[s] * dp.d;  // lands on level k+1
+    }
+    col_next(sh.buf, gm, gmn);
+    col_prev(sh.buf, gp, gpp);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!L.v[s]) continue;
+      if (L.kb[s] > 0) xbm[s].qi += gmn[s];
+      if (L.k[s] > 0) xbm[s].qi += gpp[s];
+    }
+  }
+  // melt / freeze, reversed
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s]) continue;
+    const int idx = 64 * s + lane;
+    const IcK k{c, P.nml.tice, P.t_wfr, P.nml.ql_mlt, P.tice0, 0.0, 0.0, 0.0,
+                {sh.cell[CL_DEN][idx], sh.cell[CL_DENFAC][idx], 0.0, 0.0, 0.0, p1[s]}};
+    IcInB ib{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const IcX xe{x0[s].qv, x0[s].ql, x0[s].qr, x0[s].qi, x0[s].qs, x0[s].qg, x0[s].tz, 0.0, 0.0, 0.0, 0.0};
+    blk_melt_freeze_rev(k, xe, xbm[s], ib);
+    G.xb.v[0][s] = xbm[s].qv;
+    G.xb.v[1][s] = xbm[s].ql;
+    G.xb.v[2][s] = xbm[s].qr;
+    G.xb.v[3][s] = xbm[s].qi;
+    G.xb.v[4][s] = xbm[s].qs;
+    G.xb.v[5][s] = xbm[s].qg;
+    G.xb.v[6][s] = xbm[s].tz;
+    abar(A, AB_DEN, s) += ib.den;
+    G.gp[AP_QI_GEN] += ib.qi_gen;
+  }
+  __syncthreads();
+}
+
 // The reverse of icloud (record RI): melt / freeze, then icloud_main with
 // subgrid_z_proc, on the cells sorted by condensate as the forward's icloud
 // sorts them (some condensate above QRMIN, above QCMIN, none), so a slot of
@@ -1056,6 +1195,10 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
                                            const FS& f, const FA& a, const shield_mp_params& P,
                                            const Coef<double>& c, const TapeView<S>& T, int it, bool last,
                                            InBar<S>& nb) {
+  if (P.nml.z_slope_ice) {
+    icloud_adj_slope(L, A, G, Gl, f, a, P, c, T, it, last, nb);
+    return;
+  }
   Shared<S>& sh = A.sh;
   const int lane = lane_id();
   int pos[S], src[S], osrc[S];

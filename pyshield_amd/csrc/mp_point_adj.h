@@ -39,12 +39,17 @@ struct IcIn {
 };
 struct IcInB {
   double den, denfac, vtg, vtr, vts, p1, qi_gen;
+  double dl;  // the ice slope limiter's value (z_slope_ice)
 };
 
 struct IcK {
   const Coef<double>& c;
   double tice, t_wfr, ql_mlt, tice0, hv, rh_adj, rh_rain;
   IcIn in;
+  // z_slope_ice: the ice autoconversion's width takes the column's slope
+  // limiter value dl of the post-melt cloud ice where it is larger
+  bool slope = false;
+  double dl = 0.0;
 };
 
 SH_FN double ic_cvm(const Coef<double>& c, double qv, double q_liq, double q_sol) {
@@ -510,7 +515,8 @@ SH_FN bool blk_c1(const IcK& k, IcX& x) {
   }
   const double qim = c.qi0_crt / den;
   const double tmp = c.const_vi ? c.fac_i2s : c.fac_i2s * exp(0.025 * tc);
-  const double di = mmax(mmax(QVMIN, k.hv * x.qi), QRMIN);
+  const double dm = mmax(QVMIN, k.hv * x.qi);
+  const double di = mmax(k.slope ? mmax(k.dl, dm) : dm, QRMIN);
   const double q_plus = x.qi + di;
   double psaut = 0.0;
   if (q_plus > (qim + QRMIN)) {
@@ -546,7 +552,8 @@ SH_FN void blk_c1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   }
   const double qim = c.qi0_crt / den;
   const double tmp = c.const_vi ? c.fac_i2s : c.fac_i2s * exp(0.025 * tc);
-  const double di0 = mmax(QVMIN, k.hv * x.qi);
+  const double dm = mmax(QVMIN, k.hv * x.qi);
+  const double di0 = k.slope ? mmax(k.dl, dm) : dm;
   const double di = mmax(di0, QRMIN);
   const double q_plus = x.qi + di;
   double psaut = 0.0, dq = 0.0;
@@ -594,7 +601,12 @@ SH_FN void blk_c1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     ib.den -= qimb * qim / den;
     if (!c.const_vi) tcb += tmpb * 0.025 * tmp;
   }
-  if ((di0 > QRMIN) && !(QVMIN > k.hv * x.qi)) qib += dib * k.hv;
+  if (di0 > QRMIN) {
+    if (k.slope && k.dl > dm)
+      ib.dl += dib;
+    else if (!(QVMIN > k.hv * x.qi))
+      qib += dib * k.hv;
+  }
   if (x.qs > 1.0e-7) {
     const double D = 1.0 + f1;
     const double fb = psacib * x.qi / (D * D);
@@ -1613,9 +1625,12 @@ SH_FN double blk_cf_rev(const IcK& k, const IcX& x, double icpk7, IcX& xb, IcInB
 struct IcNoMid {
   SH_FN void operator()(const Cell&, double (&)[8]) const {}
 };
+// With xb_mf, the reverse stops before the melt / freeze block: xb_mf
+// receives the cotangents of the state after it (the cell and the running
+// q_liq, q_sol, cvm), and yb is left as it was after mid.
 template <bool MID = false, class Mid = IcNoMid>
 SH_FN void icloud_vjp(const IcK& k, const Cell& s, double (&yb)[8], IcInB& ib, Cell* out = nullptr,
-                      const Mid& mid = Mid()) {
+                      const Mid& mid = Mid(), IcX* xb_mf = nullptr) {
   const Coef<double>& c = k.c;
   const IcX x0{s.qv, s.ql, s.qr, s.qi, s.qs, s.qg, s.tz, s.qa, 0.0, 0.0, 0.0};
   IcX a = x0;
@@ -1687,6 +1702,10 @@ SH_FN void icloud_vjp(const IcK& k, const Cell& s, double (&yb)[8], IcInB& ib, C
   } else if (warm) {
     blk_w2_rev(k, b, xb, ib);
     blk_w1_rev(k, a, xb, ib);
+  }
+  if (xb_mf) {
+    *xb_mf = xb;
+    return;
   }
   blk_melt_freeze_rev(k, x0, xb, ib);
   yb[0] = xb.qv;

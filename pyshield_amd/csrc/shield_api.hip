@@ -180,7 +180,7 @@ const char* shield_strerror(int code) {
     case SHIELD_ERR_TIMESTEP: return "timestep must be positive and finite";
     case SHIELD_ERR_NOGPU: return "no HIP device is visible";
     case SHIELD_ERR_ADJ_SWITCH:
-      return "the adjoint does not differentiate do_sedi_w, z_slope_ice, or de_ice on accumulated tendencies";
+      return "the adjoint does not differentiate do_sedi_w, or de_ice on accumulated tendencies";
     case SHIELD_ERR_WORKSPACE: return "adjoint workspace smaller than one wavefront's tape";
     default: return "unknown error";
   }
@@ -272,7 +272,7 @@ static int check_adjoint(const shield_mp_shape* shape, const shield_mp_params* p
   if (rc != SHIELD_OK) return rc;
   if (!f->qaz && !p->nml.reset_qaz) return SHIELD_ERR_NULL;
   if (shape->nk > 128) return SHIELD_ERR_SHAPE;
-  if (p->do_sedi_w || p->nml.z_slope_ice || (p->nml.de_ice && p->accumulate))
+  if (p->do_sedi_w || (p->nml.de_ice && p->accumulate))
     return SHIELD_ERR_ADJ_SWITCH;
   if (*empty) return SHIELD_OK;
   const void* need[] = {a->pt_dt, a->qv_dt, a->ql_dt, a->qr_dt, a->qi_dt, a->qs_dt, a->qg_dt, a->qa_dt, a->udt,

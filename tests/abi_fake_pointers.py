@@ -157,9 +157,6 @@ def check_adjoint_rejects_before_launch():
         p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
         setattr(p2, field, value)
         assert call(p=p2) == -9
-    p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
-    p2.nml.z_slope_ice = 1
-    assert call(p=p2) == -9
     # de_ice acts only on accumulated tendencies, which the adjoint does not take
     p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
     p2.nml.de_ice = 1

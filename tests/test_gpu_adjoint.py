@@ -149,6 +149,8 @@ SWITCHES = [
     {"do_sedi_heat": False, "sedi_transport": False},
     {"fast_sat_adj": True, "rad_snow": False, "rad_rain": False},
     {"reset_qaz": True},
+    {"z_slope_ice": True},
+    {"z_slope_ice": True, "fix_negative": True},
 ]
 
 
