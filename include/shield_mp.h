@@ -265,6 +265,11 @@ typedef struct shield_mp_adjoint_fields {
   double *pt, *qv, *ql, *qr, *qi, *qs, *qg, *qa, *ua, *va, *delp, *delz;
   double* qaz;
   double* params;
+  /* do_sedi_w: the cotangent of w after the call (read; NULL reads as zero)
+   * and of w before it (written; NULL leaves it unwritten).  Neither is used
+   * without do_sedi_w.                                                       */
+  const double* w_after;
+  double* w;
 } shield_mp_adjoint_fields;
 
 /* Bytes of workspace for `waves` concurrent wavefronts (each walks groups of
@@ -277,8 +282,8 @@ size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shi
  * (accumulate = 0).  The forward of each column is recomputed, so f's inputs
  * must be those of the forward call.  The workspace (ws, ws_bytes) bounds
  * the number of concurrent wavefronts; nk <= 128.  Returns
- * SHIELD_ERR_ADJ_SWITCH for do_sedi_w, or de_ice with accumulate (de_ice
- * changes nothing on overwritten tendencies),
+ * SHIELD_ERR_ADJ_SWITCH for de_ice with accumulate (de_ice changes nothing
+ * on overwritten tendencies),
  * and SHIELD_ERR_ALIAS when a written cotangent, the parameter gradient or
  * the workspace shares memory with another of them or with a field read.
  * No reference counterpart: the reference has no derivative code; this is

@@ -97,8 +97,9 @@ AD_PARAM_NAMES = ("fac_rc", "vr_fac", "vs_fac", "vg_fac", "vi_fac", "qi_gen")
 
 
 class ShieldMpAdjointFields(ctypes.Structure):
+    # d_w / g_w: the cotangents of w after and before the call (do_sedi_w)
     _fields_ = [("d_" + n, _VP) for n in ADJ_OUT_NAMES] + [("g_" + n, _VP) for n in ADJ_IN_NAMES] + [
-        ("params", _VP)]
+        ("params", _VP), ("d_w", _VP), ("g_w", _VP)]
 
 
 # physics_updated_* order of shield_update_fields.updated and the FV3 tracer

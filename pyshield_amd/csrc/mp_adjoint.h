@@ -27,10 +27,11 @@
 //     implicit fall and the sedimentation heating (backward affine scans),
 //     the liquid slope limiter, the momentum transport, and the
 //     precipitation and flux sums.
-// Supported switches: the default ones, plus the const_v*, irain_f,
-// z_slope_liq, z_slope_ice, do_sedi_heat, sedi_transport, do_qa, reset_qaz
-// and fix_negative variants; do_sedi_w and de_ice on accumulated tendencies
-// are rejected by the host.
+// Supported switches: all of the forward's but de_ice on accumulated
+// tendencies (the adjoint differentiates overwritten ones, on which de_ice
+// changes nothing), which the host rejects.  fix_negative, z_slope_ice and
+// do_sedi_w run in an instance of their own (EXT), which keeps their code out
+// of the default one.
 #pragma once
 
 #include "mp_forward.h"
@@ -76,6 +77,10 @@ __host__ __device__ constexpr int rec_offset(int rec) {
 }
 constexpr int REC_FIELDS = rec_offset(REC_N);
 static_assert(REC_FIELDS == 77, "fields per sub-step");
+// With do_sedi_w, per sub-step after all the records: w before each
+// transporting stage (the first rain fall, the solid falls, the second rain
+// fall) and qv, ql, qr before the solid falls (the ice melt changes ql, qr).
+enum { SW_W1, SW_WS, SW_W2, SW_QV, SW_QL, SW_QR, SW_FIELDS };
 
 // The adjoint kernel's argument block.  As in the forward kernel, the field
 // pointers of f and a are read from the kernarg segment where they are used
@@ -93,7 +98,7 @@ struct AdjArgs {
 };
 #define SH_AFIELDS(X)                                                                                            \
   X(pt_dt) X(qv_dt) X(ql_dt) X(qr_dt) X(qi_dt) X(qs_dt) X(qg_dt) X(qa_dt) X(udt) X(vdt) X(rain) X(snow) X(ice) \
-  X(graupel) X(pt) X(qv) X(ql) X(qr) X(qi) X(qs) X(qg) X(qa) X(ua) X(va) X(delp) X(delz) X(qaz)
+  X(graupel) X(pt) X(qv) X(ql) X(qr) X(qi) X(qs) X(qg) X(qa) X(ua) X(va) X(delp) X(delz) X(qaz) X(w_after) X(w)
 // shield_mp_adjoint_fields at kernarg offset B, member by member
 template <uint32_t B>
 struct KAdjFields {
@@ -117,6 +122,8 @@ enum { CB_RAIN, CB_SNOW, CB_ICE, CB_GRAUPEL, CB_CCN, CB_CPRAUT };
 template <int S>
 struct TapeView {
   double* base;  // this wavefront's region: ntimes * REC_FIELDS * 64 S doubles
+  double* sw;    // then, with do_sedi_w, ntimes * SW_FIELDS * 64 S doubles
+  __device__ __forceinline__ double* at_sw(int it, int z) const { return sw + ((size_t)it * SW_FIELDS + z) * (64 * S); }
   // field z of record rec (z must be one of rec_fields(rec))
   __device__ __forceinline__ double* at(int it, int rec, int z) const {
     const int k = rec_offset(rec) + nbits(rec_fields(rec) & (zbit(z) - 1u));
@@ -418,12 +425,82 @@ __device__ __forceinline__ void fall_fwd(const Lay<S>& L, Shared<S>& sh, const d
 }
 
 
+// ------------------------------------------------------- w transport (do_sedi_w)
+// The forward's transport of the vertical velocity with a species' falling
+// mass (implicit_fall with do_sedi_w, microphysics.py:499-515): in falling
+// columns w[k] becomes (dm w - F[k-1] v[k-1] + F[k] v[k]) / (dm + F[k-1] - F[k])
+// (no k-1 terms at the top), dm the layer's total mass before the fall, F
+// the fall's mass flux through each layer bottom, v its fall speed.
+template <int S>
+__device__ __forceinline__ void w_transport(const Lay<S>& L, Shared<S>& sh, const double (&dm)[S],
+                                            const double (&flux)[S], const double (&vt)[S], const bool (&fall)[S],
+                                            double (&w)[S]) {
+  double fl[S], fp[S], vp[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) fl[s] = fall[s] ? flux[s] : 0.0;
+  col_prev(sh.buf, fl, fp);
+  col_prev(sh.buf, vt, vp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!fall[s] || !L.v[s]) continue;
+    if (L.k[s] == 0)
+      w[s] = (dm[s] * w[s] + fl[s] * vt[s]) / (dm[s] - fl[s]);
+    else
+      w[s] = (dm[s] * w[s] - fp[s] * vp[s] + fl[s] * vt[s]) / (dm[s] + fp[s] - fl[s]);
+  }
+}
+// Its transpose: w0, the velocity before; wb in: the cotangent after, out:
+// before.  Accumulates the cotangents of dm, the flux and the fall speed (the
+// k-1 terms land on level k-1).
+template <int S>
+__device__ __forceinline__ void w_transport_adj(const Lay<S>& L, Shared<S>& sh, const double (&dm)[S],
+                                                const double (&flux)[S], const double (&vt)[S], const bool (&fall)[S],
+                                                const double (&w0)[S], double (&wb)[S], double (&dmb)[S],
+                                                double (&fluxb)[S], double (&vtb)[S]) {
+  double fl[S], fp[S], vp[S], fpb[S], vpb[S], fpbn[S], vpbn[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) fl[s] = fall[s] ? flux[s] : 0.0;
+  col_prev(sh.buf, fl, fp);
+  col_prev(sh.buf, vt, vp);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    fpb[s] = vpb[s] = 0.0;
+    if (!fall[s] || !L.v[s]) continue;
+    const bool top = L.k[s] == 0;
+    const double N = top ? dm[s] * w0[s] + fl[s] * vt[s] : dm[s] * w0[s] - fp[s] * vp[s] + fl[s] * vt[s];
+    const double D = top ? dm[s] - fl[s] : dm[s] + fp[s] - fl[s];
+    const double wn = N / D;
+    const double Nb = wb[s] / D, Db = -wb[s] * wn / D;
+    wb[s] = Nb * dm[s];
+    dmb[s] += Nb * w0[s] + Db;
+    fluxb[s] += Nb * vt[s] - Db;
+    vtb[s] += Nb * fl[s];
+    if (!top) {
+      fpb[s] = Db - Nb * vp[s];
+      vpb[s] = -Nb * fp[s];
+    }
+  }
+  col_next(sh.buf, fpb, fpbn);
+  col_next(sh.buf, vpb, vpbn);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s] || L.kb[s] == 0) continue;
+    fluxb[s] += fpbn[s];
+    vtb[s] += vpbn[s];
+  }
+}
+// dm = dp1 (1 + qv + ql + qr + qi + qs + qg) of a cell
+__device__ __forceinline__ double total_mass(double dp1, double qv, double ql, double qr, double qi, double qs,
+                                             double qg) {
+  return dp1 * (1.0 + qv + ql + qr + qi + qs + qg);
+}
+
 // ------------------------------------------------------- taped forward
 template <int S, class FS>
 __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X, Shared<S>& sh, const Glob<S>& G,
                                               const FS& f, const shield_mp_params& P,
                                               const Coef<double>& c, bool is_first, const TapeView<S>& T, int it,
-                                              int rb) {
+                                              int rb, double (&wv)[S], bool sw) {
   const int lane = lane_id();
   const double t_wfr = P.t_wfr;
   if (is_first) {
@@ -455,9 +532,18 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
   if (wave_any(fall)) {
     evaporation(L, X, sh, c, t_wfr, dt5, fall);
     tape_save(T, it, rb + 1, X, sh);
-    double dz[S];
+    double dz[S], dm[S];
     layer_dz(L, sh, dz);
+    if (sw) {
+      // w and the total mass before the fall (the adjoint leaves f.w alone)
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        T.at_sw(it, is_first ? SW_W1 : SW_W2)[64 * s + lane] = wv[s];
+        dm[s] = total_mass(X.dp1[s], X.qv[s], X.ql[s], X.qr[s], X.qi[s], X.qs[s], X.qg[s]);
+      }
+    }
     implicit_fall(L, X, sh, G, f, dz, X.qr, X.vtr, P.dt_rain, fall, false, flux, surf);
+    if (sw) w_transport(L, sh, dm, flux, X.vtr, fall, wv);
     if (P.nml.do_sedi_heat) sedi_heat(L, X, sh, flux, C_LIQ, fall);
     tape_save(T, it, rb + 2, X, sh);
     evaporation(L, X, sh, c, t_wfr, dt5, fall);
@@ -499,7 +585,7 @@ __device__ __forceinline__ void warm_rain_rec(const Lay<S>& L, St<S, double>& X,
 template <int S, class FS>
 __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Shared<S>& sh, const Glob<S>& G,
                                         const FS& f, const shield_mp_params& P, const Coef<double>& c,
-                                        const TapeView<S>& T, int it) {
+                                        const TapeView<S>& T, int it, double (&wv)[S], bool sw) {
   const int lane = lane_id();
   const double tice_d = P.nml.tice;
   tape_save(T, it, RS_F, X, sh);
@@ -525,15 +611,29 @@ __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Share
     }
   }
   tape_save(T, it, RS_G, X, sh);
-  double dz[S], flux[S], surf[S], m1_sol[S];
+  double dz[S], flux[S], surf[S], m1_sol[S], dm[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) m1_sol[s] = 0.0;
   layer_dz(L, sh, dz);
+  if (sw) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int idx = 64 * s + lane;
+      T.at_sw(it, SW_WS)[idx] = wv[s];
+      T.at_sw(it, SW_QV)[idx] = X.qv[s];
+      T.at_sw(it, SW_QL)[idx] = X.ql[s];
+      T.at_sw(it, SW_QR)[idx] = X.qr[s];
+    }
+  }
 #pragma unroll
   for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qi[s] > QRMIN);
   col_any(L, pred, fall);
   if (!(P.nml.vi_fac < 1.0e-5) && wave_any(fall)) {
+    if (sw)
+#pragma unroll
+      for (int s = 0; s < S; ++s) dm[s] = total_mass(X.dp1[s], X.qv[s], X.ql[s], X.qr[s], X.qi[s], X.qs[s], X.qg[s]);
     implicit_fall(L, X, sh, G, f, dz, X.qi, vti, P.dts, fall, false, flux, surf);
+    if (sw) w_transport(L, sh, dm, flux, vti, fall, wv);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       m1_sol[s] = flux[s];
@@ -544,7 +644,11 @@ __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Share
   for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qs[s] > QRMIN);
   col_any(L, pred, fall);
   if (wave_any(fall)) {
+    if (sw)
+#pragma unroll
+      for (int s = 0; s < S; ++s) dm[s] = total_mass(X.dp1[s], X.qv[s], X.ql[s], X.qr[s], X.qi[s], X.qs[s], X.qg[s]);
     implicit_fall(L, X, sh, G, f, dz, X.qs, X.vts, P.dts, fall, false, flux, surf);
+    if (sw) w_transport(L, sh, dm, flux, X.vts, fall, wv);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       m1_sol[s] = m1_sol[s] + flux[s];
@@ -555,7 +659,11 @@ __device__ __forceinline__ void sed_rec(const Lay<S>& L, St<S, double>& X, Share
   for (int s = 0; s < S; ++s) pred[s] = L.v[s] && (X.qg[s] > QRMIN);
   col_any(L, pred, fall);
   if (wave_any(fall)) {
+    if (sw)
+#pragma unroll
+      for (int s = 0; s < S; ++s) dm[s] = total_mass(X.dp1[s], X.qv[s], X.ql[s], X.qr[s], X.qi[s], X.qs[s], X.qg[s]);
     implicit_fall(L, X, sh, G, f, dz, X.qg, X.vtg, P.dts, fall, false, flux, surf);
+    if (sw) w_transport(L, sh, dm, flux, X.vtg, fall, wv);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       m1_sol[s] = m1_sol[s] + flux[s];
@@ -581,7 +689,10 @@ template <int S>
 struct AdjRegs {
   CellBar<S> xb;     // cotangent of the cell values
   double gp[AP_N];   // parameter gradient (this lane's share)
+  double wb[S];      // cotangent of the vertical velocity (do_sedi_w)
 };
+
+
 
 template <int S>
 __device__ __forceinline__ double& abar(AdjShared<S>& A, int which, int s) {
@@ -753,7 +864,8 @@ template <int S, class FS>
 __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
                                               const FS& f, const shield_mp_params& P,
                                               const Coef<double>& c, bool is_first,
-                                              const TapeView<S>& T, int it, int rb, const double (&dp1)[S]) {
+                                              const TapeView<S>& T, int it, int rb, const double (&dp1)[S],
+                                              bool sw) {
   Shared<S>& sh = A.sh;
   const double t_wfr = P.t_wfr;
   const double dt5 = 0.5 * c.dt_rain;
@@ -800,7 +912,37 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
       for (int s = 0; s < S; ++s) st.z[Z_QR][s] = qn[s];
       heat_adj(L, sh, st, dp1, flux, C_LIQ, fall, G.xb, fluxbar, dp1bar, dz1bar);
     }
+    // the w transport (do_sedi_w), inside the fall: its total mass is that of
+    // the cell before the fall, whose rain cotangent fall_adj forms
+    double dmq[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) dmq[s] = 0.0;
+    if (sw) {
+      const int lane = lane_id();
+      double dm[S], w0[S], dmb[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        dm[s] = total_mass(dp1[s], rC.z[Z_QV][s], rC.z[Z_QL][s], rC.z[Z_QR][s], rC.z[Z_QI][s], rC.z[Z_QS][s],
+                           rC.z[Z_QG][s]);
+        w0[s] = T.at_sw(it, is_first ? SW_W1 : SW_W2)[64 * s + lane];
+        dmb[s] = 0.0;
+      }
+      w_transport_adj(L, sh, dm, flux, vt, fall, w0, G.wb, dmb, fluxbar, vtbar);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        dp1bar[s] += dmb[s] * (dm[s] / dp1[s]);
+        const double g = dmb[s] * dp1[s];
+        G.xb.v[0][s] += g;
+        G.xb.v[1][s] += g;
+        G.xb.v[3][s] += g;
+        G.xb.v[4][s] += g;
+        G.xb.v[5][s] += g;
+        dmq[s] = g;
+      }
+    }
     fall_adj(L, sh, qr, vt, dz1, dp1, P.dt_rain, fall, G.xb.v[2], fluxbar, vtbar, dz1bar, dp1bar);
+#pragma unroll
+    for (int s = 0; s < S; ++s) G.xb.v[2][s] += dmq[s];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       abar(A, AB_VTR, s) += vtbar[s];
@@ -845,7 +987,7 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
 template <int S>
 __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const shield_mp_params& P,
                                         const Coef<double>& c, const TapeView<S>& T, int it,
-                                        const double (&dp1)[S]) {
+                                        const double (&dp1)[S], bool sw) {
   Shared<S>& sh = A.sh;
   const double tice_d = P.nml.tice;
   Rec<S> rF, rG;
@@ -885,9 +1027,35 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
     vtg[s] = rG.z[Z_VTG][s];
     fl_i[s] = fl_s[s] = fl_g[s] = 0.0;
   }
-  if (any_i) fall_fwd(L, sh, qi, vti, dz1, dp1, P.dts, fi, qn, fl_i);
-  if (any_s) fall_fwd(L, sh, qs, vts, dz1, dp1, P.dts, fs, qn, fl_s);
+  // the species after their falls (the total mass of a later fall's w
+  // transport reads them)
+  double qni[S], qns[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    qni[s] = qi[s];
+    qns[s] = qs[s];
+  }
+  if (any_i) fall_fwd(L, sh, qi, vti, dz1, dp1, P.dts, fi, qni, fl_i);
+  if (any_s) fall_fwd(L, sh, qs, vts, dz1, dp1, P.dts, fs, qns, fl_s);
   if (any_g) fall_fwd(L, sh, qg, vtg, dz1, dp1, P.dts, fg, qn, fl_g);
+  // do_sedi_w: w before each solid fall and each fall's total mass
+  double w0[S], w1[S], w2[S], dmi[S], dms[S], dmg[S];
+  if (sw) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int idx = 64 * s + lane;
+      const double qv = T.at_sw(it, SW_QV)[idx], ql = T.at_sw(it, SW_QL)[idx], qr = T.at_sw(it, SW_QR)[idx];
+      w0[s] = w1[s] = T.at_sw(it, SW_WS)[idx];
+      dmi[s] = total_mass(dp1[s], qv, ql, qr, qi[s], qs[s], qg[s]);
+      dms[s] = total_mass(dp1[s], qv, ql, qr, qni[s], qs[s], qg[s]);
+      dmg[s] = total_mass(dp1[s], qv, ql, qr, qni[s], qns[s], qg[s]);
+    }
+    if (any_i) w_transport(L, sh, dmi, fl_i, vti, fi, w1);
+#pragma unroll
+    for (int s = 0; s < S; ++s) w2[s] = w1[s];
+    if (any_s) w_transport(L, sh, dms, fl_s, vts, fs, w2);
+  }
   double fluxbar[S], dz1bar[S], dp1bar[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -906,7 +1074,27 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
   }
   // graupel, snow, ice falls in reverse order; each flux also carries its
   // surface value into the column's precipitation
-  double vtbar[S];
+  // With do_sedi_w each fall's w transport is reversed first (it adds to the
+  // fall's flux and speed cotangents); the cotangent of its total mass goes
+  // to the species as that fall saw them: the falling species' share after
+  // the fall's transpose (before the fall), the others' at once (the later
+  // falls' transposes have already run, the earlier ones' not yet).
+  double vtbar[S], dmb[S];
+  auto w_rev = [&](const double (&dm)[S], const double (&fl)[S], const double (&vt)[S], const bool (&fall)[S],
+                   const double (&wb0)[S], double (&fb)[S], double (&vb)[S], int own) __attribute__((always_inline)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) dmb[s] = 0.0;
+    w_transport_adj(L, sh, dm, fl, vt, fall, wb0, G.wb, dmb, fb, vb);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      dp1bar[s] += dmb[s] * (dm[s] / dp1[s]);
+      const double g = dmb[s] * dp1[s];
+      dmb[s] = g;
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        if (q != own) G.xb.v[q][s] += g;
+    }
+  };
   if (any_g) {
     double fb[S];
 #pragma unroll
@@ -914,9 +1102,13 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
       fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_GRAUPEL][L.cl[s]] : 0.0);
       vtbar[s] = 0.0;
     }
+    if (sw) w_rev(dmg, fl_g, vtg, fg, w2, fb, vtbar, 5);
     fall_adj(L, sh, qg, vtg, dz1, dp1, P.dts, fg, G.xb.v[5], fb, vtbar, dz1bar, dp1bar);
 #pragma unroll
-    for (int s = 0; s < S; ++s) abar(A, AB_VTG, s) += vtbar[s];
+    for (int s = 0; s < S; ++s) {
+      abar(A, AB_VTG, s) += vtbar[s];
+      if (sw) G.xb.v[5][s] += dmb[s];
+    }
   }
   if (any_s) {
     double fb[S];
@@ -925,9 +1117,13 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
       fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_SNOW][L.cl[s]] : 0.0);
       vtbar[s] = 0.0;
     }
+    if (sw) w_rev(dms, fl_s, vts, fs, w1, fb, vtbar, 4);
     fall_adj(L, sh, qs, vts, dz1, dp1, P.dts, fs, G.xb.v[4], fb, vtbar, dz1bar, dp1bar);
 #pragma unroll
-    for (int s = 0; s < S; ++s) abar(A, AB_VTS, s) += vtbar[s];
+    for (int s = 0; s < S; ++s) {
+      abar(A, AB_VTS, s) += vtbar[s];
+      if (sw) G.xb.v[4][s] += dmb[s];
+    }
   }
   double vtibar[S];
 #pragma unroll
@@ -936,7 +1132,11 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
     double fb[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) fb[s] = fluxbar[s] + ((L.v[s] && L.kb[s] == 0) ? A.colbar[CB_ICE][L.cl[s]] : 0.0);
+    if (sw) w_rev(dmi, fl_i, vti, fi, w0, fb, vtibar, 3);
     fall_adj(L, sh, qi, vti, dz1, dp1, P.dts, fi, G.xb.v[3], fb, vtibar, dz1bar, dp1bar);
+    if (sw)
+#pragma unroll
+      for (int s = 0; s < S; ++s) G.xb.v[3][s] += dmb[s];
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -1080,7 +1280,11 @@ __device__ __forceinline__ void icloud_adj_slope(const Lay<S>& L, AdjShared<S>& 
   for (int s = 0; s < S; ++s) {
     dlb[s] = 0.0;
     xbm[s] = IcX{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (!L.v[s]) continue;
+  }
+  // unrolled by construction: a rolled loop would keep the per-slot arrays in scratch
+  for_slots<S>([&](auto si) __attribute__((always_inline)) {
+    constexpr int s = decltype(si)::value;
+    if (!L.v[s]) return;
     const int idx = 64 * s + lane;
     const int cl = L.cl[s];
     const int32_t o = Gl.off[s];
@@ -1125,7 +1329,7 @@ __device__ __forceinline__ void icloud_adj_slope(const Lay<S>& L, AdjShared<S>& 
     abar(A, AB_DP1, s) += ib.p1 * p1[s] / dp1[s];
     abar(A, AB_DZ0, s) -= ib.p1 * p1[s] / f.delz[o];
     abar(A, AB_T0, s) += ib.p1 * p1[s] / f.pt[o];
-  }
+  });
   // the limiter's transpose: dl[k] = slope_point(q[k-1], q[k], q[k+1]) at
   // interior levels
   {
@@ -1194,8 +1398,8 @@ template <int S, class FS, class FA>
 __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, AdjRegs<S>& G, const Glob<S>& Gl,
                                            const FS& f, const FA& a, const shield_mp_params& P,
                                            const Coef<double>& c, const TapeView<S>& T, int it, bool last,
-                                           InBar<S>& nb) {
-  if (P.nml.z_slope_ice) {
+                                           InBar<S>& nb, bool slope) {
+  if (slope) {
     icloud_adj_slope(L, A, G, Gl, f, a, P, c, T, it, last, nb);
     return;
   }
@@ -1499,7 +1703,10 @@ __device__ __forceinline__ void fix_negative_adj(const Lay<S>& L, AdjShared<S>& 
 
 
 // ------------------------------------------------------------------ kernel
-template <int S>
+// EXT: the instance with fix_negative, the ice slope limiter and the w
+// transport (fix_negative, z_slope_ice, do_sedi_w); without it their code is
+// compiled out, since it raises the register pressure of the whole kernel.
+template <int S, bool EXT>
 __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
   const shield_mp_shape& sh3 = args.sh3;
   const shield_mp_params& P = args.P;
@@ -1510,13 +1717,16 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
   const int64_t tape_stride = args.tape_stride;
   double* const part = args.part;
   const int cols_per_wave = args.cols_per_wave;
+  const bool sedi_w = EXT && P.do_sedi_w;
+  const bool fixneg = EXT && P.nml.fix_negative;
   __shared__ AdjShared<S> A;
   Shared<S>& sh = A.sh;
   const int lane = lane_id();
   const int nz = (int)sh3.nk;
   const int64_t ncol = sh3.ni * sh3.nj;
   const shield_mp_namelist& n = P.nml;
-  const TapeView<S> T{tape + (int64_t)blockIdx.x * tape_stride};
+  double* const tw = tape + (int64_t)blockIdx.x * tape_stride;
+  const TapeView<S> T{tw, tw + (int64_t)P.ntimes * REC_FIELDS * 64 * S};
   const double rdt = P.rdt;
   double gp_acc[AP_N];
 #pragma unroll
@@ -1607,19 +1817,23 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     }
     __syncthreads();
     if (lane < ncols) sh.colv[CV_CPRAUT][lane] = P.cpaut * pow(sh.colv[CV_CCN][lane] * RHOR, -1.0 / 3.0);
-    if (n.fix_negative) fix_negative(L, X, sh, P);
+    if (fixneg) fix_negative(L, X, sh, P);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       sh.cell[CL_M1][64 * s + lane] = 0.0;
       sh.cell[CL_M1SOL][64 * s + lane] = 0.0;
       sh.cell[CL_DZ1][64 * s + lane] = 1.0;
     }
+    // the vertical velocity, carried in registers (do_sedi_w)
+    double wv[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) wv[s] = (sedi_w && L.v[s]) ? f.w[Gl.off[s]] : 0.0;
     __syncthreads();
     // ------------------------------------------------ sub-steps, recorded
     for (int it = 0; it < P.ntimes; ++it) {
-      warm_rain_rec(L, X, sh, Gl, f, P, c, true, T, it, RW1_B);
-      sed_rec(L, X, sh, Gl, f, P, c, T, it);
-      warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B);
+      warm_rain_rec(L, X, sh, Gl, f, P, c, true, T, it, RW1_B, wv, sedi_w);
+      sed_rec(L, X, sh, Gl, f, P, c, T, it, wv, sedi_w);
+      warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B, wv, sedi_w);
       tape_save(T, it, RI, X, sh);
       // the last sub-step's icloud runs in its reverse (icloud_adj)
       if (it + 1 < P.ntimes) icloud(L, X, sh, Gl, f, P, c);
@@ -1631,6 +1845,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     for (int p = 0; p < AP_N; ++p) G.gp[p] = 0.0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+      G.wb[s] = (sedi_w && a.w_after && L.v[s]) ? a.w_after[Gl.off[s]] : 0.0;
 #pragma unroll
       for (int q = 0; q < 8; ++q) G.xb.v[q][s] = 0.0;
 #pragma unroll
@@ -1715,16 +1930,16 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     // ------------------------------------------------ sub-steps, reversed
     for (int it = P.ntimes - 1; it >= 0; --it) {
       aux_load(T, it, sh);
-      icloud_adj(L, A, G, Gl, f, a, P, c, T, it, it == P.ntimes - 1, nb);
+      icloud_adj(L, A, G, Gl, f, a, P, c, T, it, it == P.ntimes - 1, nb, EXT && P.nml.z_slope_ice);
       __syncthreads();
-      warm_rain_adj(L, A, G, Gl, f, P, c, false, T, it, RW2_B, X.dp1);
+      warm_rain_adj(L, A, G, Gl, f, P, c, false, T, it, RW2_B, X.dp1, sedi_w);
       __syncthreads();
-      sed_adj(L, A, G, P, c, T, it, X.dp1);
+      sed_adj(L, A, G, P, c, T, it, X.dp1, sedi_w);
       __syncthreads();
-      warm_rain_adj(L, A, G, Gl, f, P, c, true, T, it, RW1_B, X.dp1);
+      warm_rain_adj(L, A, G, Gl, f, P, c, true, T, it, RW1_B, X.dp1, sedi_w);
       __syncthreads();
     }
-    if (n.fix_negative) fix_negative_adj(L, A, G, X, Gl, f, P);
+    if (fixneg) fix_negative_adj(L, A, G, X, Gl, f, P);
     // -------------------------------------------------- fields_init, reversed
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1769,6 +1984,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
       a.delp[o] = nb.dp0[s];
       a.delz[o] = dz0b;
       if (a.qaz) a.qaz[o] = n.reset_qaz ? 0.0 : G.xb.v[7][s];
+      if (sedi_w && a.w) a.w[o] = G.wb[s];
     }
 #pragma unroll
     for (int p = 0; p < AP_N; ++p) gp_acc[p] += G.gp[p];
@@ -1779,6 +1995,15 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     const double v = wave_sum(gp_acc[p]);
     if (lane == 0) part[(int64_t)blockIdx.x * AP_N + p] = v;
   }
+}
+
+// One launch of the adjoint kernel (instantiated with EXT = false in
+// mp_adjoint.hip and EXT = true in mp_adjoint_ext.hip, which compile in
+// parallel).
+template <int S, bool EXT>
+hipError_t launch_adjoint_kernel(unsigned nw, const AdjArgs& args, hipStream_t stream) {
+  hipLaunchKernelGGL((mp_adjoint_kernel<S, EXT>), dim3(nw), dim3(64), 0, stream, args);
+  return hipGetLastError();
 }
 
 }  // namespace shield

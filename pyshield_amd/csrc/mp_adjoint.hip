@@ -6,6 +6,11 @@
 
 namespace shield {
 
+// the EXT instances are compiled in mp_adjoint_ext.hip
+extern template hipError_t launch_adjoint_kernel<1, true>(unsigned, const AdjArgs&, hipStream_t);
+extern template hipError_t launch_adjoint_kernel<2, true>(unsigned, const AdjArgs&, hipStream_t);
+extern template hipError_t launch_adjoint_kernel<3, true>(unsigned, const AdjArgs&, hipStream_t);
+
 // Device scratch of shield_mp_adjoint_noalloc: the tapes and parameter
 // partial sums of as many wavefronts as fit (SHIELD_MP_ADJOINT_SCRATCH_BYTES,
 // each GPU holds its own copy for the life of the process).  Host code never
@@ -45,9 +50,9 @@ AdjLayout adjoint_layout(int64_t nk) {
 
 int adjoint_cols_per_wave(int64_t nk) { return adjoint_layout(nk).P; }
 
-// doubles of tape per wavefront
-int64_t adjoint_tape_doubles(int64_t nk, int ntimes) {
-  return (int64_t)ntimes * REC_FIELDS * 64 * adjoint_layout(nk).S;
+// doubles of tape per wavefront (with do_sedi_w, SW_FIELDS more per sub-step)
+int64_t adjoint_tape_doubles(int64_t nk, int ntimes, bool sedi_w) {
+  return (int64_t)ntimes * (REC_FIELDS + (sedi_w ? SW_FIELDS : 0)) * 64 * adjoint_layout(nk).S;
 }
 
 hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
@@ -55,7 +60,7 @@ hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, 
   const int64_t ncol = sh.ni * sh.nj;
   const AdjLayout lay = adjoint_layout(sh.nk);
   const int64_t groups = (ncol + lay.P - 1) / lay.P;
-  const int64_t tape = adjoint_tape_doubles(sh.nk, p.ntimes);
+  const int64_t tape = adjoint_tape_doubles(sh.nk, p.ntimes, p.do_sedi_w != 0);
   const int64_t per_wave = (tape + AP_N) * (int64_t)sizeof(double);
   int64_t nw = (int64_t)(ws_bytes / (size_t)per_wave);
   if (nw > groups) nw = groups;
@@ -63,14 +68,17 @@ hipError_t launch_adjoint(const shield_mp_shape& sh, const shield_mp_params& p, 
   double* base = static_cast<double*>(ws);
   double* part = base + nw * tape;
   const AdjArgs args{sh, p, make_coef<double>(p), f, a, base, tape, part, lay.P};
-  const dim3 grid((unsigned)nw), block(64);
+  const unsigned grid = (unsigned)nw;
+  // fix_negative, the ice slope limiter and the w transport have an instance
+  // of their own
+  const bool ext = p.do_sedi_w || p.nml.z_slope_ice || p.nml.fix_negative;
+  hipError_t e;
   if (lay.S == 1)
-    hipLaunchKernelGGL(mp_adjoint_kernel<1>, grid, block, 0, stream, args);
+    e = ext ? launch_adjoint_kernel<1, true>(grid, args, stream) : launch_adjoint_kernel<1, false>(grid, args, stream);
   else if (lay.S == 2)
-    hipLaunchKernelGGL(mp_adjoint_kernel<2>, grid, block, 0, stream, args);
+    e = ext ? launch_adjoint_kernel<2, true>(grid, args, stream) : launch_adjoint_kernel<2, false>(grid, args, stream);
   else
-    hipLaunchKernelGGL(mp_adjoint_kernel<3>, grid, block, 0, stream, args);
-  hipError_t e = hipGetLastError();
+    e = ext ? launch_adjoint_kernel<3, true>(grid, args, stream) : launch_adjoint_kernel<3, false>(grid, args, stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(adj_param_reduce, dim3(1), dim3(64), 0, stream, part, nw, a.params);
   return hipGetLastError();

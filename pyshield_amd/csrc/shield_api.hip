@@ -12,7 +12,7 @@ struct LayoutChoice {
   int S, P;
 };
 LayoutChoice choose_layout(int64_t nz, int mode, int ntimes, bool fused);
-int64_t adjoint_tape_doubles(int64_t nk, int ntimes);
+int64_t adjoint_tape_doubles(int64_t nk, int ntimes, bool sedi_w);
 int adjoint_cols_per_wave(int64_t nk);
 hipError_t launch_adjoint_noalloc(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
                                   const shield_mp_adjoint_fields& a, hipStream_t stream);
@@ -259,7 +259,7 @@ size_t shield_mp_adjoint_workspace_bytes(const shield_mp_shape* shape, const shi
   const int64_t ncol = shape->ni * shape->nj, P = adjoint_cols_per_wave(shape->nk);
   const int64_t groups = (ncol + P - 1) / P;
   if (waves > groups) waves = groups < 1 ? 1 : groups;
-  return (size_t)waves * (size_t)(adjoint_tape_doubles(shape->nk, p->ntimes) + SHIELD_MP_NPARAMS) * sizeof(double);
+  return (size_t)waves * (size_t)(adjoint_tape_doubles(shape->nk, p->ntimes, p->do_sedi_w != 0) + SHIELD_MP_NPARAMS) * sizeof(double);
 }
 
 // Checks of shield_mp_adjoint and shield_mp_adjoint_noalloc (ws may be NULL
@@ -272,8 +272,7 @@ static int check_adjoint(const shield_mp_shape* shape, const shield_mp_params* p
   if (rc != SHIELD_OK) return rc;
   if (!f->qaz && !p->nml.reset_qaz) return SHIELD_ERR_NULL;
   if (shape->nk > 128) return SHIELD_ERR_SHAPE;
-  if (p->do_sedi_w || (p->nml.de_ice && p->accumulate))
-    return SHIELD_ERR_ADJ_SWITCH;
+  if (p->nml.de_ice && p->accumulate) return SHIELD_ERR_ADJ_SWITCH;
   if (*empty) return SHIELD_OK;
   const void* need[] = {a->pt_dt, a->qv_dt, a->ql_dt, a->qr_dt, a->qi_dt, a->qs_dt, a->qg_dt, a->qa_dt, a->udt,
                         a->vdt,   a->pt,    a->qv,    a->ql,    a->qr,    a->qi,    a->qs,    a->qg,    a->qa,
@@ -282,12 +281,15 @@ static int check_adjoint(const shield_mp_shape* shape, const shield_mp_params* p
     if (!q) return SHIELD_ERR_NULL;
   // the written cotangents, the parameter gradient and the workspace are
   // distinct from each other and from everything the kernel reads
+  const bool sw = p->do_sedi_w != 0;
+  if (sw && !f->w) return SHIELD_ERR_NULL;
   const void* wr[] = {a->pt, a->qv, a->ql, a->qr, a->qi, a->qs, a->qg, a->qa, a->ua, a->va, a->delp, a->delz,
-                      a->qaz, a->params, ws};
+                      a->qaz, a->params, ws, sw ? a->w : nullptr};
   const void* rd[] = {f->pt,    f->qv,    f->ql,    f->qr,    f->qi,    f->qs,    f->qg,    f->qa,
                       f->ua,    f->va,    f->delp,  f->delz,  f->land,  f->area,  f->qaz,   a->pt_dt,
                       a->qv_dt, a->ql_dt, a->qr_dt, a->qi_dt, a->qs_dt, a->qg_dt, a->qa_dt, a->udt,
-                      a->vdt,   a->rain,  a->snow,  a->ice,   a->graupel};
+                      a->vdt,   a->rain,  a->snow,  a->ice,   a->graupel, sw ? f->w : nullptr,
+                      sw ? a->w_after : nullptr};
   const int nwr = (int)(sizeof(wr) / sizeof(wr[0]));
   for (int i = 0; i < nwr; ++i) {
     if (!wr[i]) continue;

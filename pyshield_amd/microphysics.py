@@ -308,9 +308,10 @@ def adjoint(fields: dict, cotangents: dict, grads: dict, params, workspace=None,
     """Reverse-mode adjoint of one FP64 forward call (shield_mp_adjoint).
     ``fields``: the forward's inputs (as for forward(); output entries are not
     used).  ``cotangents``: cotangents of the outputs keyed by AD_OUT_NAMES
-    (the 2D precipitation entries may be missing).  ``grads``: tensors that
-    receive the input cotangents, keyed by AD_STATE_NAMES and "qaz"
-    (optional), and "params" (6 float64 values).  All 3D and 2D tensors share
+    (the 2D precipitation entries may be missing) and, with do_sedi_w, "w"
+    (optional, the vertical velocity the call rewrites).  ``grads``: tensors
+    that receive the input cotangents, keyed by AD_STATE_NAMES and "qaz"
+    (optional), "w" (optional, do_sedi_w) and "params" (6 float64 values).  All 3D and 2D tensors share
     the strides of fields["pt"] and fields["land"].  Returns the workspace
     (reusable for the next call)."""
     shape, f = _mp_args(fields)
@@ -331,6 +332,15 @@ def adjoint(fields: dict, cotangents: dict, grads: dict, params, workspace=None,
             if t.stride() != ref3.stride():
                 raise ValueError(f"gradient {n} must have the strides of the forward's fields")
         setattr(a, "g_" + n, t.data_ptr() if t is not None else None)
+    # do_sedi_w: the cotangent of w after the call (None: zero) and the
+    # tensor that receives that of w before it (None: not written)
+    for key, attr, src in (("w", "d_w", cotangents), ("w", "g_w", grads)):
+        t = src.get(key)
+        if t is not None:
+            _check3(f"{attr[0]}_w", t, tuple(ref3.shape))
+            if t.stride() != ref3.stride():
+                raise ValueError("w's cotangents must have the strides of the forward's fields")
+        setattr(a, attr, t.data_ptr() if t is not None else None)
     gp = grads["params"]
     if gp.dtype != torch.float64 or gp.numel() != 6 or not gp.is_contiguous() or gp.device != ref3.device:
         raise ValueError("grads['params'] must be a contiguous float64 tensor of 6 values on the fields' device")

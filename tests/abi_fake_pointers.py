@@ -152,11 +152,12 @@ def check_adjoint_rejects_before_launch():
     a2 = _fake_adjoint()
     a2.d_pt_dt = None
     assert call(a=a2) == -1
-    # switches the adjoint does not differentiate
-    for field, value in (("do_sedi_w", 1),):
-        p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
-        setattr(p2, field, value)
-        assert call(p=p2) == -9
+    # do_sedi_w needs w
+    p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
+    p2.do_sedi_w = 1
+    f2 = _fake_fields()
+    f2.w = None
+    assert call(p=p2, f=f2) == -1
     # de_ice acts only on accumulated tendencies, which the adjoint does not take
     p2 = _lib.setup(PhysicsConfig().mp_namelist(), 150.0)
     p2.nml.de_ice = 1

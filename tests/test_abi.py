@@ -49,8 +49,8 @@ def test_struct_sizes_match_layout():
     assert ctypes.sizeof(_lib.ShieldMpNamelist) == 37 * 8 + 20 * 4
     assert ctypes.sizeof(_lib.ShieldMpShape) == 7 * 8
     assert ctypes.sizeof(_lib.ShieldMpFields) == len(_lib.FIELD_NAMES) * 8
-    # 14 output cotangents, 13 input cotangents, the parameter gradient
-    assert ctypes.sizeof(_lib.ShieldMpAdjointFields) == (14 + 13 + 1) * 8
+    # 14 output cotangents, 13 input cotangents, the parameter gradient, w's two
+    assert ctypes.sizeof(_lib.ShieldMpAdjointFields) == (14 + 13 + 1 + 2) * 8
     # 21 pointers, two int64 strides, three doubles
     assert ctypes.sizeof(_lib.ShieldUpdateFields) == 21 * 8 + 2 * 8 + 3 * 8
     # 31 pointers, two int64 strides, one double
