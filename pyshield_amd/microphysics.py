@@ -405,6 +405,8 @@ class MicrophysicsFunction(torch.autograd.Function):
         ref = state[0]
         f = {n: t.detach() for n, t in zip(AD_STATE_NAMES, state)}
         f.update(land=land, area=area, qaz=qaz)
+        # the forward's w (do_sedi_w), zero: w reaches none of the outputs
+        f["w"] = _strided_like(ref).zero_()
         cot = {}
         for n, g in zip(AD_OUT_NAMES, gout):
             like = land if n in FIELDS_2D else ref
