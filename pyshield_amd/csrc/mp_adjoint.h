@@ -284,8 +284,8 @@ __device__ __forceinline__ void fall_adj(const Lay<S>& L, Shared<S>& sh, const d
     if (fall[s] && L.v[s]) {
       const double qn = qm[s] * dz[s] / dp1[s];
       g[s] = qbar[s] * dz[s] / dp1[s] + fluxbar[s] * dd[s];
-      dzbar[s] = qbar[s] * qm[s] / dp1[s];
-      dp1bar[s] -= qbar[s] * qn / dp1[s];
+      dzbar[s] = qbar[s] * qm[s] * rcp64(dp1[s]);
+      dp1bar[s] -= qbar[s] * qn * rcp64(dp1[s]);
       ddbar[s] = fluxbar[s] * qm[s];
     } else {
       g[s] = dzbar[s] = ddbar[s] = 0.0;
@@ -366,17 +366,17 @@ __device__ __forceinline__ void heat_adj(const Lay<S>& L, Shared<S>& sh, const R
     double cvnbar, dgzbar;
     if (L.k[s] == 0) {
       xb.v[6][s] += m;
-      fluxbar[s] += m * dgz[s] * cvn[s] / (D[s] * D[s]);
-      dgzbar = m * flux[s] / D[s];
-      cvnbar = -m * flux[s] * dgz[s] / (D[s] * D[s]);
+      fluxbar[s] += m * dgz[s] * cvn[s] * rcp64(D[s] * D[s]);
+      dgzbar = m * flux[s] * rcp64(D[s]);
+      cvnbar = -m * flux[s] * dgz[s] * rcp64(D[s] * D[s]);
     } else {
-      xb.v[6][s] += m * (cvn[s] + cq * (flux[s] - fp[s])) / D[s];
-      fluxbar[s] += m * (cq * tz + dgz[s] - cq * tn[s]) / D[s];
-      fpb[s] = m * (-cq * tz + dgz[s] + cq * tnp[s]) / D[s];
-      cvnbar = m * (tz - tn[s]) / D[s];
-      dgzbar = m * (fp[s] + flux[s]) / D[s];
+      xb.v[6][s] += m * (cvn[s] + cq * (flux[s] - fp[s])) * rcp64(D[s]);
+      fluxbar[s] += m * (cq * tz + dgz[s] - cq * tn[s]) * rcp64(D[s]);
+      fpb[s] = m * (-cq * tz + dgz[s] + cq * tnp[s]) * rcp64(D[s]);
+      cvnbar = m * (tz - tn[s]) * rcp64(D[s]);
+      dgzbar = m * (fp[s] + flux[s]) * rcp64(D[s]);
     }
-    dp1bar[s] += cvnbar * cvn[s] / dp1[s];
+    dp1bar[s] += cvnbar * cvn[s] * rcp64(dp1[s]);
     xb.v[0][s] += cvnbar * dp1[s] * CV_VAP;
     xb.v[1][s] += cvnbar * dp1[s] * C_LIQ;
     xb.v[2][s] += cvnbar * dp1[s] * C_LIQ;
@@ -470,7 +470,7 @@ __device__ __forceinline__ void w_transport_adj(const Lay<S>& L, Shared<S>& sh, 
     const double N = top ? dm[s] * w0[s] + fl[s] * vt[s] : dm[s] * w0[s] - fp[s] * vp[s] + fl[s] * vt[s];
     const double D = top ? dm[s] - fl[s] : dm[s] + fp[s] - fl[s];
     const double wn = N / D;
-    const double Nb = wb[s] / D, Db = -wb[s] * wn / D;
+    const double Nb = wb[s] * rcp64(D), Db = -wb[s] * wn * rcp64(D);
     wb[s] = Nb * dm[s];
     dmb[s] += Nb * w0[s] + Db;
     fluxb[s] += Nb * vt[s] - Db;
@@ -930,7 +930,7 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
       w_transport_adj(L, sh, dm, flux, vt, fall, w0, G.wb, dmb, fluxbar, vtbar);
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        dp1bar[s] += dmb[s] * (dm[s] / dp1[s]);
+        dp1bar[s] += dmb[s] * (dm[s] * rcp64(dp1[s]));
         const double g = dmb[s] * dp1[s];
         G.xb.v[0][s] += g;
         G.xb.v[1][s] += g;
@@ -973,12 +973,12 @@ __device__ __forceinline__ void warm_rain_adj(const Lay<S>& L, AdjShared<S>& A, 
       const int32_t o = Gl.off[s];
       const double t0 = f.pt[o], dz0 = f.delz[o], tz = rB.z[Z_TZ][s];
       const double dz1 = rB.z[Z_DZ1][s], den = rB.z[Z_DEN][s], denfac = rB.z[Z_DENFAC][s];
-      const double denb = db - dfb * 0.5 * denfac / den;
-      const double dz1b = dzb - denb * den / dz1;
-      G.xb.v[6][s] += dz1b * dz0 / t0;
-      abar(A, AB_DZ0, s) += dz1b * tz / t0;
-      abar(A, AB_T0, s) -= dz1b * dz1 / t0;
-      abar(A, AB_DP1, s) += denb * den / dp1[s];
+      const double denb = db - dfb * 0.5 * denfac * rcp64(den);
+      const double dz1b = dzb - denb * den * rcp64(dz1);
+      G.xb.v[6][s] += dz1b * dz0 * rcp64(t0);
+      abar(A, AB_DZ0, s) += dz1b * tz * rcp64(t0);
+      abar(A, AB_T0, s) -= dz1b * dz1 * rcp64(t0);
+      abar(A, AB_DP1, s) += denb * den * rcp64(dp1[s]);
     }
   }
 }
@@ -1087,7 +1087,7 @@ __device__ __forceinline__ void sed_adj(const Lay<S>& L, AdjShared<S>& A, AdjReg
     w_transport_adj(L, sh, dm, fl, vt, fall, wb0, G.wb, dmb, fb, vb);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      dp1bar[s] += dmb[s] * (dm[s] / dp1[s]);
+      dp1bar[s] += dmb[s] * (dm[s] * rcp64(dp1[s]));
       const double g = dmb[s] * dp1[s];
       dmb[s] = g;
 #pragma unroll
@@ -1218,12 +1218,12 @@ __device__ __forceinline__ void fields_update_adj(const shield_mp_params& P, dou
     omqb += in.gq[q] * rdt * (xq[q] - in.qin[q] * omq0);
     omq0b -= in.gq[q] * rdt * in.qin[q] * omq;
   }
-  b.dp1 += omqb / in.dp0 - omq0b * omq0 / in.dp1;
-  b.dp0 += omq0b / in.dp1 - omqb * omq / in.dp0;
+  b.dp1 += omqb * rcp64(in.dp0) - omq0b * omq0 * rcp64(in.dp1);
+  b.dp0 += omq0b * rcp64(in.dp1) - omqb * omq * rcp64(in.dp0);
   const double cvm = P.c_air + x.qv * P.c_vap + (x.qr + x.ql) * C_LIQ + (x.qi + x.qs + x.qg) * C_ICE;
-  yb[6] += in.gt * rdt * cvm / CP_AIR;
-  b.pt -= in.gt * rdt * cvm / CP_AIR;
-  const double cvmb = in.gt * rdt * (x.tz - in.pt0) / CP_AIR;
+  yb[6] += in.gt * rdt * cvm * (1.0 / CP_AIR);
+  b.pt -= in.gt * rdt * cvm * (1.0 / CP_AIR);
+  const double cvmb = in.gt * rdt * (x.tz - in.pt0) * (1.0 / CP_AIR);
   yb[0] += cvmb * P.c_vap;
   yb[1] += cvmb * C_LIQ;
   yb[2] += cvmb * C_LIQ;
@@ -1231,7 +1231,7 @@ __device__ __forceinline__ void fields_update_adj(const shield_mp_params& P, dou
   yb[4] += cvmb * C_ICE;
   yb[5] += cvmb * C_ICE;
   if (!P.nml.do_qa) {
-    yb[7] += in.ga * rdt / P.ntimes;
+    yb[7] += in.ga * rdt * rcp64(P.ntimes);
     b.qa -= in.ga * rdt;
   }
 }
@@ -1326,9 +1326,9 @@ __device__ __forceinline__ void icloud_adj_slope(const Lay<S>& L, AdjShared<S>& 
     abar(A, AB_VTS, s) += ib.vts;
     G.gp[AP_QI_GEN] += ib.qi_gen;
     // p1 = -dp1 / (g dz0) rdgas t0
-    abar(A, AB_DP1, s) += ib.p1 * p1[s] / dp1[s];
-    abar(A, AB_DZ0, s) -= ib.p1 * p1[s] / f.delz[o];
-    abar(A, AB_T0, s) += ib.p1 * p1[s] / f.pt[o];
+    abar(A, AB_DP1, s) += ib.p1 * p1[s] * rcp64(dp1[s]);
+    abar(A, AB_DZ0, s) -= ib.p1 * p1[s] * rcp64(f.delz[o]);
+    abar(A, AB_T0, s) += ib.p1 * p1[s] * rcp64(f.pt[o]);
   });
   // the limiter's transpose: dl[k] = slope_point(q[k-1], q[k], q[k+1]) at
   // interior levels
@@ -1511,9 +1511,9 @@ __device__ __forceinline__ void icloud_adj(const Lay<S>& L, AdjShared<S>& A, Adj
     A.bar[AB_VTS][id] += ib.vts;
     G.gp[AP_QI_GEN] += ib.qi_gen;
     // p1 = -dp1 / (g dz0) rdgas t0
-    A.bar[AB_DP1][id] += ib.p1 * p1 / dp1;
-    A.bar[AB_DZ0][id] -= ib.p1 * p1 / dz0;
-    A.bar[AB_T0][id] += ib.p1 * p1 / t0;
+    A.bar[AB_DP1][id] += ib.p1 * p1 * rcp64(dp1);
+    A.bar[AB_DZ0][id] -= ib.p1 * p1 * rcp64(dz0);
+    A.bar[AB_T0][id] += ib.p1 * p1 * rcp64(t0);
   });
   __syncthreads();
   put_cells(xs[0], sh.buf); put_cells(xs[1], b1); put_cells(xs[2], b2); put_cells(xs[3], b3);
@@ -1576,8 +1576,8 @@ SH_FN void fix_negative_cell_vjp(const shield_mp_params& P, const Cell& x0, doub
     const double lcpk = (P.lv00 + P.d0_vap * tz0) / cvm;
     ql4b = b[0] - b[6] * lcpk;
     const double lcpkb = -b[6] * ql4;
-    tz0b += lcpkb * P.d0_vap / cvm;
-    cvmb -= lcpkb * lcpk / cvm;
+    tz0b += lcpkb * P.d0_vap * rcp64(cvm);
+    cvmb -= lcpkb * lcpk * rcp64(cvm);
   }
   // rain from cloud water
   const double ql0b = ql4b, qr3b = c4 ? ql4b : b[2];
@@ -1588,8 +1588,8 @@ SH_FN void fix_negative_cell_vjp(const shield_mp_params& P, const Cell& x0, doub
     const double icpk = (LI00 + DC_ICE * tz0) / cvm;
     qg2b = qr3b - tz3b * icpk;
     const double icpkb = -tz3b * qg2;
-    tz0b += icpkb * DC_ICE / cvm;
-    cvmb -= icpkb * icpk / cvm;
+    tz0b += icpkb * DC_ICE * rcp64(cvm);
+    cvmb -= icpkb * icpk * rcp64(cvm);
   }
   tz0b += tz3b;
   // snow from graupel, ice from snow
@@ -1617,9 +1617,9 @@ SH_FN void fix_negative_column_vjp(double* q, const double* dp, int nz, double* 
   if ((qkb < 0.0) && (qka > 0.0)) {
     const double a = -qkb * dp[kb], c = qka * dp[ka];
     const double dq = mmin(a, c);
-    const double dqb = -qb[ka] / dp[ka] + qb[kb] / dp[kb];
-    dpb[ka] += qb[ka] * dq / (dp[ka] * dp[ka]);
-    dpb[kb] -= qb[kb] * dq / (dp[kb] * dp[kb]);
+    const double dqb = -qb[ka] * rcp64(dp[ka]) + qb[kb] * rcp64(dp[kb]);
+    dpb[ka] += qb[ka] * dq * rcp64(dp[ka] * dp[ka]);
+    dpb[kb] -= qb[kb] * dq * rcp64(dp[kb] * dp[kb]);
     if (a < c) {
       qb[kb] -= dqb * dp[kb];
       dpb[kb] -= dqb * qkb;
@@ -1636,8 +1636,8 @@ SH_FN void fix_negative_column_vjp(double* q, const double* dp, int nz, double* 
     if (q[k - 1] < 0.0) {
       const double r = dp[k - 1] / dp[k];
       qb[k - 1] += qb[k] * r;
-      dpb[k - 1] += qb[k] * q[k - 1] / dp[k];
-      dpb[k] -= qb[k] * q[k - 1] * r / dp[k];
+      dpb[k - 1] += qb[k] * q[k - 1] * rcp64(dp[k]);
+      dpb[k] -= qb[k] * q[k - 1] * r * rcp64(dp[k]);
     }
   }
 }
@@ -1946,7 +1946,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
       if (!L.v[s]) continue;
       const int cl = L.cl[s];
       const double ccn = sh.colv[CV_CCN][cl], cpraut = sh.colv[CV_CPRAUT][cl];
-      const double ccnbar = A.colbar[CB_CCN][cl] - A.colbar[CB_CPRAUT][cl] * cpraut / (3.0 * ccn);
+      const double ccnbar = A.colbar[CB_CCN][cl] - A.colbar[CB_CPRAUT][cl] * cpraut * rcp64(3.0 * ccn);
       const int32_t o = Gl.off[s];
       const double dp0 = f.delp[o], qv = f.qv[o], dp1 = X.dp1[s];
       const double omq = dp0 / dp1;
@@ -1958,16 +1958,16 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
         omqb += G.xb.v[q][s] * qin[q];
       }
       nb.pt[s] += G.xb.v[6][s] + abar(A, AB_T0, s);
-      double dp1b = abar(A, AB_DP1, s) - omqb * omq / dp1;
-      nb.dp0[s] += omqb / dp1;
+      double dp1b = abar(A, AB_DP1, s) - omqb * omq * rcp64(dp1);
+      nb.dp0[s] += omqb * rcp64(dp1);
       double dz0b = abar(A, AB_DZ0, s);
       if (L.kb[s] == 0 && P.use_ccn) {
         // ccn = ccn0 rdgas tz / p1, p1 = -dp1 / (g dz0) rdgas t0, with tz = t0 = pt
-        nb.pt[s] += ccnbar * ccn / f.pt[o];
-        const double p1b = -ccnbar * ccn / p1_b[s];
-        dp1b += p1b * p1_b[s] / dp1;
-        dz0b -= p1b * p1_b[s] / f.delz[o];
-        nb.pt[s] += p1b * p1_b[s] / f.pt[o];
+        nb.pt[s] += ccnbar * ccn * rcp64(f.pt[o]);
+        const double p1b = -ccnbar * ccn * rcp64(p1_b[s]);
+        dp1b += p1b * p1_b[s] * rcp64(dp1);
+        dz0b -= p1b * p1_b[s] * rcp64(f.delz[o]);
+        nb.pt[s] += p1b * p1_b[s] * rcp64(f.pt[o]);
       }
       nb.dp0[s] += dp1b * (1.0 - qv);
       nb.q[0][s] -= dp1b * dp0;

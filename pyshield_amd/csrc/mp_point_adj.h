@@ -17,6 +17,12 @@
 // every branch is the one the forward took) and then propagates the
 // cotangent of its output state to its entry state.
 //
+// Divisions: the re-evaluated forward values keep the IEEE division of the
+// process code, so they and their branches are the forward's bit for bit;
+// the cotangent expressions multiply by rcp64 (v_rcp_f64 and one Newton step,
+// 2.2e-15 relative), a third of the IEEE sequence's instructions and a shorter
+// dependent chain.  The same holds for the column reverses of mp_adjoint.h.
+//
 // Derivative conventions are those of the dual-number path and of the
 // reference's select-based min / max: min(a, b) follows a where a < b and b
 // otherwise; max(a, b) follows a where a > b; dim(x, y) = x - y where that is
@@ -64,9 +70,9 @@ SH_FN double ic_cvm(const Coef<double>& c, double qv, double q_liq, double q_sol
 SH_FN double heat_rev(const Coef<double>& c, double sg, double a, double L, double cvm1, IcX& xb, double& lb) {
   const double tb = xb.tz;
   const double dT = (a * L) / cvm1;
-  const double ab = sg * tb * L / cvm1;
-  lb += sg * tb * a / cvm1;
-  const double cb = xb.cvm - sg * tb * dT / cvm1;
+  const double ab = sg * tb * L * rcp64(cvm1);
+  lb += sg * tb * a * rcp64(cvm1);
+  const double cb = xb.cvm - sg * tb * dT * rcp64(cvm1);
   xb.qv += cb * c.c_vap_d;
   xb.q_liq += cb * C_LIQ;
   xb.q_sol += cb * C_ICE;
@@ -90,7 +96,7 @@ SH_FN void sat_rev(bool ice, double ta, double den, double q, double dqdt, doubl
       active = ta <= TICE + 102.0;
     }
   }
-  denb -= (qb * q + dqdtb * dqdt) / den;
+  denb -= (qb * q + dqdtb * dqdt) * rcp64(den);
   if (active) {
     const double g = (d + l / ta) / (RVGAS * ta);
     const double qt = q * (g - 1.0 / ta);
@@ -116,11 +122,11 @@ SH_FN void acr3d_rev(double v1, double v2, double q1, double q2, double cc, doub
   const double adb = Pb * cc * q1 * s2;
   q1b += Pb * cc * ad * s2;
   double s2b = Pb * cc * ad * q1 + Qb * a1 * st1;
-  const double t1b = Qb * a0 + Qb * a1 * s2 * 0.5 / st1;
-  const double s1b = Qb * a2 + s2b * 0.5 / s2;
-  q1b += t1b * 0.5 / t1 * rho;
-  rhob += t1b * 0.5 / t1 * q1 + s1b * 0.5 / s1 * q2;
-  q2b += s1b * 0.5 / s1 * rho;
+  const double t1b = Qb * a0 + Qb * a1 * s2 * 0.5 * rcp64(st1);
+  const double s1b = Qb * a2 + s2b * 0.5 * rcp64(s2);
+  q1b += t1b * 0.5 * rcp64(t1) * rho;
+  rhob += t1b * 0.5 * rcp64(t1) * q1 + s1b * 0.5 * rcp64(s1) * q2;
+  q2b += s1b * 0.5 * rcp64(s1) * rho;
   v1b += adb * sg;
   v2b -= adb * sg;
 }
@@ -128,8 +134,8 @@ SH_FN void acr3d_rev(double v1, double v2, double q1, double q2, double cc, doub
 // x / y with cotangent ob: adds to xb, yb
 SH_FN void div_rev(double x, double y, double ob, double& xb, double& yb) {
   const double q = x / y;
-  xb += ob / y;
-  yb -= ob * q / y;
+  xb += ob * rcp64(y);
+  yb -= ob * q * rcp64(y);
 }
 
 // the derived heats of a block entry: lhl, lhi, lcpk = lhl / cvm,
@@ -147,9 +153,9 @@ SH_FN IcHeat ic_heat(const Coef<double>& c, const IcX& x) {
 }
 SH_FN void ic_heat_rev(const Coef<double>& c, const IcX& x, const IcHeat& h, double lhlb, double lhib, double lcpkb,
                        double icpkb, IcX& xb) {
-  lhlb += lcpkb / x.cvm;
-  lhib += icpkb / x.cvm;
-  xb.cvm -= (lcpkb * h.lcpk + icpkb * h.icpk) / x.cvm;
+  lhlb += lcpkb * rcp64(x.cvm);
+  lhib += icpkb * rcp64(x.cvm);
+  xb.cvm -= (lcpkb * h.lcpk + icpkb * h.icpk) * rcp64(x.cvm);
   xb.tz += lhlb * c.d0_vap_d + lhib * DC_ICE;
 }
 
@@ -214,8 +220,8 @@ SH_FN void blk_melt_freeze_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib, b
     if (x.qi < a) {
       xb.qi += meltb;
     } else {
-      xb.tz += meltb * c.fac_imlt / icpk;
-      icpkb -= meltb * a / icpk;
+      xb.tz += meltb * c.fac_imlt * rcp64(icpk);
+      icpkb -= meltb * a * rcp64(icpk);
     }
   } else if (br == 2) {
     const double dtmp = k.t_wfr - x.tz;
@@ -235,23 +241,23 @@ SH_FN void blk_melt_freeze_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib, b
     } else if (qi_crt - x.qi > 0.0) {
       const double qcb = tmpb;
       xb.qi -= qcb;
-      ib.qi_gen += qcb * mq / k.in.den;
-      ib.den -= qcb * qi_crt / k.in.den;
-      if (!(c.qi_lim < xq)) xb.tz -= qcb * c.qi_gen / k.in.den * 0.1;
+      ib.qi_gen += qcb * mq * rcp64(k.in.den);
+      ib.den -= qcb * qi_crt * rcp64(k.in.den);
+      if (!(c.qi_lim < xq)) xb.tz -= qcb * c.qi_gen * rcp64(k.in.den) * 0.1;
     }
     double dtmpb = 0.0;
     if (s1 < s2) {
       xb.ql += sinkb * factor;
-      if (!(1.0 < r8)) dtmpb += sinkb * x.ql / DT_FR;
+      if (!(1.0 < r8)) dtmpb += sinkb * x.ql * (1.0 / DT_FR);
     } else {
-      dtmpb += sinkb / icpk;
-      icpkb -= sinkb * s2 / icpk;
+      dtmpb += sinkb * rcp64(icpk);
+      icpkb -= sinkb * s2 * rcp64(icpk);
     }
     xb.tz -= dtmpb;
   }
   // icpk = lhi / cvm0; cvm0 = cvm(qv, q_liq0, q_sol0)
-  lhib += icpkb / cvm0;
-  const double cb = xb.cvm - icpkb * icpk / cvm0;
+  lhib += icpkb * rcp64(cvm0);
+  const double cb = xb.cvm - icpkb * icpk * rcp64(cvm0);
   xb.qv += cb * c.c_vap_d;
   const double qlb = xb.q_liq + cb * C_LIQ, qsb = xb.q_sol + cb * C_ICE;
   xb.ql += qlb;
@@ -342,8 +348,8 @@ SH_FN void blk_w1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   if (b1 < b2) {
     b1b = innerb;
   } else {
-    xb.tz += innerb / icpk;
-    icpkb -= innerb * b2 / icpk;
+    xb.tz += innerb * rcp64(icpk);
+    icpkb -= innerb * b2 * rcp64(icpk);
   }
   const double psmltb = b1b * c.dts, pracsb = b1b * c.dts;
   const double smb = (0.0 > sm) ? 0.0 : psmltb;
@@ -356,11 +362,11 @@ SH_FN void blk_w1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   const double Ab = smb * B, Bb = smb * A;
   tcb += smb * ck(c, kk + 4) * (psacw + psacr);
   const double psacwb = smb * ck(c, kk + 4) * tc, psacrb = psacwb;
-  tcb += Ab * ck(c, kk) / den;
-  ib.den -= Ab * (ck(c, kk) * tc / den) / den;
+  tcb += Ab * ck(c, kk) * rcp64(den);
+  ib.den -= Ab * (ck(c, kk) * tc * rcp64(den)) * rcp64(den);
   dqs0b -= Ab * ck(c, kk + 1);
-  const double qsrhob = Bb * (ck(c, kk + 2) * 0.5 / sq + ck(c, kk + 3) * 0.65625 * pwq / qsrho * sdf);
-  ib.denfac += Bb * ck(c, kk + 3) * pwq * 0.5 / sdf;
+  const double qsrhob = Bb * (ck(c, kk + 2) * 0.5 * rcp64(sq) + ck(c, kk + 3) * 0.65625 * pwq * rcp64(qsrho) * sdf);
+  ib.denfac += Bb * ck(c, kk + 3) * pwq * 0.5 * rcp64(sdf);
   xb.qs += qsrhob * den;
   ib.den += qsrhob * x.qs;
   if (x.qr > QRMIN) {
@@ -376,18 +382,18 @@ SH_FN void blk_w1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     const double pe = pw_el<double>(x.qs * den, 0.8125);
     const double factor = denfac * c.csacw * pe;
     const double D = 1.0 + c.dts * factor;
-    const double fb = psacwb * x.ql / (D * D);
-    xb.ql += psacwb * factor / D;
+    const double fb = psacwb * x.ql * rcp64(D * D);
+    xb.ql += psacwb * factor * rcp64(D);
     ib.denfac += fb * c.csacw * pe;
-    const double qdb = fb * denfac * c.csacw * 0.8125 * pe / (x.qs * den);
+    const double qdb = fb * denfac * c.csacw * 0.8125 * pe * rcp64(x.qs * den);
     xb.qs += qdb * den;
     ib.den += qdb * x.qs;
   }
-  ib.p1 -= dqs0b * (c.ces0 / k.in.p1) / k.in.p1;
+  ib.p1 -= dqs0b * (c.ces0 * rcp64(k.in.p1)) * rcp64(k.in.p1);
   xb.qv -= dqs0b;
   xb.tz += tcb;
-  lhib += icpkb / x.cvm;
-  xb.cvm -= icpkb * icpk / x.cvm;
+  lhib += icpkb * rcp64(x.cvm);
+  xb.cvm -= icpkb * icpk * rcp64(x.cvm);
   xb.tz += lhib * DC_ICE;
 }
 
@@ -455,8 +461,8 @@ SH_FN void blk_w2_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   if (x.qg < b2) {
     xb.qg += m1b;
   } else {
-    xb.tz += m1b / icpk;
-    icpkb -= m1b * b2 / icpk;
+    xb.tz += m1b * rcp64(icpk);
+    icpkb -= m1b * b2 * rcp64(icpk);
   }
   const double gmb = ((0.0 > pgmlt) ? 0.0 : m0b) * c.dts;
   // gmlt (microphysics_funcs.py:172-179)
@@ -467,20 +473,20 @@ SH_FN void blk_w2_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   const double Ab = gmb * B, Bb = gmb * A;
   tcb += gmb * ck(c, kk + 4) * (pgacw + pgacr);
   const double pgacwb = gmb * ck(c, kk + 4) * tc, pgacrb = pgacwb;
-  tcb += Ab * ck(c, kk) / den;
-  ib.den -= Ab * (ck(c, kk) * tc / den) / den;
+  tcb += Ab * ck(c, kk) * rcp64(den);
+  ib.den -= Ab * (ck(c, kk) * tc * rcp64(den)) * rcp64(den);
   dqs0b -= Ab * ck(c, kk + 1);
-  double qdenb = Bb * (ck(c, kk + 2) * 0.5 / sq2 + ck(c, kk + 3) * 0.6875 * pq / qden / pr);
-  ib.den -= Bb * ck(c, kk + 3) * (pq / pr) * 0.25 / den;
+  double qdenb = Bb * (ck(c, kk + 2) * 0.5 * rcp64(sq2) + ck(c, kk + 3) * 0.6875 * pq * rcp64(qden) * rcp64(pr));
+  ib.den -= Bb * ck(c, kk + 3) * (pq * rcp64(pr)) * 0.25 * rcp64(den);
   if (x.ql > QRMIN) {
     const double D = 1.0 + c.dts * factor;
-    const double fb = pgacwb * x.ql / (D * D);
-    xb.ql += pgacwb * factor / D;
-    qdenb += fb * c.cgacw / dn;
-    const double dnb = -fb * factor / dn;
-    ib.den += dnb * 0.5 / dn * sq4;
-    const double sq4b = dnb * 0.5 / dn * den;
-    qdenb += sq4b * 0.5 / sq4 * 0.5 / sq2;
+    const double fb = pgacwb * x.ql * rcp64(D * D);
+    xb.ql += pgacwb * factor * rcp64(D);
+    qdenb += fb * c.cgacw * rcp64(dn);
+    const double dnb = -fb * factor * rcp64(dn);
+    ib.den += dnb * 0.5 * rcp64(dn) * sq4;
+    const double sq4b = dnb * 0.5 * rcp64(dn) * den;
+    qdenb += sq4b * 0.5 * rcp64(sq4) * 0.5 * rcp64(sq2);
   }
   xb.qg += qdenb * den;
   ib.den += qdenb * x.qg;
@@ -491,11 +497,11 @@ SH_FN void blk_w2_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     else
       xb.qr += pgacrb * c.rdts;
   }
-  ib.p1 -= dqs0b * (c.ces0 / k.in.p1) / k.in.p1;
+  ib.p1 -= dqs0b * (c.ces0 * rcp64(k.in.p1)) * rcp64(k.in.p1);
   xb.qv -= dqs0b;
   xb.tz += tcb;
-  lhib += icpkb / x.cvm;
-  xb.cvm -= icpkb * icpk / x.cvm;
+  lhib += icpkb * rcp64(x.cvm);
+  xb.cvm -= icpkb * icpk * rcp64(x.cvm);
   xb.tz += lhib * DC_ICE;
 }
 
@@ -571,10 +577,10 @@ SH_FN void blk_c1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     const double f2 = c.dts * c.cgaci * sd * x.qg;
     const double D = 1.0 + f2;
     const double pgacib = xb.qg - xb.qi;
-    qi1b += pgacib * f2 / D;
-    const double f2b = pgacib * qi1 / (D * D);
+    qi1b += pgacib * f2 * rcp64(D);
+    const double f2b = pgacib * qi1 * rcp64(D * D);
     xb.qg += f2b * (c.dts * c.cgaci * sd);
-    ib.den += f2b * (c.dts * c.cgaci) * x.qg * 0.5 / sd;
+    ib.den += f2b * (c.dts * c.cgaci) * x.qg * 0.5 * rcp64(sd);
   }
   const double sinkb = xb.qs - qi1b;
   double qib = qi1b, psacib = 0.0, psautb = 0.0;
@@ -590,15 +596,15 @@ SH_FN void blk_c1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     double qimb;
     if (quad) {
       const double dd = q_plus - qim;
-      const double ddb = dqb * 0.5 * dd / di;
-      dib += ddb - dqb * dq / di;
+      const double ddb = dqb * 0.5 * dd * rcp64(di);
+      dib += ddb - dqb * dq * rcp64(di);
       qib += ddb;
       qimb = -ddb;
     } else {
       qib += dqb;
       qimb = -dqb;
     }
-    ib.den -= qimb * qim / den;
+    ib.den -= qimb * qim * rcp64(den);
     if (!c.const_vi) tcb += tmpb * 0.025 * tmp;
   }
   if (di0 > QRMIN) {
@@ -609,12 +615,12 @@ SH_FN void blk_c1_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   }
   if (x.qs > 1.0e-7) {
     const double D = 1.0 + f1;
-    const double fb = psacib * x.qi / (D * D);
-    qib += psacib * f1 / D;
+    const double fb = psacib * x.qi * rcp64(D * D);
+    qib += psacib * f1 * rcp64(D);
     ib.denfac += fb * c.dts * c.csaci * e;
     const double argb = fb * (c.dts * denfac * c.csaci) * e;
     tcb += argb * 0.05;
-    const double qdb = argb * 0.8125 / (x.qs * den);
+    const double qdb = argb * 0.8125 * rcp64(x.qs * den);
     xb.qs += qdb * den;
     ib.den += qdb * x.qs;
   }
@@ -679,7 +685,7 @@ SH_FN void blk_c2_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   const double psb = xb.qs + skb, pgb = xb.qg + skb;
   const double factorb = psb * psacr + pgb * pgfr;
   double psacrb = psb * factor, pgfrb = pgb * factor;
-  const double numb = factorb / d2, d2b = -factorb * factor / d2;
+  const double numb = factorb * rcp64(d2), d2b = -factorb * factor * rcp64(d2);
   double sinkb = (sink > QRMIN) ? d2b : 0.0;
   double m1b = 0.0;
   if (sink < m1)
@@ -689,22 +695,22 @@ SH_FN void blk_c2_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   if (x.qr < lim) {
     xb.qr += m1b;
   } else {
-    xb.tz -= m1b / icpk;
-    icpkb -= m1b * lim / icpk;
+    xb.tz -= m1b * rcp64(icpk);
+    icpkb -= m1b * lim * rcp64(icpk);
   }
   psacrb += sinkb;
   pgfrb += sinkb;
-  ib.den -= pgfrb * (ex - 1.0) * pq * g0 / den;
+  ib.den -= pgfrb * (ex - 1.0) * pq * g0 * rcp64(den);
   const double tcb = pgfrb * g0 * pq * ex * (-ck(c, T_CGFR + 1));
-  const double qrdb = pgfrb * g0 * (ex - 1.0) * 1.75 * pq / (x.qr * den);
+  const double qrdb = pgfrb * g0 * (ex - 1.0) * 1.75 * pq * rcp64(x.qr * den);
   xb.qr += qrdb * den;
   ib.den += qrdb * x.qr;
   if (x.qs > 1.0e-7)
     acr3d_rev(k.in.vts, k.in.vtr, x.qr, x.qs, c.csacr, ck(c, T_ACCO + 1), ck(c, T_ACCO + 5), ck(c, T_ACCO + 9), den,
               psacrb * c.dts, ib.vts, ib.vtr, xb.qr, xb.qs, ib.den);
   xb.tz += tcb;
-  lhib += icpkb / x.cvm;
-  xb.cvm -= icpkb * icpk / x.cvm;
+  lhib += icpkb * rcp64(x.cvm);
+  xb.cvm -= icpkb * icpk * rcp64(x.cvm);
   xb.tz += lhib * DC_ICE;
 }
 // snow -> graupel (accretion pgacs, autoconversion psaut), qs > 1e-7
@@ -751,9 +757,9 @@ SH_FN void blk_c3_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   if (aut) {
     const double D = 1.0 + f;
     const double t = f / D;
-    const double fb = sinkb * (x.qs - qsm) / (D * D);
+    const double fb = sinkb * (x.qs - qsm) * rcp64(D * D);
     xb.qs += sinkb * t;
-    ib.den += sinkb * t * qsm / den;
+    ib.den += sinkb * t * qsm * rcp64(den);
     xb.tz += fb * f * 0.09;
   }
   if (x.qg > QRMIN)
@@ -821,13 +827,13 @@ SH_FN void blk_c4_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   const double prb = skb - xb.qr, plb = skb - xb.ql;
   const double factorb = prb * pgacr + plb * pgacw;
   double pgacrb = prb * factor, pgacwb = plb * factor;
-  const double numb = factorb / d2, d2b = -factorb * factor / d2;
+  const double numb = factorb * rcp64(d2), d2b = -factorb * factor * rcp64(d2);
   double sinkb = (sink > QRMIN) ? d2b : 0.0;
   if (sink < lim) {
     sinkb += numb;
   } else {
-    if (k.tice - x.tz > 0.0) xb.tz -= numb / icpk;
-    icpkb -= numb * lim / icpk;
+    if (k.tice - x.tz > 0.0) xb.tz -= numb * rcp64(icpk);
+    icpkb -= numb * lim * rcp64(icpk);
   }
   pgacrb += sinkb;
   pgacwb += sinkb;
@@ -840,18 +846,18 @@ SH_FN void blk_c4_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   }
   if (x.ql > 1.0e-6) {
     const double D = 1.0 + fw;
-    const double fb = pgacwb * x.ql / (D * D);
-    xb.ql += pgacwb * fw / D;
-    double qdenb = fb * (c.dts * c.cgacw) / dn;
-    const double dnb = -fb * fw / dn;
-    ib.den += dnb * 0.5 / dn * sq4;
-    const double sq4b = dnb * 0.5 / dn * den;
-    qdenb += sq4b * 0.5 / sq4 * 0.5 / sq2;
+    const double fb = pgacwb * x.ql * rcp64(D * D);
+    xb.ql += pgacwb * fw * rcp64(D);
+    double qdenb = fb * (c.dts * c.cgacw) * rcp64(dn);
+    const double dnb = -fb * fw * rcp64(dn);
+    ib.den += dnb * 0.5 * rcp64(dn) * sq4;
+    const double sq4b = dnb * 0.5 * rcp64(dn) * den;
+    qdenb += sq4b * 0.5 * rcp64(sq4) * 0.5 * rcp64(sq2);
     xb.qg += qdenb * den;
     ib.den += qdenb * x.qg;
   }
-  lhib += icpkb / x.cvm;
-  xb.cvm -= icpkb * icpk / x.cvm;
+  lhib += icpkb * rcp64(x.cvm);
+  xb.cvm -= icpkb * icpk * rcp64(x.cvm);
   xb.tz += lhib * DC_ICE;
 }
 
@@ -934,7 +940,7 @@ SH_FN void blk_coll_rev(const IcK& k, const IcX& x, IcX& xb) {
   const double dtin = N / Dd;
   const double tinb = xb.tz;
   double qpzb = xb.qv;
-  const double Nb = -tinb / Dd, Ddb = tinb * dtin / Dd;
+  const double Nb = -tinb * rcp64(Dd), Ddb = tinb * dtin * rcp64(Dd);
   xb.tz = tinb + (Nb * (x.ql + x.qi)) * c.d0_vap_d + (Nb * x.qi) * DC_ICE;
   qpzb += Ddb * c.c_vap_d;
   xb.qr += Ddb * C_LIQ;
@@ -998,23 +1004,23 @@ SH_FN void blk_evap_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
       xb.ql += evapb;
     else
       e2b = evapb;
-    const double fb = e2b * dq0 / Dn;
-    dq0b += e2b * factor / Dn;
-    Dnb -= e2b * e2 / Dn;
+    const double fb = e2b * dq0 * rcp64(Dn);
+    dq0b += e2b * factor * rcp64(Dn);
+    Dnb -= e2b * e2 * rcp64(Dn);
     if (!(1.0 < fm)) {
       const double r10b = fb * c.fac_l2v;
-      dq0b += r10b * 10.0 / qsw;
-      qswb -= r10b * r10 / qsw;
+      dq0b += r10b * 10.0 * rcp64(qsw);
+      qswb -= r10b * r10 * rcp64(qsw);
     }
   } else {
-    dq0b += evapb / Dn;
-    Dnb -= evapb * evap / Dn;
+    dq0b += evapb * rcp64(Dn);
+    Dnb -= evapb * evap * rcp64(Dn);
   }
   const double tcp3b = Dnb * dwsdt, dwsdtb = Dnb * tcp3;
   qswb += dq0b;
   xb.qv -= dq0b;
   sat_rev(false, x.tz, k.in.den, qsw, dwsdt, qswb, dwsdtb, xb.tz, ib.den);
-  if (!(1.0 < ratio) && (k.tice - x.tz > 0.0)) xb.tz -= tcp3b * h.icpk / (c.tice - c.t_wfr);
+  if (!(1.0 < ratio) && (k.tice - x.tz > 0.0)) xb.tz -= tcp3b * h.icpk * rcp64(c.tice - c.t_wfr);
   ic_heat_rev(c, x, h, lhlb, 0.0, tcp3b, tcp3b * mn, xb);
 }
 
@@ -1057,12 +1063,12 @@ SH_FN void blk_frz_rev(const IcK& k, const IcX& x, IcX& xb) {
     xb.ql += mb * dtmp * 0.125;
     dtmpb += mb * x.ql * 0.125;
   } else {
-    dtmpb += mb / icpk;
-    icpkb -= mb * a2 / icpk;
+    dtmpb += mb * rcp64(icpk);
+    icpkb -= mb * a2 * rcp64(icpk);
   }
   xb.tz -= dtmpb;
-  lhib += icpkb / x.cvm;
-  xb.cvm -= icpkb * icpk / x.cvm;
+  lhib += icpkb * rcp64(x.cvm);
+  xb.cvm -= icpkb * icpk * rcp64(x.cvm);
   xb.tz += lhib * DC_ICE;
 }
 
@@ -1108,16 +1114,16 @@ SH_FN void blk_bigg_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     mb = skb;
   double tcdb = 0.0;
   if (a2 < sink) {
-    tcdb += mb / icpk;
-    icpkb -= mb * a2 / icpk;
+    tcdb += mb * rcp64(icpk);
+    icpkb -= mb * a2 * rcp64(icpk);
   } else {
     tcdb += mb * C0 * ex * 0.66 * den * x.ql * x.ql;
     ib.den += mb * C0 * (ex - 1.0) * x.ql * x.ql;
     xb.ql += mb * 2.0 * C0 * (ex - 1.0) * den * x.ql;
   }
   xb.tz -= tcdb;
-  lhib += icpkb / x.cvm;
-  xb.cvm -= icpkb * icpk / x.cvm;
+  lhib += icpkb * rcp64(x.cvm);
+  xb.cvm -= icpkb * icpk * rcp64(x.cvm);
   xb.tz += lhib * DC_ICE;
 }
 // sublimation / deposition of cloud ice below tice
@@ -1192,15 +1198,15 @@ SH_FN void blk_idep_rev(const IcK& k, const IcX& x, double dt_pisub, IcX& xb, Ic
     if (x1 < x2) {
       if (qi_crt - x.qi > pidep) {
         xb.qi -= mb;
-        ib.qi_gen += mb * mq / den;
-        ib.den -= mb * qi_crt / den;
-        if (!(c.qi_lim < xq)) tmpb += mb * c.qi_gen / den * 0.1;
+        ib.qi_gen += mb * mq * rcp64(den);
+        ib.den -= mb * qi_crt * rcp64(den);
+        if (!(c.qi_lim < xq)) tmpb += mb * c.qi_gen * rcp64(den) * 0.1;
       } else {
         pidepb += mb;
       }
     } else {
-      tmpb += mb / tcpk;
-      tcpkb -= mb * x2 / tcpk;
+      tmpb += mb * rcp64(tcpk);
+      tcpkb -= mb * x2 * rcp64(tcpk);
     }
     xb.tz -= tmpb;
   } else {
@@ -1223,18 +1229,18 @@ SH_FN void blk_idep_rev(const IcK& k, const IcX& x, double dt_pisub, IcX& xb, Ic
   }
   double dqb = 0.0, qsib = 0.0;
   if (hasqi) {
-    const double numb = pidepb / dd, ddb = -pidepb * pidep / dd;
+    const double numb = pidepb * rcp64(dd), ddb = -pidepb * pidep * rcp64(dd);
     dqb += numb * dt_pisub * 349138.78 * pq;
     const double pqb = numb * dt_pisub * dq * 349138.78;
-    const double qidb = pqb * 0.875 * pq / (x.qi * den);
+    const double qidb = pqb * 0.875 * pq * rcp64(x.qi * den);
     xb.qi += qidb * den;
     ib.den += qidb * x.qi;
-    qsib += ddb * den * LAT2 / (K2 * T2);
-    ib.den += ddb * qsi * LAT2 / (K2 * T2);
-    xb.tz -= ddb * 2.0 * dd0 / x.tz;
+    qsib += ddb * den * LAT2 * rcp64(K2 * T2);
+    ib.den += ddb * qsi * LAT2 * rcp64(K2 * T2);
+    xb.tz -= ddb * 2.0 * dd0 * rcp64(x.tz);
   }
-  dqb += sinkb / Dn;
-  const double Dnb = -sinkb * sink / Dn;
+  dqb += sinkb * rcp64(Dn);
+  const double Dnb = -sinkb * sink * rcp64(Dn);
   tcpkb += Dnb * dqsdt;
   const double dqsdtb = Dnb * tcpk;
   xb.qv += dqb;
@@ -1327,28 +1333,28 @@ SH_FN void blk_sdep_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     if (dq > x3) {
       dqb += mb;
     } else {
-      xb.tz += mb / tcpk;
-      tcpkb -= mb * x3 / tcpk;
+      xb.tz += mb * rcp64(tcpk);
+      tcpkb -= mb * x3 * rcp64(tcpk);
     }
   }
   double dsatb = pssubb * c.dts * pss0;
   const double pss0b = pssubb * (dsat * c.dts);
-  const double numb = pss0b / dd, ddb = -pss0b * pss0 / dd;
+  const double numb = pss0b * rcp64(dd), ddb = -pss0b * pss0 * rcp64(dd);
   double tsqb = numb * ck(c, T_CSSUB + 0) * Bq + ddb * ck(c, T_CSSUB + 3);
   const double Bqb = numb * ck(c, T_CSSUB + 0) * tsq;
-  double qdenb = Bqb * ck(c, T_CSSUB + 1) * 0.5 / sqd;
+  double qdenb = Bqb * ck(c, T_CSSUB + 1) * 0.5 * rcp64(sqd);
   const double tmpb = Bqb * ck(c, T_CSSUB + 2) * sdf;
-  ib.denfac += Bqb * ck(c, T_CSSUB + 2) * tmp * 0.5 / sdf;
+  ib.denfac += Bqb * ck(c, T_CSSUB + 2) * tmp * 0.5 * rcp64(sdf);
   double qsib = ddb * ck(c, T_CSSUB + 4) * den;
   ib.den += ddb * ck(c, T_CSSUB + 4) * qsi;
-  dsatb += dqb / Dn;
-  const double Dnb = -dqb * dq / Dn;
+  dsatb += dqb * rcp64(Dn);
+  const double Dnb = -dqb * dq * rcp64(Dn);
   tcpkb += Dnb * dqsdt;
   const double dqsdtb = Dnb * tcpk;
   qsib += dsatb;
   xb.qv -= dsatb;
   xb.tz += tsqb * 2.0 * x.tz;
-  qdenb += tmpb * 0.65625 * tmp / qden;
+  qdenb += tmpb * 0.65625 * tmp * rcp64(qden);
   xb.qs += qdenb * den;
   ib.den += qdenb * x.qs;
   sat_rev(true, x.tz, den, qsi, dqsdt, qsib, dqsdtb, xb.tz, ib.den);
@@ -1426,8 +1432,8 @@ SH_FN void blk_gdep_rev(const IcK& k, const IcX& x, double icpk_ext, IcX& xb, Ic
         xb.ql += m2b;
         xb.qr += m2b;
       } else {
-        xb.tz -= m2b / tcpk;
-        tcpkb -= m2b * d / tcpk;
+        xb.tz -= m2b * rcp64(tcpk);
+        tcpkb -= m2b * d * rcp64(tcpk);
       }
     }
   } else {
@@ -1444,9 +1450,9 @@ SH_FN void blk_gdep_rev(const IcK& k, const IcX& x, double icpk_ext, IcX& xb, Ic
   }
   const double rqb = pgsubb * x.qg;
   xb.qg += pgsubb * (rq - 1.0);
-  xb.qv += rqb / qsi;
-  double qsib = -rqb * rq / qsi;
-  const double nb = dqb / Dn, Dnb = -dqb * dq / Dn;
+  xb.qv += rqb * rcp64(qsi);
+  double qsib = -rqb * rq * rcp64(qsi);
+  const double nb = dqb * rcp64(Dn), Dnb = -dqb * dq * rcp64(Dn);
   xb.qv += nb;
   qsib -= nb;
   tcpkb += Dnb * dqsdt;
@@ -1491,7 +1497,7 @@ SH_FN void blk_revp_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
     xb.qr += sinkb;
   else
     bb = sinkb;
-  const double ddb = bb / Dn, Dnb = -bb * b / Dn;
+  const double ddb = bb * rcp64(Dn), Dnb = -bb * b * rcp64(Dn);
   double qswb = 0.0;
   if (a - x.qv > 0.0) {
     qswb = ddb * k.rh_rain;
@@ -1499,8 +1505,8 @@ SH_FN void blk_revp_rev(const IcK& k, const IcX& x, IcX& xb, IcInB& ib) {
   }
   const double lcpkb = Dnb * dqsdt, dqsdtb = Dnb * lcpk;
   sat_rev(false, x.tz, k.in.den, qsw2, dqsdt, qswb, dqsdtb, xb.tz, ib.den);
-  lhlb += lcpkb / x.cvm;
-  xb.cvm -= lcpkb * lcpk / x.cvm;
+  lhlb += lcpkb * rcp64(x.cvm);
+  xb.cvm -= lcpkb * lcpk * rcp64(x.cvm);
   xb.tz += lhlb * c.d0_vap_d;
 }
 
@@ -1573,7 +1579,7 @@ SH_FN double blk_cf_rev(const IcK& k, const IcX& x, double icpk7, IcX& xb, IcInB
   if (!((qstar < q_plus) && (q_cond > c.qc_crt_d))) return 0.0;
   const double d2 = dq + dq;
   const double f = (q_plus - qstar) / d2;
-  const double nb = xb.qa / d2, d2b = -xb.qa * f / d2;
+  const double nb = xb.qa * rcp64(d2), d2b = -xb.qa * f * rcp64(d2);
   const double qstarb = -nb;
   double qpz2b = nb;
   const double dqb = 2.0 * d2b + nb;
@@ -1588,10 +1594,10 @@ SH_FN double blk_cf_rev(const IcK& k, const IcX& x, double icpk7, IcX& xb, IcInB
     sat_rev(true, tin2, den, qsi, 0.0, qstarb * rqi, 0.0, tin2b, ib.den);
     sat_rev(false, tin2, den, qsw3, 0.0, qstarb * (1.0 - rqi), 0.0, tin2b, ib.den);
     if (q_cond > 3.0e-6) {
-      q_sol2b += rqib / q_cond;
-      q_condb -= rqib * rqi / q_cond;
+      q_sol2b += rqib * rcp64(q_cond);
+      q_condb -= rqib * rqi * rcp64(q_cond);
     } else {
-      tin2b -= rqib / (k.tice - c.t_wfr_d);
+      tin2b -= rqib * rcp64(k.tice - c.t_wfr_d);
     }
   }
   xb.tz += tin2b;
@@ -1606,11 +1612,11 @@ SH_FN double blk_cf_rev(const IcK& k, const IcX& x, double icpk7, IcX& xb, IcInB
   if (c.rad_rain) xb.qr += q_condb;
   xb.qi += q_sol2b;
   if (c.rad_snow) xb.qs += q_sol2b;
-  const double cvmb = -lcpkb * lcpk / cvm;
+  const double cvmb = -lcpkb * lcpk * rcp64(cvm);
   xb.qv += cvmb * c.c_vap_d;
   xb.q_liq += cvmb * c.c_vap_d;
   xb.q_sol += cvmb * c.c_vap_d;
-  xb.tz += lcpkb / cvm * c.d0_vap_d;
+  xb.tz += lcpkb * rcp64(cvm) * c.d0_vap_d;
   return icpk7b;
 }
 
@@ -1769,10 +1775,10 @@ SH_FN void revap_vjp(const Coef<double>& c, double t_wfr, double dt, double h_va
     const double sink = dt * denfac * c.cracw * pe;
     const double D = 1.0 + sink;
     const double skb = xb.qr - xb.ql;
-    xb.ql += skb * sink / D;
-    const double sinkb = skb * s.ql / (D * D);
+    xb.ql += skb * sink * rcp64(D);
+    const double sinkb = skb * s.ql * rcp64(D * D);
     denfacb += sinkb * dt * c.cracw * pe;
-    const double qdb = sinkb * dt * denfac * c.cracw * 0.95 * pe / (qr1 * den);
+    const double qdb = sinkb * dt * denfac * c.cracw * 0.95 * pe * rcp64(qr1 * den);
     xb.qr += qdb * den;
     denb += qdb * qr1;
   }
@@ -1790,19 +1796,19 @@ SH_FN void revap_vjp(const Coef<double>& c, double t_wfr, double dt, double h_va
     if (a < b) {
       evapb = mb * dt;
     } else {
-      dqvb += mb / Dn;
-      const double Dnb = -mb * b / Dn;
+      dqvb += mb * rcp64(Dn);
+      const double Dnb = -mb * b * rcp64(Dn);
       lcpkb += Dnb * dqsdt;
       dqsdtb += Dnb * lcpk;
     }
-    const double Nb = evapb / Dd, Ddb = -evapb * evap / Dd;
+    const double Nb = evapb * rcp64(Dd), Ddb = -evapb * evap * rcp64(Dd);
     const double c0 = ck(c, T_CREVP + 0);
     const double t2b = Nb * c0 * dq * Bq + Ddb * ck(c, T_CREVP + 3);
     const double dqb = Nb * c0 * t2 * Bq;
     const double Bqb = Nb * c0 * t2 * dq;
     qsatb += Ddb * ck(c, T_CREVP + 4) * den;
     denb += Ddb * ck(c, T_CREVP + 4) * qsat;
-    const double qdenb = Bqb * (ck(c, T_CREVP + 1) * 0.5 / sqrt(qden) + ck(c, T_CREVP + 2) * 0.725 * pe7 / qden);
+    const double qdenb = Bqb * (ck(c, T_CREVP + 1) * 0.5 * rcp64(sqrt(qden)) + ck(c, T_CREVP + 2) * 0.725 * pe7 * rcp64(qden));
     xb.qr += qdenb * den;
     denb += qdenb * s.qr;
     tinb += t2b * 2.0 * tin;
@@ -1811,8 +1817,8 @@ SH_FN void revap_vjp(const Coef<double>& c, double t_wfr, double dt, double h_va
       qpzb -= dqb;
     } else {
       const double e = q_minus - qsat;
-      const double eb = dqb * 0.5 * e / dqh;
-      dqhb -= dqb * dq / dqh;
+      const double eb = dqb * 0.5 * e * rcp64(dqh);
+      dqhb -= dqb * dq * rcp64(dqh);
       qpzb += eb;  // q_minus = qpz - dqh
       dqhb -= eb;
       qsatb -= eb;
@@ -1836,8 +1842,8 @@ SH_FN void revap_vjp(const Coef<double>& c, double t_wfr, double dt, double h_va
   xb.tz += tinb;
   lcpkb -= tinb * s.ql;
   xb.ql -= tinb * lcpk;
-  lhlb += lcpkb / cvm;
-  const double cb = xb.cvm - lcpkb * lcpk / cvm;
+  lhlb += lcpkb * rcp64(cvm);
+  const double cb = xb.cvm - lcpkb * lcpk * rcp64(cvm);
   xb.qv += cb * c.c_vap_d;
   const double lb = xb.q_liq + cb * C_LIQ, sb = xb.q_sol + cb * C_ICE;
   xb.ql += lb;
@@ -1875,7 +1881,7 @@ SH_FN void autoconv_vjp(const Coef<double>& c, double t_wfr, bool subgrid, doubl
     } else {
       ab.c_praut += sinkb * c.dt_rain * den * pe;
       ab.den += sinkb * c.dt_rain * c_praut * pe;
-      qlb += sinkb * c.dt_rain * c_praut * den * (7.0 / 3.0) * pe / ql;
+      qlb += sinkb * c.dt_rain * c_praut * den * (7.0 / 3.0) * pe * rcp64(ql);
     }
   } else {
     if (!(tz > t_wfr + DT_FR)) return;
@@ -1891,11 +1897,11 @@ SH_FN void autoconv_vjp(const Coef<double>& c, double t_wfr, bool subgrid, doubl
     const double mb = sinkb * K;
     ab.c_praut += sinkb * m * c.dt_rain * den * pe;
     ab.den += sinkb * m * c.dt_rain * c_praut * pe;
-    qlb += sinkb * m * c.dt_rain * c_praut * den * (7.0 / 3.0) * pe / ql;
+    qlb += sinkb * m * c.dt_rain * c_praut * den * (7.0 / 3.0) * pe * rcp64(ql);
     double dqb = 0.0, dl1b = 0.0;
     if (!(1.0 < r)) {
-      dqb += mb / dl1;
-      dl1b -= mb * r / dl1;
+      dqb += mb * rcp64(dl1);
+      dl1b -= mb * r * rcp64(dl1);
     }
     qlb += 0.5 * dqb;
     dl1b += 0.5 * dqb;
@@ -1908,8 +1914,8 @@ SH_FN void autoconv_vjp(const Coef<double>& c, double t_wfr, bool subgrid, doubl
   }
   double qc0b = qcb;
   if (!c.use_ccn) {
-    qc0b = qcb / den;
-    ab.den -= qcb * qc / den;
+    qc0b = qcb * rcp64(den);
+    ab.den -= qcb * qc * rcp64(den);
   }
   ab.fac_rc += qc0b * ccn;
   ab.ccn += qc0b * c.fac_rc;
@@ -1934,7 +1940,7 @@ struct FsB {
 SH_FN double rhof_of(double den) { return sqrt(mmin(10.0, SFCRHO / den)); }
 SH_FN void rhof_rev(double den, double rb, double& denb) {
   const double x = SFCRHO / den;
-  if (!(10.0 < x)) denb -= rb * 0.5 / sqrt(x) * x / den;
+  if (!(10.0 < x)) denb -= rb * 0.5 * rcp64(sqrt(x)) * x * rcp64(den);
 }
 
 // Vector-Jacobian product of fall_speed with cotangents bg, bi, bs of vtg,
@@ -1956,10 +1962,10 @@ SH_FN void fall_speed_vjp(const Coef<double>& c, double qg, double qi, double qs
     if (!(c.vi_max < mx) && !(VF_MIN > v2)) {
       fb.vi_fac += bi * 0.01 * e * 0.8;
       const double v1b = bi * vi0 * 0.8 * e * c.log_10;
-      const double lgb = v1b * poly / c.log_10;
-      fb.tc += v1b * (3.0 + lg / c.log_10) * (2.0 * AA * tc + BB) + v1b * DD_FS;
-      fb.qi += lgb / qi;
-      fb.den += lgb / den;
+      const double lgb = v1b * poly * rcp64(c.log_10);
+      fb.tc += v1b * (3.0 + lg * rcp64(c.log_10)) * (2.0 * AA * tc + BB) + v1b * DD_FS;
+      fb.qi += lgb * rcp64(qi);
+      fb.den += lgb * rcp64(den);
     }
   }
   if (c.const_vs) {
@@ -1971,9 +1977,9 @@ SH_FN void fall_speed_vjp(const Coef<double>& c, double qg, double qi, double qs
     if (!(c.vs_max < mmax(VF_MIN, v)) && !(VF_MIN > v)) {
       fb.vs_fac += bs * VCONS * rhof * p;
       rhofb += bs * c.vs_fac * VCONS * p;
-      const double xb = bs * c.vs_fac * VCONS * rhof * 0.0625 * p / x;
-      fb.qs += xb * den / NORMS;
-      fb.den += xb * qs / NORMS;
+      const double xb = bs * c.vs_fac * VCONS * rhof * 0.0625 * p * rcp64(x);
+      fb.qs += xb * den * (1.0 / NORMS);
+      fb.den += xb * qs * (1.0 / NORMS);
     }
   }
   if (c.const_vg) {
@@ -1985,9 +1991,9 @@ SH_FN void fall_speed_vjp(const Coef<double>& c, double qg, double qi, double qs
     if (!(c.vg_max < mmax(VF_MIN, v)) && !(VF_MIN > v)) {
       fb.vg_fac += bg * VCONG * rhof * r8;
       rhofb += bg * c.vg_fac * VCONG * r8;
-      const double xb = bg * c.vg_fac * VCONG * rhof * 0.125 * r8 / x;
-      fb.qg += xb * den / NORMG;
-      fb.den += xb * qg / NORMG;
+      const double xb = bg * c.vg_fac * VCONG * rhof * 0.125 * r8 * rcp64(x);
+      fb.qg += xb * den * (1.0 / NORMG);
+      fb.den += xb * qg * (1.0 / NORMG);
     }
   }
   rhof_rev(den, rhofb, fb.den);
@@ -2010,8 +2016,8 @@ SH_FN void rain_fspeed_vjp(const Coef<double>& c, bool no_fall, double qr, doubl
   if (c.vr_max < mmax(VR_MIN, v) || VR_MIN > v) return;
   vr_facb += vb * VCONR * rhof * p;
   rhof_rev(den, vb * c.vr_fac * VCONR * p, denb);
-  const double xb = vb * c.vr_fac * VCONR * rhof * 0.2 * p / x;
-  qrb += xb * den / NORMR;
-  denb += xb * qr / NORMR;
+  const double xb = vb * c.vr_fac * VCONR * rhof * 0.2 * p * rcp64(x);
+  qrb += xb * den * (1.0 / NORMR);
+  denb += xb * qr * (1.0 / NORMR);
 }
 }  // namespace shield
