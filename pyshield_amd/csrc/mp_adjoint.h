@@ -107,6 +107,22 @@ struct KAdjFields {
 #undef SH_KAF
 };
 
+// Member MEMBER of the argument block, read from the kernarg segment through
+// an offset the compiler cannot see (as KPtr does for the field pointers).
+// The coefficient block and the parameters are passed to each stage of the
+// persistent loop this way: read directly, their loads are loop-invariant and
+// the compiler hoists them all to the kernel's entry, where the scalar
+// registers cannot hold them and they spill to VGPR lanes, every use then a
+// v_readlane.  Through karg each stage loads what it uses with scalar loads.
+template <class T, uint32_t OFF>
+__device__ __forceinline__ const T& karg() {
+  uint32_t o = OFF;
+  asm volatile("" : "+s"(o));
+  const __attribute__((address_space(4))) char* base =
+      (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+  return *(const T*)(const __attribute__((address_space(4))) T*)(base + o);
+}
+
 // per-cell cotangents of the auxiliary fields, kept in LDS
 enum { AB_DEN, AB_DENFAC, AB_DZ1, AB_VTR, AB_VTG, AB_VTS, AB_M1, AB_M1SOL, AB_DP1, AB_T0, AB_DZ0, AB_N };
 
@@ -1711,6 +1727,11 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
   const shield_mp_shape& sh3 = args.sh3;
   const shield_mp_params& P = args.P;
   const Coef<double>& c = args.c;
+  // the coefficients and parameters of one stage (karg)
+  const auto kc = []() -> const Coef<double>& { return karg<Coef<double>, (uint32_t)offsetof(AdjArgs, c)>(); };
+  const auto kP = []() -> const shield_mp_params& {
+    return karg<shield_mp_params, (uint32_t)offsetof(AdjArgs, P)>();
+  };
   const KFields<(uint32_t)offsetof(AdjArgs, f)> f;
   const KAdjFields<(uint32_t)offsetof(AdjArgs, a)> a;
   double* const tape = args.tape;
@@ -1817,7 +1838,7 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     }
     __syncthreads();
     if (lane < ncols) sh.colv[CV_CPRAUT][lane] = P.cpaut * pow(sh.colv[CV_CCN][lane] * RHOR, -1.0 / 3.0);
-    if (fixneg) fix_negative(L, X, sh, P);
+    if (fixneg) fix_negative(L, X, sh, kP());
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       sh.cell[CL_M1][64 * s + lane] = 0.0;
@@ -1831,12 +1852,12 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     __syncthreads();
     // ------------------------------------------------ sub-steps, recorded
     for (int it = 0; it < P.ntimes; ++it) {
-      warm_rain_rec(L, X, sh, Gl, f, P, c, true, T, it, RW1_B, wv, sedi_w);
-      sed_rec(L, X, sh, Gl, f, P, c, T, it, wv, sedi_w);
-      warm_rain_rec(L, X, sh, Gl, f, P, c, false, T, it, RW2_B, wv, sedi_w);
+      warm_rain_rec(L, X, sh, Gl, f, kP(), kc(), true, T, it, RW1_B, wv, sedi_w);
+      sed_rec(L, X, sh, Gl, f, kP(), kc(), T, it, wv, sedi_w);
+      warm_rain_rec(L, X, sh, Gl, f, kP(), kc(), false, T, it, RW2_B, wv, sedi_w);
       tape_save(T, it, RI, X, sh);
       // the last sub-step's icloud runs in its reverse (icloud_adj)
-      if (it + 1 < P.ntimes) icloud(L, X, sh, Gl, f, P, c);
+      if (it + 1 < P.ntimes) icloud(L, X, sh, Gl, f, kP(), kc());
     }
     __syncthreads();
     // ----------------------------------------------------- reverse walk
@@ -1930,16 +1951,16 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
     // ------------------------------------------------ sub-steps, reversed
     for (int it = P.ntimes - 1; it >= 0; --it) {
       aux_load(T, it, sh);
-      icloud_adj(L, A, G, Gl, f, a, P, c, T, it, it == P.ntimes - 1, nb, EXT && P.nml.z_slope_ice);
+      icloud_adj(L, A, G, Gl, f, a, kP(), kc(), T, it, it == P.ntimes - 1, nb, EXT && P.nml.z_slope_ice);
       __syncthreads();
-      warm_rain_adj(L, A, G, Gl, f, P, c, false, T, it, RW2_B, X.dp1, sedi_w);
+      warm_rain_adj(L, A, G, Gl, f, kP(), kc(), false, T, it, RW2_B, X.dp1, sedi_w);
       __syncthreads();
-      sed_adj(L, A, G, P, c, T, it, X.dp1, sedi_w);
+      sed_adj(L, A, G, kP(), kc(), T, it, X.dp1, sedi_w);
       __syncthreads();
-      warm_rain_adj(L, A, G, Gl, f, P, c, true, T, it, RW1_B, X.dp1, sedi_w);
+      warm_rain_adj(L, A, G, Gl, f, kP(), kc(), true, T, it, RW1_B, X.dp1, sedi_w);
       __syncthreads();
     }
-    if (fixneg) fix_negative_adj(L, A, G, X, Gl, f, P);
+    if (fixneg) fix_negative_adj(L, A, G, X, Gl, f, kP());
     // -------------------------------------------------- fields_init, reversed
 #pragma unroll
     for (int s = 0; s < S; ++s) {
