@@ -1677,7 +1677,20 @@ SH_FN void icloud_vjp(const IcK& k, const Cell& s, double (&yb)[8], IcInB& ib, C
   i7 = i6;
   if (run) blk_revp(k, i7);
   if (MID || out) {
-    IcX o = run ? i7 : t;
+    // member by member: a select of the whole struct is compiled as a
+    // per-lane pointer into a private copy of both (scratch memory)
+    IcX o;
+    o.qv = run ? i7.qv : t.qv;
+    o.ql = run ? i7.ql : t.ql;
+    o.qr = run ? i7.qr : t.qr;
+    o.qi = run ? i7.qi : t.qi;
+    o.qs = run ? i7.qs : t.qs;
+    o.qg = run ? i7.qg : t.qg;
+    o.tz = run ? i7.tz : t.tz;
+    o.qa = run ? i7.qa : t.qa;
+    o.q_liq = run ? i7.q_liq : t.q_liq;
+    o.q_sol = run ? i7.q_sol : t.q_sol;
+    o.cvm = run ? i7.cvm : t.cvm;
     if (run) blk_cf(k, o, icpk7);
     const Cell oc{o.qv, o.ql, o.qr, o.qi, o.qs, o.qg, o.tz, o.qa};
     if (out) *out = oc;
