@@ -749,22 +749,41 @@ __device__ __forceinline__ void state_update(const Lay<S>& L, const Glob<S>& G, 
   const int nz = L.nz;
   const double* in[10] = {f.qv, f.ql, f.qr, f.qi, f.qs, f.qg, f.qa, f.pt, f.ua, f.va};
   const double* td[10] = {f.qv_dt, f.ql_dt, f.qr_dt, f.qi_dt, f.qs_dt, f.qg_dt, f.qa_dt, f.pt_dt, f.udt, f.vdt};
-  // forward Euler; the updated vapour and the dycore delp go to LDS
+  // In memory phases: every load of a phase precedes its first store.  A
+  // store may alias a later load as far as the compiler knows (the entry
+  // point rejects calls where outputs alias), so interleaved per field the
+  // loads and stores serialise one HBM round trip per field and slot.
+  // forward Euler; the updated vapour and the dycore delp go to LDS, and the
+  // updated values the tracer conversion reads stay in registers
+  double xin[10][S], xtd[10][S], dp0[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    double qv1 = 0.0, dp = 1.0;
+    const int32_t o = G.off[s];
+    dp0[s] = 1.0;
+#pragma unroll
+    for (int n = 0; n < 10; ++n) xin[n][s] = xtd[n][s] = 0.0;
     if (L.v[s]) {
-      const int32_t o = G.off[s];
 #pragma unroll
       for (int n = 0; n < 10; ++n) {
-        const double x = in[n][o] + td[n][o] * u.dt;
-        u.updated[n][o] = x;
-        if (n == 0) qv1 = x;
+        xin[n][s] = in[n][o];
+        xtd[n][s] = td[n][o];
       }
-      dp = u.delp[o];
+      dp0[s] = u.delp[o];
     }
-    sh.buf[64 * s + lane] = qv1;
-    sh.cell[CL_TMP][64 * s + lane] = dp;
+  }
+  double x[10][S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+#pragma unroll
+    for (int n = 0; n < 10; ++n) x[n][s] = xin[n][s] + xtd[n][s] * u.dt;
+    if (L.v[s]) {
+      const int32_t o = G.off[s];
+      // updated[0] is written after the fill
+#pragma unroll
+      for (int n = 1; n < 10; ++n) u.updated[n][o] = x[n][s];
+    }
+    sh.buf[64 * s + lane] = L.v[s] ? x[0][s] : 0.0;
+    sh.cell[CL_TMP][64 * s + lane] = dp0[s];
   }
   __syncthreads();
   // fill_gfs_delp: a non-linear vertical recurrence, one lane per column
@@ -782,30 +801,47 @@ __device__ __forceinline__ void state_update(const Lay<S>& L, const Glob<S>& G, 
       if (q[k] < 0.0) q[k] = 0.0;
   }
   __syncthreads();
-  // prepare_tendencies_and_update_tracers
+  // prepare_tendencies_and_update_tracers: its loads, then its arithmetic
+  // and stores
+  double* const* tr = u.tracers;  // qv, ql, qr, qs, qi, qg
+  double gu[S], gv[S], gt[S], pdn[S], pup[S], trq[6][S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
+    gu[s] = gv[s] = gt[s] = pdn[s] = pup[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) trq[q][s] = 0.0;
     if (!L.v[s]) continue;
     const int32_t o = G.off[s];
     const int64_t col = c0 + L.cl[s];
     const int64_t i = col / sh3.nj, j = col - i * sh3.nj;
     const int64_t oi = i * u.si_i + j * u.sj_i + L.k[s];
+    gu[s] = u.u_dt[o];
+    gv[s] = u.v_dt[o];
+    gt[s] = u.pt_dt[o];
+    pup[s] = u.prsi[oi];
+    pdn[s] = u.prsi[oi + 1];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) trq[q][s] = tr[q][o];
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (!L.v[s]) continue;
+    const int32_t o = G.off[s];
     const double qv1 = sh.buf[64 * s + lane];
     u.updated[0][o] = qv1;
-    u.u_dt[o] += (u.updated[8][o] - f.ua[o]) * u.rdt;
-    u.v_dt[o] += (u.updated[9][o] - f.va[o]) * u.rdt;
-    u.pt_dt[o] += (u.updated[7][o] - f.pt[o]) * u.rdt;
-    const double dp = u.prsi[oi + 1] - u.prsi[oi];
+    u.u_dt[o] = gu[s] + (x[8][s] - xin[8][s]) * u.rdt;
+    u.v_dt[o] = gv[s] + (x[9][s] - xin[9][s]) * u.rdt;
+    u.pt_dt[o] = gt[s] + (x[7][s] - xin[7][s]) * u.rdt;
+    const double dp = pdn[s] - pup[s];
     const double qwat_qv = dp * qv1;
-    const double qwat_ql = dp * u.updated[1][o];
-    const double qwat_qr = dp * u.updated[2][o];
-    const double qwat_qs = dp * u.updated[4][o];
-    const double qwat_qi = dp * u.updated[3][o];
-    const double qwat_qg = dp * u.updated[5][o];
+    const double qwat_ql = dp * x[1][s];
+    const double qwat_qr = dp * x[2][s];
+    const double qwat_qs = dp * x[4][s];
+    const double qwat_qi = dp * x[3][s];
+    const double qwat_qg = dp * x[5][s];
     const double qt = qwat_qv + qwat_ql + qwat_qr + qwat_qs + qwat_qi + qwat_qg;
-    double* const* tr = u.tracers;  // qv, ql, qr, qs, qi, qg
-    const double q_sum = tr[0][o] + tr[1][o] + tr[2][o] + tr[3][o] + tr[4][o] + tr[5][o];
-    const double q0 = u.delp[o] * (1.0 - q_sum) + qt;
+    const double q_sum = trq[0][s] + trq[1][s] + trq[2][s] + trq[3][s] + trq[4][s] + trq[5][s];
+    const double q0 = dp0[s] * (1.0 - q_sum) + qt;
     u.delp[o] = q0;
     tr[0][o] = qwat_qv / q0;
     tr[1][o] = qwat_ql / q0;
