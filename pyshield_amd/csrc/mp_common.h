@@ -323,4 +323,37 @@ inline Coef<R> make_coef(const shield_mp_params& p) {
   return c;
 }
 
+// fill_gfs_delp (update_atmos_state.py:21-37) on one column of nk layers in
+// memory (k = 0 the top): q below q_min is filled from the layer above it,
+// then floored, filled downward from a negative layer and clamped at zero.
+// The running value is carried in registers; the reference's three downward
+// passes run as one (the borrow takes the layer above after its own borrow,
+// before the clamp), with every element getting the reference's operations
+// in the reference's order.
+SH_FN void fill_column(double* qc, const double* dp, int nk, double q_min) {
+  // upward: fill from the layer below (its value as just updated)
+  double qn = qc[nk - 1], dn = dp[nk - 1];
+  for (int k = nk - 2; k >= 0; --k) {
+    double qk = qc[k];
+    const double dk = dp[k];
+    if (qn < q_min) qk = qk + (qn - q_min) * dn / dk;
+    qc[k] = qk;
+    qn = qk;
+    dn = dk;
+  }
+  // downward: the floor at q_min (k >= 1), the borrow from a negative layer
+  // above, the clamp at zero
+  double qp = qc[0], dpv = dp[0];
+  qc[0] = qp < 0.0 ? 0.0 : qp;
+  for (int k = 1; k < nk; ++k) {
+    double qk = qc[k];
+    if (qk < q_min) qk = q_min;
+    const double dk = dp[k];
+    if (qp < 0.0) qk = qk + qp * (dpv) / (dk);
+    qc[k] = qk < 0.0 ? 0.0 : qk;
+    qp = qk;
+    dpv = dk;
+  }
+}
+
 }  // namespace shield

@@ -787,19 +787,7 @@ __device__ __forceinline__ void state_update(const Lay<S>& L, const Glob<S>& G, 
   }
   __syncthreads();
   // fill_gfs_delp: a non-linear vertical recurrence, one lane per column
-  if (lane < L.ncols) {
-    double* q = sh.buf + lane * nz;
-    const double* dp = sh.cell[CL_TMP] + lane * nz;
-    const double q_min = u.q_min;
-    for (int k = nz - 2; k >= 0; --k)
-      if (q[k + 1] < q_min) q[k] = q[k] + (q[k + 1] - q_min) * dp[k + 1] / dp[k];
-    for (int k = 1; k < nz; ++k)
-      if (q[k] < q_min) q[k] = q_min;
-    for (int k = 1; k < nz; ++k)
-      if (q[k - 1] < 0.0) q[k] = q[k] + q[k - 1] * (dp[k - 1]) / (dp[k]);
-    for (int k = 0; k < nz; ++k)
-      if (q[k] < 0.0) q[k] = 0.0;
-  }
+  if (lane < L.ncols) fill_column(sh.buf + lane * nz, sh.cell[CL_TMP] + lane * nz, nz, u.q_min);
   __syncthreads();
   // prepare_tendencies_and_update_tracers: its loads, then its arithmetic
   // and stores

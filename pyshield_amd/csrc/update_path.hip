@@ -91,34 +91,7 @@ __global__ __launch_bounds__(kBlock) void fill_gfs_delp_kernel(Idx ix, const dou
     ds[e] = delp[o];
   }
   __syncthreads();
-  if ((int)threadIdx.x < nc) {
-    double* qc = qs + threadIdx.x * nk;
-    const double* dp = ds + threadIdx.x * nk;
-    // upward: fill from the layer below (its value as just updated)
-    double qn = qc[nk - 1], dn = dp[nk - 1];
-    for (int k = nk - 2; k >= 0; --k) {
-      double qk = qc[k];
-      const double dk = dp[k];
-      if (qn < q_min) qk = qk + (qn - q_min) * dn / dk;
-      qc[k] = qk;
-      qn = qk;
-      dn = dk;
-    }
-    // downward, the reference's three passes in one: the floor at q_min
-    // (k >= 1), the borrow from a negative layer above (its value after its
-    // own borrow, before the final clamp), the clamp at zero
-    double qp = qc[0], dpv = dp[0];
-    qc[0] = qp < 0.0 ? 0.0 : qp;
-    for (int k = 1; k < nk; ++k) {
-      double qk = qc[k];
-      if (qk < q_min) qk = q_min;
-      const double dk = dp[k];
-      if (qp < 0.0) qk = qk + qp * (dpv) / (dk);
-      qc[k] = qk < 0.0 ? 0.0 : qk;
-      qp = qk;
-      dpv = dk;
-    }
-  }
+  if ((int)threadIdx.x < nc) shield::fill_column(qs + threadIdx.x * nk, ds + threadIdx.x * nk, nk, q_min);
   __syncthreads();
   for (int e = threadIdx.x; e < nc * nk; e += kBlock) {
     const int cl = e / nk, k = e - cl * nk;
