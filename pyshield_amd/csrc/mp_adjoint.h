@@ -246,6 +246,32 @@ __device__ __forceinline__ void scan_bwd(const Lay<S>& L, const double (&a)[S], 
     carry = readlane(x[s], 0);
   }
 }
+// Two of them with the same multipliers a (each output gets exactly the
+// operations of scan_bwd; the multipliers' composition is shared)
+template <int S>
+__device__ __forceinline__ void scan_bwd2(const Lay<S>& L, const double (&a)[S], const double (&b)[S],
+                                          const double (&c)[S], double (&x)[S], double (&y)[S]) {
+  const int lane = lane_id();
+  double carry_x = 0.0, carry_y = 0.0;
+#pragma unroll
+  for (int s = S - 1; s >= 0; --s) {
+    double A = a[s], B = b[s], C = c[s];
+    const int kb = L.kb[s];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double Ap = __shfl_down(A, d), Bp = __shfl_down(B, d), Cp = __shfl_down(C, d);
+      if (lane + d < 64 && kb >= d) {
+        B = A * Bp + B;
+        C = A * Cp + C;
+        A = A * Ap;
+      }
+    }
+    x[s] = (kb <= 63 - lane) ? B : A * carry_x + B;
+    y[s] = (kb <= 63 - lane) ? C : A * carry_y + C;
+    carry_x = readlane(x[s], 0);
+    carry_y = readlane(y[s], 0);
+  }
+}
 
 // cotangents of the eight cell values, in registers
 template <int S>
@@ -1914,14 +1940,12 @@ __global__ __launch_bounds__(64) void mp_adjoint_kernel(const AdjArgs args) {
         nb.ua[s] -= gu[s];
         nb.va[s] -= gv[s];
       }
-      scan_fwd(L, al, bu, u1);
-      scan_fwd(L, al, bv, v1);
+      scan_fwd2(L, al, bu, bv, u1, v1);
       col_next(sh.buf, al, an);
 #pragma unroll
       for (int s = 0; s < S; ++s)
         if (L.kb[s] == 0) an[s] = 0.0;
-      scan_bwd(L, an, gu, nu);
-      scan_bwd(L, an, gv, nv);
+      scan_bwd2(L, an, gu, gv, nu, nv);
       double u1p[S], v1p[S], mb[S], mbn[S];
       col_prev(sh.buf, u1, u1p);
       col_prev(sh.buf, v1, v1p);

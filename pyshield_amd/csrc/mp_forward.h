@@ -1099,8 +1099,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
         }
       }
     }
-    scan_fwd(L, a, bu, u1);
-    scan_fwd(L, a, bv, v1);
+    scan_fwd2(L, a, bu, bv, u1, v1);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       if (!L.v[s]) continue;

@@ -342,4 +342,41 @@ __device__ __forceinline__ void scan_fwd(const Lay<S>& L, const double (&a)[S], 
   }
 }
 
+// Two scans with the same maps' multipliers a (the momentum transport's u and
+// v): each output gets exactly the operations of scan_fwd; the multipliers'
+// composition and its DPP moves are shared.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step2(double& A, double& B, double& C, bool combine) {
+  const double Ap = dpp_move<CTRL, ROW_MASK>(A);
+  const double Bp = dpp_move<CTRL, ROW_MASK>(B);
+  const double Cp = dpp_move<CTRL, ROW_MASK>(C);
+  if (combine) {
+    B = A * Bp + B;
+    C = A * Cp + C;
+    A = A * Ap;
+  }
+}
+template <int S>
+__device__ __forceinline__ void scan_fwd2(const Lay<S>& L, const double (&a)[S], const double (&b)[S],
+                                          const double (&c)[S], double (&x)[S], double (&y)[S]) {
+  const int lane = lane_id();
+  const int in_row = lane & 15;
+  double carry_x = 0.0, carry_y = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double A = a[s], B = b[s], C = c[s];
+    const int k = L.k[s];
+    scan_step2<DPP_ROW_SHR + 1, 0xF>(A, B, C, in_row >= 1 && k >= 1);
+    scan_step2<DPP_ROW_SHR + 2, 0xF>(A, B, C, in_row >= 2 && k >= 2);
+    scan_step2<DPP_ROW_SHR + 4, 0xF>(A, B, C, in_row >= 4 && k >= 4);
+    scan_step2<DPP_ROW_SHR + 8, 0xF>(A, B, C, in_row >= 8 && k >= 8);
+    scan_step2<DPP_ROW_BCAST15, 0xA>(A, B, C, (lane & 16) && k >= in_row + 1);
+    scan_step2<DPP_ROW_BCAST31, 0xC>(A, B, C, lane >= 32 && k >= lane - 31);
+    x[s] = (k <= lane) ? B : A * carry_x + B;
+    y[s] = (k <= lane) ? C : A * carry_y + C;
+    carry_x = readlane(x[s], 63);
+    carry_y = readlane(y[s], 63);
+  }
+}
+
 }  // namespace shield
