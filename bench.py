@@ -400,6 +400,10 @@ def main():
                       "fwd_plus_adjoint_columns_per_s": ncol / ((k_ms + ams) * 1e-3),
                       "adjoint_algorithmic_bytes": adj_bytes,
                       "adjoint_roofline_frac": adj_bytes / (ams * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        fp64_ms = k_ms if args.mode == "fp64" else extra.get("fp64_kernel_ms")
+        if fp64_ms:
+            # the adjoint's cost in units of the FP64 forward of the same call
+            extra["adjoint_over_fp64_forward"] = ams / fp64_ms
         print(f"# adjoint (fp64): kernel {ams:.3f} ms", file=sys.stderr, flush=True)
 
     if not args.no_dycore and not args.only_kernel:
