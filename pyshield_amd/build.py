@@ -30,14 +30,17 @@ COMMON = [
 ]
 
 
-# The forward kernel (one file per slot count) is scheduled for instruction-
-# level parallelism: at three wavefronts per SIMD its dependent FP64 chains are
-# not all hidden by the other wavefronts, and the max-ilp strategy interleaves
-# independent work without more registers (166 VGPRs, no spills).  Measured on
-# the C384 state: mixed 10.24 -> 9.95 ms, FP64 17.06 -> 16.81 ms; the adjoint
-# and the update kernels do not gain from it.
+# The single-sub-step forward kernel (mp_forward_s<S>.hip) is scheduled for
+# instruction-level parallelism: at three wavefronts per SIMD its dependent
+# FP64 chains are not all hidden by the other wavefronts, and the max-ilp
+# strategy interleaves independent work without more registers (166 VGPRs, no
+# spills).  Measured on the C384 state: mixed 10.24 -> 9.95 ms, FP64 17.06 ->
+# 16.81 ms.  The looping
+# instances (mp_forward_multi_s<S>.hip) keep the default scheduler: with
+# max-ilp the two-sub-step FP64 kernel ran 3.05 ms against 2.61 ms at C96.
+# The adjoint and the update kernels do not gain from it.
 PER_FILE = {
-    "mp_forward": ["-mllvm", "--amdgpu-sched-strategy=max-ilp"],
+    "mp_forward_s": ["-mllvm", "--amdgpu-sched-strategy=max-ilp"],
 }
 
 

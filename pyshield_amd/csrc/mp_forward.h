@@ -1159,44 +1159,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(min_waves<S,
 }
 
 // Launch the kernel for one slot count: a grid of ceil(ncol / P) one-wave
-// workgroups.  u.delp != NULL selects the fused state update.
+// workgroups.  u.delp != NULL selects the fused state update.  The single-
+// sub-step and the looping (MULTI) instances are instantiated in files of
+// their own (mp_forward_s<S>.hip, mp_forward_multi_s<S>.hip), so each is
+// compiled with the flags that suit it.
+template <int S, class R, bool MULTI>
+hipError_t launch_kernels(const FwdArgs<R>& a, bool fused, unsigned nblocks, hipStream_t stream) {
+  const dim3 grid(nblocks), block(64);
+  if (fused)
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, MULTI, true>), grid, block, 0, stream, a);
+  else
+    hipLaunchKernelGGL((mp_forward_kernel<S, R, MULTI, false>), grid, block, 0, stream, a);
+  return hipGetLastError();
+}
 template <int S, class R>
 hipError_t launch_slots(const shield_mp_shape& sh, const shield_mp_params& p, const shield_mp_fields& f,
                         const shield_update_fields& u, int P, hipStream_t stream) {
   const int64_t ncol = sh.ni * sh.nj;
-  const dim3 grid((unsigned)((ncol + P - 1) / P)), block(64);
+  const unsigned nblocks = (unsigned)((ncol + P - 1) / P);
   const FwdArgs<R> a{sh, p, make_coef<R>(p), f, u, P};
-  const bool multi = p.ntimes > 1, fused = u.delp != nullptr;
-  if (multi && fused)
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, true>), grid, block, 0, stream, a);
-  else if (multi)
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, true, false>), grid, block, 0, stream, a);
-  else if (fused)
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, true>), grid, block, 0, stream, a);
-  else
-    hipLaunchKernelGGL((mp_forward_kernel<S, R, false, false>), grid, block, 0, stream, a);
-  return hipGetLastError();
+  const bool fused = u.delp != nullptr;
+  return p.ntimes > 1 ? launch_kernels<S, R, true>(a, fused, nblocks, stream)
+                      : launch_kernels<S, R, false>(a, fused, nblocks, stream);
 }
 
 // Resident wavefronts per SIMD of one kernel instance on the current device
 // (0 when the query fails, e.g. without a device).
-template <int S, class R>
-int waves_per_simd(bool multi, bool fused) {
+template <int S, class R, bool MULTI>
+int waves_kernels(bool fused) {
   int blocks = 0;
-  hipError_t e;
-  if (multi && fused)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, true, true>, 64, 0);
-  else if (multi)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, true, false>, 64, 0);
-  else if (fused)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, false, true>, 64, 0);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, false, false>, 64, 0);
+  const hipError_t e =
+      fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, MULTI, true>, 64, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mp_forward_kernel<S, R, MULTI, false>, 64, 0);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }
   return blocks / 4;  // one-wave workgroups over the 4 SIMDs of a CU
+}
+template <int S, class R>
+int waves_per_simd(bool multi, bool fused) {
+  return multi ? waves_kernels<S, R, true>(fused) : waves_kernels<S, R, false>(fused);
 }
 
 }  // namespace shield
